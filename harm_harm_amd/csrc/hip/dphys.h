@@ -1,0 +1,588 @@
+// dphys.h -- per-zone GRMHD physics as gfx950 device functions (FP64).
+//
+// Everything here runs in registers for one zone or one face state; the
+// kernels that call it own the memory layout.  Operation order follows the
+// reference expressions so that a build with -ffp-contract=off reproduces the
+// CPU results bit for bit.
+//
+// Reference (HARM): get_state phys.tools.c:1513, primtoflux :30,
+// mhd_calc_norestmass_ma :1071, mhd_calc_primfield_em :1133, vchar vchar.c:34,
+// simplefast :260, source_conn phys.c:132, Utoprim_jon utoprim_jon.c:136-4520.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <float.h>
+#include "harmstep.h"
+
+#define D_SMALL 1.e-300
+#define D_KINDASMALL 1.e-100
+#define D_VERYBIG 1.e150
+#define D_NUMEPSILON DBL_EPSILON
+#define D_SQRTMINNUMREPRESENT 1.4916681462400413e-154
+
+namespace hd {
+
+// Geometry of one grid point, unpacked from the 27-field record.
+struct Geom {
+  double gcov[10];  // packed symmetric
+  double gcon[10];
+  double gpert_tt, gdet, alpha, bsqoa2, beta1, beta2, beta3;
+};
+
+__host__ __device__ constexpr int sym(int a, int b)
+{
+  return (a <= b) ? ((a == 0) ? b : (a == 1) ? 3 + b : (a == 2) ? 5 + b : 9)
+                  : ((b == 0) ? a : (b == 1) ? 3 + a : (b == 2) ? 5 + a : 9);
+}
+
+// Load a record stored SoA with field stride `fs` starting at `base`.
+__device__ __forceinline__ void load_geom(const double* __restrict__ tab, long base, long fs, Geom& g)
+{
+#pragma unroll
+  for (int f = 0; f < 10; f++) g.gcov[f] = tab[base + f * fs];
+#pragma unroll
+  for (int f = 0; f < 10; f++) g.gcon[f] = tab[base + (10 + f) * fs];
+  g.gpert_tt = tab[base + HARM_G_GCOVPERT_TT * fs];
+  g.gdet = tab[base + HARM_G_GDET * fs];
+  g.alpha = tab[base + HARM_G_ALPHA * fs];
+  g.bsqoa2 = tab[base + HARM_G_BETASQOALPHASQ * fs];
+  g.beta1 = tab[base + HARM_G_BETA1 * fs];
+  g.beta2 = tab[base + HARM_G_BETA2 * fs];
+  g.beta3 = tab[base + HARM_G_BETA3 * fs];
+}
+
+__device__ __forceinline__ double gco(const Geom& g, int a, int b) { return g.gcov[sym(a, b)]; }
+__device__ __forceinline__ double gcn(const Geom& g, int a, int b) { return g.gcon[sym(a, b)]; }
+
+__device__ __forceinline__ void lower(const double* v, const Geom& g, double* w)
+{
+#pragma unroll
+  for (int a = 0; a < 4; a++) w[a] = gco(g, a, 0) * v[0] + gco(g, a, 1) * v[1] + gco(g, a, 2) * v[2] + gco(g, a, 3) * v[3];
+}
+__device__ __forceinline__ void raise(const double* v, const Geom& g, double* w)
+{
+#pragma unroll
+  for (int a = 0; a < 4; a++) w[a] = gcn(g, a, 0) * v[0] + gcn(g, a, 1) * v[1] + gcn(g, a, 2) * v[2] + gcn(g, a, 3) * v[3];
+}
+
+struct State {
+  double ucon[4], ucov[4];
+  double gamma, qsq, pressure, bsq, p1ud0;
+};
+
+// get_state for VELREL4 primitives (ucon_calc_rel4vel, compute_1plusud0_rel4vel,
+// bsq_calc_fromq_rel4vel, ideal-gas pressure).
+__device__ __forceinline__ void get_state(const double* pr, const Geom& g, double gam, State& q)
+{
+  const double u1 = pr[HARM_U1], u2 = pr[HARM_U2], u3 = pr[HARM_U3];
+  double qsq = gco(g, 1, 1) * u1 * u1 + gco(g, 2, 2) * u2 * u2 + gco(g, 3, 3) * u3 * u3
+               + 2. * (gco(g, 1, 2) * u1 * u2 + gco(g, 1, 3) * u1 * u3);
+  qsq = qsq * (qsq >= 0.0);
+  const double gamma = sqrt(1. + qsq);
+  q.qsq = qsq;
+  q.gamma = gamma;
+  q.ucon[0] = gamma / g.alpha;
+  q.ucon[1] = u1 - q.ucon[0] * g.beta1;
+  q.ucon[2] = u2 - q.ucon[0] * g.beta2;
+  q.ucon[3] = u3 - q.ucon[0] * g.beta3;
+  lower(q.ucon, g, q.ucov);
+  double ud0tilde = 0.0;
+  ud0tilde += u1 * gco(g, 0, 1);
+  ud0tilde += u2 * gco(g, 0, 2);
+  ud0tilde += u3 * gco(g, 0, 3);
+  const double alpha = g.alpha, alphasq = alpha * alpha;
+  q.p1ud0 = ud0tilde + (g.gpert_tt - alphasq * (g.bsqoa2 + qsq)) / (1.0 + gamma * alpha);
+  const double Bcon1 = pr[HARM_B1], Bcon2 = pr[HARM_B2], Bcon3 = pr[HARM_B3];
+  double Bcov[4];
+#pragma unroll
+  for (int jj = 0; jj < 4; jj++) {
+    Bcov[jj] = 0.0;
+    Bcov[jj] += Bcon1 * gco(g, jj, 1);
+    Bcov[jj] += Bcon2 * gco(g, jj, 2);
+    Bcov[jj] += Bcon3 * gco(g, jj, 3);
+  }
+  double Bsq = 0.0, udotB = 0.0;
+  Bsq += Bcon1 * Bcov[1]; Bsq += Bcon2 * Bcov[2]; Bsq += Bcon3 * Bcov[3];
+  udotB += Bcon1 * q.ucov[1]; udotB += Bcon2 * q.ucov[2]; udotB += Bcon3 * q.ucov[3];
+  double bsq = (Bsq + udotB * udotB);
+  bsq *= 1.0 / (q.ucon[0] * q.ucon[0]);
+  q.bsq = bsq;
+  q.pressure = (gam - 1.0) * pr[HARM_UU];
+}
+
+// One row T^dir_mu of the rest-mass-removed stress tensor: matter + EM parts
+// kept separate so callers can add them in the reference order.
+__device__ __forceinline__ void mhd_parts(const double* pr, int dir, const Geom& g, const State& q, double* ma, double* em)
+{
+  const double rho = pr[HARM_RHO], u = pr[HARM_UU], P = q.pressure;
+  const double w = P + rho + u;
+  const double ud = q.ucon[dir];
+  ma[0] = (P + u) * ud * q.ucov[0] + rho * ud * q.p1ud0;
+#pragma unroll
+  for (int j = 1; j < 4; j++) ma[j] = w * ud * q.ucov[j];
+  ma[dir] += P;
+  const double Bc[4] = {0.0, pr[HARM_B1], pr[HARM_B2], pr[HARM_B3]};
+  double Bcov[4];
+#pragma unroll
+  for (int jj = 0; jj < 4; jj++) {
+    Bcov[jj] = 0.0;
+    Bcov[jj] += Bc[1] * gco(g, jj, 1);
+    Bcov[jj] += Bc[2] * gco(g, jj, 2);
+    Bcov[jj] += Bc[3] * gco(g, jj, 3);
+  }
+  double Bsq = 0.0, udotB = 0.0;
+  Bsq += Bc[1] * Bcov[1]; Bsq += Bc[2] * Bcov[2]; Bsq += Bc[3] * Bcov[3];
+  udotB += Bc[1] * q.ucov[1]; udotB += Bc[2] * q.ucov[2]; udotB += Bc[3] * q.ucov[3];
+  const double Bd = Bc[dir];
+#pragma unroll
+  for (int mu = 0; mu < 4; mu++)
+    em[mu] = (Bsq * ud * (q.ucov[mu]) - Bd * Bcov[mu] - udotB * (Bd * (q.ucov[mu]) + Bcov[mu] * ud));
+  em[dir] += 0.5 * (Bsq + udotB * udotB);
+  const double ogs = 1.0 / ((q.ucon[0]) * (q.ucon[0]));
+#pragma unroll
+  for (int mu = 0; mu < 4; mu++) em[mu] *= ogs;
+}
+
+// primtoflux(UEVOLVE): flux (dir>0) or conserved state (dir=0), x gdet.
+__device__ __forceinline__ void primtoflux(const double* pr, const State& q, int dir, const Geom& g, double* F)
+{
+  double ma[4], em[4], fl[HARM_NPR];
+  fl[HARM_RHO] = pr[HARM_RHO] * q.ucon[dir];
+  mhd_parts(pr, dir, g, q, ma, em);
+#pragma unroll
+  for (int j = 0; j < 4; j++) fl[HARM_UU + j] = ma[j] + em[j];
+  if (dir > 0) {
+    const double Bd = pr[HARM_B1 + dir - 1];
+    fl[HARM_B1] = (pr[HARM_B1] * q.ucon[dir] - Bd * q.ucon[1]) / q.ucon[0];
+    fl[HARM_B2] = (pr[HARM_B2] * q.ucon[dir] - Bd * q.ucon[2]) / q.ucon[0];
+    fl[HARM_B3] = (pr[HARM_B3] * q.ucon[dir] - Bd * q.ucon[3]) / q.ucon[0];
+  } else {
+    fl[HARM_B1] = pr[HARM_B1];
+    fl[HARM_B2] = pr[HARM_B2];
+    fl[HARM_B3] = pr[HARM_B3];
+  }
+#pragma unroll
+  for (int pl = 0; pl < HARM_NPR; pl++) F[pl] = fl[pl] * g.gdet;
+}
+
+// vchar + simplefast: fast magnetosonic speeds along dir.
+__device__ __forceinline__ void vchar(const double* pr, const State& q, int dir, const Geom& g, double gam,
+                                      double& vmax, double& vmin)
+{
+  const double bsq = q.bsq;
+  double rho = pr[HARM_RHO], u = pr[HARM_UU];
+  if (u < 0) u = 0;
+  if (rho < 0) rho = D_SMALL;
+  const double P = q.pressure;
+  double WW = rho + u + P, cs2;
+  if (WW > 0.0) {
+    const double pressure = (gam - 1.0) * u;
+    const double h = rho + u + pressure;
+    cs2 = gam * pressure / h;
+  } else {
+    WW = D_SMALL;
+    cs2 = D_SMALL;
+  }
+  if (cs2 >= 1.0) cs2 = 1.0 - D_NUMEPSILON * 10.0;
+  else if (cs2 <= 0.0) cs2 = D_SMALL;
+  const double EF = D_SMALL + fabs(bsq + WW);
+  const double va2 = bsq / EF;
+  double cms2 = cs2 + va2 - cs2 * va2;
+  if (cms2 < 0.0) cms2 = 0.0;
+  if (cms2 > 1.0) cms2 = 1.0;
+  const double Asq = gcn(g, dir, dir);
+  const double Bsq = gcn(g, 0, 0);
+  const double Au = q.ucon[dir];
+  const double Bu = q.ucon[0];
+  const double AB = gcn(g, dir, 0);
+  const double Au2 = Au * Au, Bu2 = Bu * Bu, AuBu = Au * Bu;
+  const double B = 2. * (AuBu * (1.0 - cms2) - AB * cms2);
+  const double A = Bu2 * (1.0 - cms2) - Bsq * cms2;
+  double discr = 4.0 * cms2 * ((AB * AB - Asq * Bsq) * cms2 + (2.0 * AB * Au * Bu - Asq * Bu2 - Bsq * Au2) * (cms2 - 1.0));
+  if (discr < 0.0) discr = 0.0;
+  discr = sqrt(discr);
+  const double vm = -(-B + discr) / (2. * A);
+  const double vp = -(-B - discr) / (2. * A);
+  if (vp < vm) { vmax = vm; vmin = vp; }
+  else { vmax = vp; vmin = vm; }
+}
+
+// source_conn: dU^{UU+l} = gdet (sum_{j,k} T^j_k Gamma^k_{lj} - rho u^j Gamma^t_{lj}).
+// conn(c) returns packed Gamma^a_{bc} entry c = a*10 + sym(b,c).
+template <class ConnF>
+__device__ __forceinline__ void source(const double* pr, const Geom& g, const State& q, ConnF conn, double* dU)
+{
+  double mhd[4][4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    double ma[4], em[4];
+    mhd_parts(pr, j, g, q, ma, em);
+#pragma unroll
+    for (int k = 0; k < 4; k++) mhd[j][k] = (ma[k] + 0.0) + em[k];
+  }
+  double dUc[4];
+#pragma unroll
+  for (int l = 0; l < 4; l++) {
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+      for (int k = 0; k < 4; k++) s += mhd[j][k] * conn(k * 10 + sym(l, j));
+    dUc[l] = s;
+  }
+#pragma unroll
+  for (int l = 0; l < 4; l++) {
+    double s = dUc[l];
+#pragma unroll
+    for (int j = 0; j < 4; j++) s += -pr[HARM_RHO] * q.ucon[j] * conn(0 * 10 + sym(l, j));
+    dUc[l] = s;
+  }
+  dU[HARM_RHO] = 0.0 + g.gdet * 0.0;
+#pragma unroll
+  for (int l = 0; l < 4; l++) dU[HARM_UU + l] = 0.0 + g.gdet * dUc[l];
+  dU[HARM_B1] = 0.0 + g.gdet * 0.0;
+  dU[HARM_B2] = 0.0 + g.gdet * 0.0;
+  dU[HARM_B3] = 0.0 + g.gdet * 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// Inversion: Jon's 1-D W' Newton (WHICHHOTINVERTER 3), no line search.
+// ---------------------------------------------------------------------------
+struct Inv {
+  double Bsq, QdotB, QdotBsq, Qtsq, Qdotnp, D;
+  double w0, w1, w2;
+  double gam, gsq_too_big, utsq_too_big;
+};
+
+__device__ __forceinline__ double utsq_calc(double W, const Inv& s)
+{
+  const double S = s.QdotB, SoW = S / W;
+  const double top = s.Qtsq + (s.Bsq + 2.0 * W) * SoW * SoW;
+  const double bot = (s.Bsq * s.Bsq - s.Qtsq) + 2.0 * s.Bsq * W + W * W - SoW * SoW * (s.Bsq + 2.0 * W);
+  return top / bot;
+}
+
+__device__ __forceinline__ bool vsqgtr1(double W, const Inv& s)
+{
+  return (W * W * W * (W + 2. * s.Bsq) - s.QdotBsq * (2. * W + s.Bsq) <= W * W * (s.Qtsq - s.Bsq * s.Bsq));
+}
+
+__device__ __forceinline__ int coldgrhd(double Qtsq, double D, double& Wp)
+{
+  const double det = D * D + Qtsq;
+  if (det < 0.0) return 1;
+  if (D > 0.0) Wp = Qtsq / (D + sqrt(det));
+  else Wp = Qtsq / (D - sqrt(det));
+  return 0;
+}
+
+// Residual E'(W') and its derivative (func_Eprime_opt).
+__device__ __forceinline__ void func_eprime(double Wp, const Inv& s, double& dx, double& resid, double& jac, double& f,
+                                            double& df)
+{
+  const double D = s.D, Bsq = s.Bsq;
+  const double W = Wp + D;
+  const double X = Bsq + W + D_SMALL;
+  const double X2 = X * X, X3 = X2 * X;
+  const double utsq = utsq_calc(W, s);
+  const double gammasq = 1.0 + utsq;
+  const double gamma = sqrt(gammasq);
+  const double wmrho0 = (Wp - D * utsq / (1.0 + gamma)) / gammasq;
+  if (utsq < -0.9 * s.utsq_too_big || utsq != utsq) {
+    const double Ep = -D_VERYBIG;
+    resid = Ep;
+    jac = 0.0;
+    dx = D_VERYBIG;
+    f = 0.5 * (Ep * Ep);
+    df = -2. * f;
+    return;
+  }
+  const double igr = (s.gam - 1.0) / s.gam;
+  const double pofchi = igr * wmrho0, dpdrho = 0.0, dpdchi = (s.gam - 1.0) / s.gam;
+  double Ep = s.Qdotnp + Wp - pofchi + 0.5 * Bsq + 0.5 * (Bsq * s.Qtsq - s.QdotBsq) / X2;
+  if (pofchi != pofchi || Ep != Ep) Ep = -D_VERYBIG;
+  const double Xd = Bsq + W, Wsq = W * W, W3 = Wsq * W, Xd3 = Xd * Xd * Xd;
+  const double dvsq = -2.0 / Xd3 * (s.Qtsq + s.QdotBsq * (3.0 * W * Xd + Bsq * Bsq) / W3);
+  const double dwmrho0dW = 1.0 / gammasq;
+  const double drho0dW = 0.0;
+  const double dwmrho0dvsq = (D * (gamma * 0.5 - 1.0) - Wp);
+  const double drho0dvsq = -D * gamma * 0.5;
+  const double dpdW = drho0dW * dpdrho + dwmrho0dW * dpdchi;
+  const double dpdvsq = drho0dvsq * dpdrho + dwmrho0dvsq * dpdchi;
+  const double dpdWp = dpdW + dpdvsq * dvsq;
+  double dE = (1.0 - dpdWp) - (Bsq * s.Qtsq - s.QdotBsq) / X3;
+  if (dvsq != dvsq || dpdW != dpdW || dpdvsq != dpdvsq || dpdWp != dpdWp || dE != dE) dE = 0.0;
+  resid = Ep;
+  jac = dE;
+  dx = -Ep / dE;
+  f = 0.5 * (Ep * Ep);
+  df = -2. * f;
+}
+
+__device__ __forceinline__ bool ultrarel(double x0, const Inv& s) { return (fabs(s.w1) > s.w2) || (fabs(x0) > s.w2); }
+__device__ __forceinline__ bool errcheck(double errx, double x0, const Inv& s)
+{
+  return ultrarel(x0, s) ? (fabs(errx) <= D_NUMEPSILON) : (fabs(errx) <= 1E3 * D_NUMEPSILON);
+}
+__device__ __forceinline__ int extracheck(double x0, const Inv& s) { return ultrarel(x0, s) ? 4 : 2; }
+
+__device__ __forceinline__ void validate_wp(double Wpold, double& Wpnew, const Inv& s)
+{
+  if (Wpnew < -s.D && s.D > 0.0) {
+    double Wp = 0.0;
+    coldgrhd(s.Qtsq, s.D, Wp);
+    Wpnew = Wp * 10.0;
+  }
+  Wpnew = (fabs(Wpnew / s.w1) > s.gsq_too_big) ? 0.5 * (Wpold + Wpnew) : Wpnew;
+}
+
+// general_newton_raphson for n=1 without line search.  Returns retval.
+__device__ __forceinline__ int newton(int& pflag, double& xio, const Inv& s, int& nstroke)
+{
+  double x = xio, x_old = x, x_older = x, dx = 0.0, dx_old = D_VERYBIG, resid, jac, f = 1., df = 1.;
+  double errx = 1., damp = 1.0;
+  int n_iter = 0, i_extra = 0, lntries = 0;
+  bool doing_extra = false, keep = true, diddamp = false;
+  while (keep) {
+    nstroke++;
+    lntries++;
+    if (lntries > 10 && !diddamp) damp = 0.5;
+    func_eprime(x, s, dx, resid, jac, f, df);
+    x_older = x_old;
+    x_old = x;
+    diddamp = false;
+    if (n_iter == 0) {
+      x_older = x;
+      x_old = x;
+      dx_old = dx;
+    } else {
+      if (resid <= -D_VERYBIG || jac == 0.0) {
+        damp = 0.5 * damp;
+        x = x_older;
+        dx = dx_old;
+        diddamp = true;
+        func_eprime(x, s, dx, resid, jac, f, df);
+        if (resid <= -D_VERYBIG || jac == 0.0) { xio = x; return 1; }
+        x_older = x;
+        x_old = x;
+        dx_old = dx;
+      }
+    }
+    x += damp * dx;
+    errx = (s.w0 == 0.) ? fabs(dx_old) : fabs(dx_old / fmax(fabs(s.w0), fabs(x)));
+    validate_wp(x_old, x, s);
+    dx_old = dx;
+    const bool ec = errcheck(errx, x, s);
+    const int xc = extracheck(x, s);
+    if (ec && !doing_extra && xc > 0) doing_extra = true;
+    if (doing_extra) i_extra++;
+    if ((ec && !doing_extra) || (i_extra > xc) || (n_iter >= 199)) keep = false;
+    n_iter++;
+  }
+  if (!isfinite(x) || !isfinite(f) || !isfinite(df)) {
+    pflag = 8;
+    xio = x;
+    return 1;
+  }
+  x = x_old;
+  xio = x;
+  if (errcheck(errx, x, s)) return 0;
+  if (fabs(errx) > 1E-6) {
+    pflag = 54;
+    return 1;
+  }
+  return 0;
+}
+
+// Utoprim_new_body on alpha-scaled U (Ut) and guess (pt).  If wp_seed is
+// finite it replaces the W' guess (MLP-seeded form); Newton stays the judge.
+__device__ __forceinline__ int new_body(int& pflag, const double* Ut, const Geom& g, double gam, double gammafail,
+                                        double* prim, int& nstroke, double wp_seed)
+{
+  Inv s;
+  pflag = 0;
+  s.gam = gam;
+  s.gsq_too_big = gammafail * gammafail;
+  s.utsq_too_big = (gammafail - 1.0) * (gammafail - 1.0);
+  s.D = Ut[HARM_RHO];
+  prim[HARM_B1] = Ut[HARM_B1];
+  prim[HARM_B2] = Ut[HARM_B2];
+  prim[HARM_B3] = Ut[HARM_B3];
+  const double Bcon[4] = {0., Ut[HARM_B1], Ut[HARM_B2], Ut[HARM_B3]};
+  double Bcov[4];
+  lower(Bcon, g, Bcov);
+  s.Bsq = 0.;
+  s.Bsq += Bcon[1] * Bcov[1]; s.Bsq += Bcon[2] * Bcov[2]; s.Bsq += Bcon[3] * Bcov[3];
+  const double ncov[4] = {-g.alpha, 0., 0., 0.};
+  double ncon[4];
+  raise(ncov, g, ncon);
+  const double Qcovp[4] = {Ut[HARM_UU], Ut[HARM_U1], Ut[HARM_U2], Ut[HARM_U3]};
+  double Qconp[4];
+  raise(Qcovp, g, Qconp);
+  double Qtcon[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int j = 1; j < 4; j++)
+#pragma unroll
+    for (int i = 1; i < 4; i++) Qtcon[j] += Qcovp[i] * (gcn(g, i, j) + ncon[i] * ncon[j]);
+  double Qtcov[4];
+  lower(Qtcon, g, Qtcov);
+  s.Qtsq = 0.;
+#pragma unroll
+  for (int i = 0; i < 4; i++) s.Qtsq += Qtcov[i] * Qtcon[i];
+  const double Qcov[4] = {Qcovp[0] - s.D, Qcovp[1], Qcovp[2], Qcovp[3]};
+  double Qcon[4];
+  raise(Qcov, g, Qcon);
+  s.QdotB = 0.;
+  s.QdotB += Qcov[1] * Bcon[1]; s.QdotB += Qcov[2] * Bcon[2]; s.QdotB += Qcov[3] * Bcon[3];
+  s.QdotBsq = s.QdotB * s.QdotB;
+  {
+    const double alpha = g.alpha, alphasq = alpha * alpha;
+    const double Dfactor = (-g.gpert_tt + alphasq * (g.bsqoa2)) / (alphasq + alpha);
+    s.Qdotnp = Qconp[0] * ncov[0] + (s.D) * (Dfactor);
+  }
+
+  // set_guess_Wp + verify_Wlast
+  double Wp_last, W_last;
+  {
+    double utsq = 0.;
+#pragma unroll
+    for (int i = 1; i < 4; i++)
+#pragma unroll
+      for (int j = 1; j < 4; j++) utsq += gco(g, i, j) * prim[HARM_U1 + i - 1] * prim[HARM_U1 + j - 1];
+    if ((utsq < 0.) && (fabs(utsq) < 0.9 * s.utsq_too_big)) utsq = 0.0;
+    if (!(utsq >= 0. && utsq < s.utsq_too_big)) utsq = 0.0;
+    const double gammasq = 1. + utsq;
+    const double gamma = sqrt(gammasq);
+    const double rho0 = (s.D) / gamma;
+    const double u = prim[HARM_UU];
+    const double p = (gam - 1.0) * u;
+    const double BB[4] = {0.0, prim[HARM_B1], prim[HARM_B2], prim[HARM_B3]};
+    double BBc[4];
+    lower(BB, g, BBc);
+    double bsq = 0.0;
+    bsq += BB[1] * BBc[1]; bsq += BB[2] * BBc[2]; bsq += BB[3] * BBc[3];
+    bsq *= ((g.alpha * g.alpha)) / gammasq;
+    Wp_last = (s.D) * utsq / (1.0 + gamma) + (u + p) * gammasq;
+    W_last = Wp_last + (s.D);
+    const double Wpabs = fabs((s.D) * utsq) / (1.0 + fabs(gamma)) + (fabs(u) + fabs(p)) * fabs(gammasq) + D_SQRTMINNUMREPRESENT;
+    const double nuabs = fabs(u) + fabs(p) + fabs(bsq);
+    const double etaabs = fabs(rho0) + nuabs;
+    s.w2 = 1e2 * fabs(rho0);
+    if (gammasq * etaabs >= (s.w2)) {
+      double Wpcoldhd = 0.0;
+      if (coldgrhd(s.Qtsq, s.D, Wpcoldhd)) { pflag = 3; Wpcoldhd = Wp_last; }
+      s.w0 = fmax(fmax(Wpcoldhd, Wp_last), Wpabs);
+      Wp_last = fmax(2.0 * Wpcoldhd, Wp_last);
+    } else {
+      s.w0 = Wpabs;
+    }
+    s.w1 = fabs(rho0) + s.w0 + D_SQRTMINNUMREPRESENT;
+    if (isfinite(wp_seed) && wp_seed > 0.0) {
+      Wp_last = wp_seed;
+      W_last = Wp_last + s.D;
+    }
+    if (W_last < -(s.D) || Wp_last < 0) { W_last = -(s.D); Wp_last = 0; }
+    int i_increase = 0;
+    while (vsqgtr1(W_last, s) && (i_increase < 10)) {
+      Wp_last *= 10.;
+      W_last = Wp_last + (s.D);
+      i_increase++;
+    }
+    int nattempt = 0;
+    while (true) {
+      const double ut2 = utsq_calc(W_last, s);
+      if (ut2 >= 0.0 && ut2 == ut2) break;
+      double coldWp = 0.0;
+      coldgrhd(s.Qtsq, s.D, coldWp);
+      if (Wp_last < coldWp) Wp_last = coldWp;
+      if (nattempt > 0) Wp_last = fmax(fmax(fabs(Wp_last) * 10.0, D_NUMEPSILON * 100.0 * fabs(s.D)), D_KINDASMALL);
+      W_last = Wp_last + (s.D);
+      if (nattempt > 1000) { pflag = 4; return 1; }
+      nattempt++;
+    }
+  }
+
+  double Wp = Wp_last;
+  const int rv = newton(pflag, Wp, s, nstroke);
+  if (rv != 0) return rv;
+
+  if (Wp == 1.e100) pflag = 51;
+  else if (Wp <= -(s.D)) pflag = 51;
+  else if (Wp / s.w1 > s.gsq_too_big) pflag = 2;
+
+  // Wp2prim
+  double pr0[HARM_NPR];
+  prim[HARM_B1] = Ut[HARM_B1];
+  prim[HARM_B2] = Ut[HARM_B2];
+  prim[HARM_B3] = Ut[HARM_B3];
+#pragma unroll
+  for (int pl = 0; pl < HARM_NPR; pl++) pr0[pl] = prim[pl];
+  const double W = Wp + s.D;
+  const double utsq = utsq_calc(W, s);
+  const double gammasq = 1.0 + utsq;
+  const double gamma = sqrt(gammasq);
+  const double wmrho0 = (Wp - s.D * utsq / (1.0 + gamma)) / gammasq;
+  const double rho0 = s.D / gamma;
+  const double p = ((gam - 1.0) / gam) * wmrho0;
+  const double u = wmrho0 - p;
+  if ((rho0 <= 0.) || (p < 0.)) {
+    if ((rho0 <= 0.) && (p >= 0.)) pflag = 9;
+    if ((rho0 > 0.) && (p < 0.)) pflag = 10;
+    if ((rho0 <= 0.) && (p < 0.)) pflag = 11;
+  }
+  prim[HARM_RHO] = rho0;
+  prim[HARM_UU] = u;
+#pragma unroll
+  for (int i = 1; i < 4; i++) prim[HARM_U1 + i - 1] = gamma / (W + s.Bsq) * (Qtcon[i] + s.QdotB * Bcon[i] / W);
+  bool foundnan = false;
+#pragma unroll
+  for (int pl = 0; pl < HARM_NPR; pl++)
+    if (!isfinite(prim[pl])) foundnan = true;
+  if (foundnan) {
+    pflag = 8;
+#pragma unroll
+    for (int pl = 0; pl < HARM_NPR; pl++) prim[pl] = pr0[pl];
+    return 1;
+  }
+  return 0;
+}
+
+// Utoprim_jon_nonrelcompat_inputnorestmass on geometry-free Ugf.
+__device__ __forceinline__ int jon(const double* Ugf, const Geom& g, double gam, double gammafail, double* pr,
+                                   int& nstroke, double wp_seed)
+{
+  double Ut[HARM_NPR], pt[HARM_NPR];
+  pr[HARM_B1] = Ugf[HARM_B1];
+  pr[HARM_B2] = Ugf[HARM_B2];
+  pr[HARM_B3] = Ugf[HARM_B3];
+  const double alpha = g.alpha;
+#pragma unroll
+  for (int i = 0; i < HARM_NPR; i++) Ut[i] = alpha * Ugf[i];
+#pragma unroll
+  for (int i = 0; i < HARM_NPR; i++) pt[i] = pr[i];
+  pt[HARM_B1] = alpha * pr[HARM_B1];
+  pt[HARM_B2] = alpha * pr[HARM_B2];
+  pt[HARM_B3] = alpha * pr[HARM_B3];
+  int pflag = 0;
+  const int ret = new_body(pflag, Ut, g, gam, gammafail, pt, nstroke, wp_seed);
+  if (ret == 0) {
+#pragma unroll
+    for (int i = 0; i <= HARM_U3; i++) pr[i] = pt[i];
+  } else if (pflag <= 0) {
+    pflag = ret + 51;
+  }
+  return pflag;
+}
+
+// Utoprimgen(UEVOLVE): strip gdet then invert.
+__device__ __forceinline__ int utoprimgen(const double* U, const Geom& g, double gam, double gammafail, double* pr,
+                                          int& nstroke, double wp_seed)
+{
+  double Ugf[HARM_NPR];
+  const double ig = (g.gdet >= 0.0 ? 1.0 : -1.0) / (fabs(g.gdet) + D_SMALL);
+#pragma unroll
+  for (int pl = 0; pl < HARM_NPR; pl++) Ugf[pl] = U[pl] * ig;
+  return jon(Ugf, g, gam, gammafail, pr, nstroke, wp_seed);
+}
+
+}  // namespace hd

@@ -1,0 +1,237 @@
+/*
+ * harmstep.h -- C ABI of the MI355X-native HARM advance() step.
+ *
+ * One handle owns one grid resident in HBM (structure-of-arrays, i fastest,
+ * ghost width ng on every dimension with more than one zone).  The host passes
+ * HARM-layout arrays (array-of-structs, p[k][j][i][NPR] over the full ghosted
+ * box) only through the set/get calls; every compute entry point works on
+ * device-resident state and launches hand-written gfx950 FP64 kernels on the
+ * handle's stream.  No entry point exits the process: each returns a
+ * harm_status, and per-zone inversion failures land in an int32 pflag array.
+ *
+ * Every compute entry point names the reference function it replaces
+ * (HARM, pseudotensor/harm_harm; file:line of the reference tree).
+ */
+#ifndef HARMSTEP_H
+#define HARMSTEP_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- primitive / conserved indices (HARM's NPR=8 ordering, definit.h) ---- */
+enum { HARM_RHO = 0, HARM_UU = 1, HARM_U1 = 2, HARM_U2 = 3, HARM_U3 = 4,
+       HARM_B1 = 5, HARM_B2 = 6, HARM_B3 = 7, HARM_NPR = 8 };
+
+/* ---- status codes ---- */
+typedef enum {
+  HARM_OK = 0,
+  HARM_EINVAL = 1,     /* bad argument / size / configuration            */
+  HARM_EHIP = 2,       /* a HIP runtime call failed                      */
+  HARM_ENOMEM = 3,     /* device allocation failed                       */
+  HARM_ENOGPU = 4,     /* no usable gfx950 device                        */
+  HARM_EFAIL = 5       /* numerical failure that HARM would treat as fatal */
+} harm_status;
+
+/* ---- per-zone inversion flags (global.nondepnmemonics.h:824-844) ---- */
+enum {
+  HARM_UTOPRIMNOFAIL = 0,
+  HARM_UTOPRIMFAILCONV = 1,
+  HARM_UTOPRIMFAILCONVW = 2,
+  HARM_UTOPRIMFAILCONVUTSQ = 3,
+  HARM_UTOPRIMFAILCONVGUESSUTSQ = 4,
+  HARM_UTOPRIMFAILNANRESULT = 8,
+  HARM_UTOPRIMFAILRHONEG = 9,
+  HARM_UTOPRIMFAILUNEG = 10,
+  HARM_UTOPRIMFAILRHOUNEG = 11,
+  HARM_UTOPRIMFAILCONVRET = 50
+};
+
+/* ---- method selectors (HARM's runtime globals lim[], fluxmethod, TIMEORDER) */
+enum { HARM_RECON_DONOR = 0, HARM_RECON_MC = 1, HARM_RECON_PARA = 2, HARM_RECON_WENO5 = 3 };
+enum { HARM_FLUX_LAXF = 0, HARM_FLUX_HLL = 1 };
+
+/* ---- per-point geometry record (one of the NPG grid locations) ----------
+ * gcov/gcon are packed symmetric (00,01,02,03,11,12,13,22,23,33); the other
+ * fields are HARM's of_geom extras (global.structs.h:85-144).              */
+enum {
+  HARM_G_GCOV = 0, HARM_G_GCON = 10, HARM_G_GCOVPERT_TT = 20, HARM_G_GDET = 21,
+  HARM_G_ALPHA = 22, HARM_G_BETASQOALPHASQ = 23, HARM_G_BETA1 = 24,
+  HARM_G_BETA2 = 25, HARM_G_BETA3 = 26, HARM_G_NFIELD = 27
+};
+/* connection Gamma^a_{bc} at CENT, packed a*10 + sym(b,c): 40 values */
+enum { HARM_NCONN = 40 };
+/* grid locations with geometry tables */
+enum { HARM_LOC_CENT = 0, HARM_LOC_FACE1 = 1, HARM_LOC_FACE2 = 2, HARM_LOC_FACE3 = 3, HARM_NLOC = 4 };
+
+/* ---- runtime configuration (HARM's init_global/init_grid globals) ---- */
+typedef struct harm_config {
+  int32_t n1, n2, n3;       /* active zones (N1,N2,N3)                        */
+  int32_t ng;               /* ghost width (MAXBND); 4 for PARA and WENO5     */
+  int32_t recon;            /* HARM_RECON_*  (lim[1..3])                      */
+  int32_t flux;             /* HARM_FLUX_*   (fluxmethod)                     */
+  int32_t timeorder;        /* 2 (RK2 midpoint) or 3 (RK3 TVD)               */
+  int32_t pad0;
+  double a;                 /* Kerr spin                                      */
+  double gam;               /* ideal-gas adiabatic index                      */
+  double R0, Rin, Rout;     /* MKS radial map r = R0 + exp(X1)                */
+  double hslope;            /* MKS theta concentration                        */
+  double cour;              /* Courant factor                                 */
+  double safe;              /* dt growth cap (SAFE)                           */
+  double rhomin, uumin;     /* floors rho=rhomin r^-1.5, u=uumin r^-2.5 (rescaletype 0) */
+  double gammamax;          /* limit_gamma ceiling (GAMMAMAX)                 */
+  double gammafail;         /* inversion failure Lorentz factor (GAMMAFAIL)   */
+} harm_config;
+
+typedef struct harm_handle harm_handle;
+
+/* Newton statistics of one inversion call (of_newtonstats nstroke, summed). */
+typedef struct harm_newton_stats {
+  int64_t nstroke;          /* total Newton iterations over all zones         */
+  int64_t nfail;            /* zones with pflag > 0 after the call            */
+} harm_newton_stats;
+
+/* ------------------------------------------------------------------------ */
+/* Library / handle lifetime                                                 */
+/* ------------------------------------------------------------------------ */
+
+/* Fills cfg with HARM's torus defaults for an n1 x n2 x n3 grid
+ * (init.fishmon.c:194-290, init.tools.c:89-263, rescaletype 0 floors). */
+void harm_default_config(harm_config* cfg, int32_t n1, int32_t n2, int32_t n3);
+
+/* Replaces init()'s grid/geometry part (initbase.c:7 -> set_grid.c:114):
+ * builds MKS Kerr-Schild geometry and connection tables on the host and
+ * uploads the 2-D (r,theta) tables; allocates all device state. */
+harm_status harm_create(const harm_config* cfg, harm_handle** out);
+harm_status harm_destroy(harm_handle* h);
+harm_status harm_sync(harm_handle* h);
+const char* harm_last_error(void);
+
+/* Sizes of the full ghosted storage box (NSTORE1..3). */
+harm_status harm_storage_dims(const harm_handle* h, int32_t dims[3]);
+
+/* Host copies of geometry: loc in HARM_LOC_*, out[(j*ns1+i)*HARM_G_NFIELD + f]
+ * over the 2-D storage plane (ns1 x ns2); conn out[(j*ns1+i)*40 + c]. */
+harm_status harm_get_geometry(harm_handle* h, int32_t loc, double* out);
+harm_status harm_get_connection(harm_handle* h, double* out);
+/* MKS coordinates (X1..X3) and BL radius/theta of a zone centre. */
+harm_status harm_get_coords(harm_handle* h, int32_t i, int32_t j, int32_t k, double X[4], double V[4]);
+
+/* ------------------------------------------------------------------------ */
+/* State exchange in HARM layout: a[((k*ns2 + j)*ns1 + i)*NPR + pl]           */
+/* ------------------------------------------------------------------------ */
+enum { HARM_ARR_P = 0,     /* pglobal (primitives)                           */
+       HARM_ARR_PK0 = 1,   /* pk[0] RK temporary primitives                  */
+       HARM_ARR_PK1 = 2,   /* pk[1]                                          */
+       HARM_ARR_UI = 3,    /* uinitial                                       */
+       HARM_ARR_UF = 4,    /* ulast                                          */
+       HARM_ARR_UCUM = 5,  /* unewglobal                                     */
+       HARM_ARR_F1 = 6, HARM_ARR_F2 = 7, HARM_ARR_F3 = 8, /* face fluxes     */
+       HARM_NARR = 9 };
+harm_status harm_set_array(harm_handle* h, int32_t which, const double* host_aos);
+harm_status harm_get_array(harm_handle* h, int32_t which, double* host_aos);
+/* pflag[k][j][i] (FLAGUTOPRIMFAIL) over the storage box */
+harm_status harm_get_pflag(harm_handle* h, int32_t* host_flags);
+harm_status harm_set_time(harm_handle* h, double t, double dt, int64_t nstep);
+harm_status harm_get_time(harm_handle* h, double* t, double* dt, int64_t* nstep);
+
+/* ------------------------------------------------------------------------ */
+/* Problem setup (host init, then upload)                                    */
+/* ------------------------------------------------------------------------ */
+/* FM torus + atmosphere + A_phi field normalised to beta, u perturbed by
+ * +-randfact/2 from HARM's ranc(seed) stream (init.fishmon.c:403-750,
+ * init.tools.c:295-894).  Leaves p bounded and sets dt from set_dt(). */
+harm_status harm_init_torus(harm_handle* h, double rin, double rmax, double kappa,
+                            double beta, double randfact, uint32_t seed);
+
+/* ------------------------------------------------------------------------ */
+/* Hot path                                                                  */
+/* ------------------------------------------------------------------------ */
+
+/* Replaces fluxcalc() (flux.c:38 -> fluxcalc_flux :658 -> fluxcalc_standard
+ * :852): reconstruct arr `which_p`, face Riemann fluxes into F1..F3 and
+ * ndt[3] = min over faces of cour*dx/ctop; then flux_ct() (fluxct.c:186,
+ * Toth) when do_ct != 0. */
+harm_status harm_fluxcalc(harm_handle* h, int32_t which_p, int32_t do_ct, double ndt[3]);
+
+/* Replaces the per-zone update loop of advance_standard (advance.c:420-564):
+ * Ui = U(pi); dU = flux2dUavg + source(pb); Uf = UFSET; ucum += UCUMUPDATE;
+ * accdt = source dt limit (dUtodt -> compute_dt_fromsource). */
+harm_status harm_update(harm_handle* h, int32_t which_pi, int32_t which_pb,
+                        const double CUf[4], const double Cunew[4], double dt,
+                        int32_t timeorder, double* accdt);
+
+/* Replaces advance.c:654-808: pf = pb (guess), invert (finalstep ? ucum : uf)
+ * with Utoprimgen (utoprimgen.c:32 -> utoprim_jon.c:136), then fixup1zone
+ * (fixup.c:561) on the final substep. */
+harm_status harm_invert_grid(harm_handle* h, int32_t which_pb, int32_t which_pf,
+                             int32_t finalstep, harm_newton_stats* stats);
+
+/* Replaces bound_prim_user_dir (initboundcode/bounds.fishmon.c:15) for arr
+ * `which`: x1 outflow (inner inflow check), x2 polar axis, x3 periodic. */
+harm_status harm_bound(harm_handle* h, int32_t which);
+
+/* Replaces post_fixup (fixup.c:53 -> fixup_utoprim :1061): failed zones get
+ * the average of good neighbours. */
+harm_status harm_post_fixup(harm_handle* h, int32_t which);
+
+/* Replaces advance() (advance.c:85) for one RK substep. */
+harm_status harm_advance(harm_handle* h, int32_t timeorder, double* ndt);
+
+/* Replaces step_ch_full() (step_ch.c:35 -> step_ch_simplempi :524 ->
+ * post_stepch): one full RK step with bounds, fixups and the dt update. */
+harm_status harm_step(harm_handle* h, int32_t nsteps);
+
+/* Accretion-rate diagnostic (diag_flux_pureflux, diag.c:1020): F1[RHO] and
+ * F1[UU] summed over the inner and outer radial boundary faces times
+ * dx2*dx3*dt, accumulated since the last reset. out[4] = {mdot_in, edot_in,
+ * mdot_out, edot_out}. */
+harm_status harm_get_boundary_fluxes(harm_handle* h, double out[4]);
+
+/* ------------------------------------------------------------------------ */
+/* Device memory, streams and timing (so callers need no other HIP runtime)  */
+/* ------------------------------------------------------------------------ */
+harm_status harm_device_count(int32_t* n);
+harm_status harm_dev_malloc(void** ptr, size_t bytes);
+harm_status harm_dev_free(void* ptr);
+harm_status harm_memcpy_htod(void* dst, const void* src, size_t bytes);
+harm_status harm_memcpy_dtoh(void* dst, const void* src, size_t bytes);
+harm_status harm_memcpy_dtod(void* dst, const void* src, size_t bytes);
+harm_status harm_device_synchronize(void);
+/* the handle's stream (kernels of h run there) */
+harm_status harm_get_stream(harm_handle* h, void** stream);
+harm_status harm_event_create(void** ev);
+harm_status harm_event_destroy(void* ev);
+harm_status harm_event_record(void* ev, void* stream);
+harm_status harm_event_elapsed_ms(void* ev0, void* ev1, float* ms);
+
+/* ------------------------------------------------------------------------ */
+/* Batched inversion on caller-provided device buffers (config-3 microbench) */
+/* ------------------------------------------------------------------------ */
+/* Replaces the per-zone Utoprimgen call (utoprimgen.c:32) for n independent
+ * zones.  Device pointers, SoA: U[pl*n + z] (UEVOLVE, gdet-weighted),
+ * geom[f*n + z] (HARM_G_NFIELD fields), p_inout[pl*n + z] (guess in,
+ * solution out), pflag[z], niter[z] (may be NULL).  stream may be NULL. */
+harm_status harm_invert_batch(const double* U, const double* geom, double* p_inout,
+                              int32_t* pflag, int32_t* niter, int64_t n, double gam,
+                              double gammafail, void* stream);
+
+/* Replaces primtoU(UEVOLVE, ...) (phys.tools.c:475) over n independent zones:
+ * device SoA p[pl*n+z], geom[f*n+z] -> U[pl*n+z]. */
+harm_status harm_prim_to_cons_batch(const double* p, const double* geom, double* U, int64_t n,
+                                    double gam, void* stream);
+
+/* Host geometry records (HARM_G_NFIELD doubles): Kerr-Schild coordinates at
+ * (a, r, th) (set_gcov_ksmetric, metric.c:1301), or MKS at X for cfg. */
+void harm_geom_ks_record(double a, double r, double th, double* rec);
+void harm_geom_mks_record(const harm_config* cfg, const double X[4], double* rec);
+/* n Kerr-Schild records into host SoA rec[f*n + z] (host threads). */
+void harm_geom_ks_batch(int64_t n, const double* a, const double* r, const double* th, double* rec);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HARMSTEP_H */

@@ -1,0 +1,67 @@
+/*
+ * oracle.h -- CPU restatement of HARM's advance() hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This code is the checker the HIP path is compared
+ * against (and, timed, the CPU baseline); nothing in the shipped library calls
+ * it.  Every function cites the reference (pseudotensor/harm_harm) file:line
+ * whose algorithm it restates.  Arrays are HARM layout: a[((k*ns2+j)*ns1+i)*8+pl]
+ * over the ghosted storage box; geometry is the library's per-point record
+ * (27 doubles) on the 2-D (i,j) storage plane.
+ */
+#ifndef HARM_ORACLE_H
+#define HARM_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define O_NPR 8
+enum { O_RHO = 0, O_UU = 1, O_U1 = 2, O_U2 = 3, O_U3 = 4, O_B1 = 5, O_B2 = 6, O_B3 = 7 };
+
+typedef struct o_geom {
+  double gcov[4][4], gcon[4][4];
+  double gpert_tt, gdet, alpha, bsqoa2, beta[4];
+} o_geom;
+
+typedef struct o_state {
+  double ucon[4], ucov[4];
+  double gamma, qsq, pressure, bsq, p1ud0;
+} o_state;
+
+/* physical parameters the per-zone kernels need */
+typedef struct o_params {
+  double gam;        /* adiabatic index                 */
+  double gammafail;  /* GAMMAFAIL (inversion)           */
+  double gammamax;   /* GAMMAMAX (limit_gamma)          */
+} o_params;
+
+void o_geom_from_rec(const double* rec, o_geom* g);
+
+/* phys.tools.c:1513 get_state with VELREL4, UD0PLUS1FIX, rel4vel bsq */
+void o_get_state(const double* pr, const o_geom* g, double gam, o_state* q);
+/* phys.tools.c:30 primtoflux(UEVOLVE, ...) -> gdet * flux */
+void o_primtoflux(const double* pr, const o_state* q, int dir, const o_geom* g, double* F);
+/* phys.tools.c:1047 mhd_calc_norestmass (MA + primitive-field EM) */
+void o_mhd_calc(const double* pr, int dir, const o_geom* g, const o_state* q, double* mhd);
+/* vchar.c:34 vchar + simplefast :260 */
+void o_vchar(const double* pr, const o_state* q, int dir, const o_geom* g, double gam, double* vmax, double* vmin);
+/* phys.tools.c:1294 source -> phys.c:132 source_conn, conn packed a*10+sym(b,c) */
+void o_source(const double* pr, const o_geom* g, const o_state* q, const double* conn, double* dU);
+
+/* utoprimgen.c:32 Utoprimgen(UEVOLVE) -> utoprim_jon.c:136; returns pflag,
+ * *nstroke gets the Newton iterations (of_newtonstats.nstroke). pr is the
+ * guess on entry, the solution on exit. */
+int o_utoprimgen(const double* U, const o_geom* g, const o_params* P, double* pr, int* nstroke);
+/* same with geometry-free U (inputtype UNOTHING), as fixup1zone uses */
+int o_utoprimgen_unothing(const double* Ugf, const o_geom* g, const o_params* P, double* pr, int* nstroke);
+
+/* batched: SoA like harm_invert_batch */
+void o_invert_batch(const double* U, const double* geomrec, double* p, int32_t* pflag, int32_t* niter,
+                    int64_t n, double gam, double gammafail, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
