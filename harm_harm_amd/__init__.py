@@ -59,6 +59,8 @@ _PROTOS = [
     ("harm_geom_ks_batch", None, [_L, _P, _P, _P, _P]),
     ("harm_invert_batch", _I, [_P, _P, _P, _P, _P, _L, _D, _D, _P]),
     ("harm_prim_to_cons_batch", _I, [_P, _P, _P, _L, _D, _P]),
+    ("harm_host_tables", _I, [ctypes.POINTER(HarmConfig), _P, _P, _P, _P, _P, _P]),
+    ("harm_host_init_torus", _I, [ctypes.POINTER(HarmConfig), _D, _D, _D, _D, _D, ctypes.c_uint32, _P]),
     ("harm_device_count", _I, [ctypes.POINTER(_I)]),
     ("harm_dev_malloc", _I, [ctypes.POINTER(_P), ctypes.c_size_t]),
     ("harm_dev_free", _I, [_P]),
@@ -79,6 +81,7 @@ _OPTIONAL_PROTOS = [
     ("harm_storage_dims", _I, [_P, _P]),
     ("harm_get_geometry", _I, [_P, _I, _P]),
     ("harm_get_connection", _I, [_P, _P]),
+    ("harm_get_corner_gdet", _I, [_P, _P]),
     ("harm_get_coords", _I, [_P, _I, _I, _I, _P, _P]),
     ("harm_set_array", _I, [_P, _I, _P]),
     ("harm_get_array", _I, [_P, _I, _P]),

@@ -1,0 +1,1340 @@
+// grid.hip -- the HARM advance() step on one GPU's whole domain.
+//
+// Device state is structure-of-arrays a[pl][k][j][i] over the ghosted storage
+// box (i fastest, so every kernel's lanes walk i and HBM accesses coalesce);
+// geometry is stored once per (i,j) plane and reused across phi.  Each hot loop
+// of advance() is one kernel:
+//   k_flux<dir>   reconstruct + Riemann flux per face, fused   (flux.c:852)
+//   k_emf/k_ct    Toth corner EMFs and face B fluxes           (fluxct.c:186)
+//   k_update      U(pi), flux divergence, connection source,
+//                 RK combine and the source dt limit           (advance.c:420)
+//   k_invert      Newton inversion + floors                    (advance.c:748)
+//   k_bound_*     radial extrapolation, polar axis, periodic   (bounds.tools.c)
+//   k_post_fixup  failure averaging                            (fixup.c:1061)
+// The timestep lives in device memory: reductions use atomicMin on the bit
+// pattern of non-negative doubles and a one-thread kernel finishes the step,
+// so a whole step enqueues without a host round trip.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <vector>
+#include <algorithm>
+#include "dgrid.h"
+#include "hhost.h"
+#include "../host/hgeom.h"
+#include "../host/hinit.h"
+
+using hd::Dims;
+using hd::Geom;
+using hd::State;
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr double kBIG = 1.e+100;
+
+// device scalar slots
+enum { DS_DT = 0, DS_T = 1, DS_NDT1 = 2, DS_NDT2 = 3, DS_NDT3 = 4, DS_ACCDT = 5, DS_LASTNDT = 6, DS_FLUXACC = 8,
+       DS_N = 16 };
+enum { DI_NSTROKE = 0, DI_NFAIL = 1, DI_N = 4 };
+
+struct Params {
+  double gam, cour, safe, gammamax, gammafail;
+  double dx[4], startx[4];
+};
+
+__device__ __forceinline__ void atomic_min_pos(double* addr, double v)
+{
+  // valid for v >= 0 (IEEE order of non-negative doubles equals integer order)
+  atomicMin((unsigned long long*)addr, (unsigned long long)__double_as_longlong(v));
+}
+
+// block-wide minimum followed by one atomic per block
+__device__ __forceinline__ void block_min_to(double v, double* addr)
+{
+  __shared__ double red[kBlock / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_down(v, off, 64));
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = red[0];
+    for (int w = 1; w < kBlock / 64; w++) m = fmin(m, red[w]);
+    atomic_min_pos(addr, m);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// fluxes
+// ---------------------------------------------------------------------------
+
+struct Box {
+  int is, js, ks, ni, nj, nk;
+  __device__ __forceinline__ bool at(long t, int& i, int& j, int& k) const
+  {
+    const long n = (long)ni * nj * nk;
+    if (t >= n) return false;
+    i = is + (int)(t % ni);
+    const long r = t / ni;
+    j = js + (int)(r % nj);
+    k = ks + (int)(r / nj);
+    return true;
+  }
+  long count() const { return (long)ni * nj * nk; }
+};
+
+Box face_box(const Dims& d, int dir)
+{
+  const int e0 = d.n1 > 1, e1 = d.n2 > 1, e2 = d.n3 > 1;
+  int is = 0, ie = d.n1 - 1, js = 0, je = d.n2 - 1, ks = 0, ke = d.n3 - 1;
+  if (dir == 1) { ie = d.n1; js -= e1; je += e1; ks -= e2; ke += e2; }
+  if (dir == 2) { je = d.n2; is -= e0; ie += e0; ks -= e2; ke += e2; }
+  if (dir == 3) { ke = d.n3; is -= e0; ie += e0; js -= e1; je += e1; }
+  Box b{is, js, ks, ie - is + 1, je - js + 1, ke - ks + 1};
+  return b;
+}
+
+template <int DIR, int RECON>
+__global__ __launch_bounds__(kBlock) void k_flux(Dims d, Box box, const double* __restrict__ pb, double* __restrict__ F,
+                                                 const double* __restrict__ gface, Params P, int fluxtype, int do_ct,
+                                                 double* __restrict__ ds)
+{
+  int i, j, k;
+  double dtij = kBIG;
+  const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (box.at(t, i, j, k)) {
+    Geom G;
+    hd::load_geom(gface, hd::pidx(d, i, j), d.nplane, G);
+    const long c = hd::cidx(d, i, j, k);
+    const long st = (DIR == 1) ? 1 : (DIR == 2) ? (long)d.ns1 : (long)d.ns1 * d.ns2;
+    double p_l[HARM_NPR], p_r[HARM_NPR];
+#pragma unroll
+    for (int pl = 0; pl < HARM_NPR; pl++) {
+      const double* a = pb + (long)pl * d.ncell + c;
+      double y[6];
+#pragma unroll
+      for (int m = 0; m < 6; m++) y[m] = a[(m - 3) * st];
+      double l, r;
+      hd::recon_cell<RECON>(y + 2, l, r);  // cell c-1
+      p_l[pl] = r;
+      hd::recon_cell<RECON>(y + 3, l, r);  // cell c
+      p_r[pl] = l;
+    }
+    const int pbd = HARM_B1 + DIR - 1;
+    p_l[pbd] = p_r[pbd] = 0.5 * (p_l[pbd] + p_r[pbd]);
+    hd::limit_gamma(P.gammamax, p_l, G);
+    hd::limit_gamma(P.gammamax, p_r, G);
+    State ql, qr;
+    double F_l[HARM_NPR], F_r[HARM_NPR], U_l[HARM_NPR], U_r[HARM_NPR];
+    hd::get_state(p_l, G, P.gam, ql);
+    hd::primtoflux(p_l, ql, DIR, G, F_l);
+    hd::primtoflux(p_l, ql, 0, G, U_l);
+    hd::get_state(p_r, G, P.gam, qr);
+    hd::primtoflux(p_r, qr, DIR, G, F_r);
+    hd::primtoflux(p_r, qr, 0, G, U_r);
+    double cmaxl, cminl, cmaxr, cminr;
+    hd::vchar(p_l, ql, DIR, G, P.gam, cmaxl, cminl);
+    hd::vchar(p_r, qr, DIR, G, P.gam, cmaxr, cminr);
+    double lmin = fmin(cminl, cminr), lmax = fmax(cmaxl, cmaxr);
+    lmin = fabs(fmax(0., -lmin));
+    lmax = fabs(fmax(0., lmax));
+    const double cmin = lmin, cmax = lmax, ct = fmax(lmax, lmin);
+    double Fo[HARM_NPR];
+    if (fluxtype == HARM_FLUX_HLL && cmax + cmin != 0.0) {
+#pragma unroll
+      for (int pl = 0; pl < HARM_NPR; pl++)
+        Fo[pl] = ((cmax * F_l[pl] + cmin * F_r[pl]) - (cmax * cmin) * (U_r[pl] - U_l[pl])) / (cmax + cmin);
+    } else {
+#pragma unroll
+      for (int pl = 0; pl < HARM_NPR; pl++) Fo[pl] = (0.5 * ((F_l[pl] + F_r[pl]) - ct * (U_r[pl] - U_l[pl])));
+    }
+    if (do_ct) {
+      const double ign = hd::igdetnosing(G.gdet);
+      Fo[HARM_B1] *= ign;
+      Fo[HARM_B2] *= ign;
+      Fo[HARM_B3] *= ign;
+    }
+#pragma unroll
+    for (int pl = 0; pl < HARM_NPR; pl++) F[(long)pl * d.ncell + c] = Fo[pl];
+    dtij = P.cour * P.dx[DIR] / ct;
+  }
+  block_min_to(dtij, ds + DS_NDT1 + DIR - 1);
+}
+
+// Toth EMFs at corners, each multiplied by the corner gdet (CORNGDETVERSION)
+__global__ __launch_bounds__(kBlock) void k_emf(Dims d, int ni, int nj, int nk, const double* __restrict__ F1,
+                                                const double* __restrict__ F2, const double* __restrict__ F3,
+                                                const double* __restrict__ gf1, const double* __restrict__ gf2,
+                                                const double* __restrict__ gc3, double* __restrict__ emf)
+{
+  const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+  const long n = (long)ni * nj * nk;
+  if (t >= n) return;
+  const int i = (int)(t % ni), j = (int)((t / ni) % nj), k = (int)(t / ((long)ni * nj));
+  const int e0 = d.n1 > 1, e1 = d.n2 > 1, e2 = d.n3 > 1;
+  const double c1 = (e1 && e2) ? 0.25 : 0.5, c2 = (e0 && e2) ? 0.25 : 0.5, c3 = (e0 && e1) ? 0.25 : 0.5;
+  const int km = k - e2, jm = j - e1, im = i - e0;
+  const long nc = d.ncell;
+  auto f = [&](const double* F, int pl, int ii, int jj, int kk) { return F[(long)pl * nc + hd::cidx(d, ii, jj, kk)]; };
+  double e;
+  if (e1 || e2) {
+    if (e1 && e2) e = c1 * ((+f(F2, HARM_B3, i, j, k) + f(F2, HARM_B3, i, j, km)) - f(F3, HARM_B2, i, j, k) - f(F3, HARM_B2, i, jm, k));
+    else if (e1) e = c1 * (+f(F2, HARM_B3, i, j, k) + f(F2, HARM_B3, i, j, km));
+    else e = c1 * (-f(F3, HARM_B2, i, j, k) - f(F3, HARM_B2, i, jm, k));
+  } else e = 0.0;
+  const long pp = hd::pidx(d, i, j);
+  emf[t] = e * gf2[(long)HARM_G_GDET * d.nplane + pp];
+  if (e0 || e2) {
+    if (e0 && e2) e = c2 * ((+f(F3, HARM_B1, i, j, k) + f(F3, HARM_B1, im, j, k)) - f(F1, HARM_B3, i, j, k) - f(F1, HARM_B3, i, j, km));
+    else if (e0) e = c2 * (-f(F1, HARM_B3, i, j, k) - f(F1, HARM_B3, i, j, km));
+    else e = c2 * (+f(F3, HARM_B1, i, j, k) + f(F3, HARM_B1, im, j, k));
+  } else e = 0.0;
+  emf[n + t] = e * gf1[(long)HARM_G_GDET * d.nplane + pp];
+  if (e0 || e1) {
+    if (e0 && e1) e = c3 * ((+f(F1, HARM_B2, i, j, k) + f(F1, HARM_B2, i, jm, k)) - f(F2, HARM_B1, i, j, k) - f(F2, HARM_B1, im, j, k));
+    else if (e0) e = c3 * (+f(F1, HARM_B2, i, j, k) + f(F1, HARM_B2, i, jm, k));
+    else e = c3 * (-f(F2, HARM_B1, i, j, k) - f(F2, HARM_B1, im, j, k));
+  } else e = 0.0;
+  emf[2 * n + t] = e * gc3[pp];
+}
+
+template <int DIR>
+__global__ __launch_bounds__(kBlock) void k_ct(Dims d, int ni, int nj, int nk, const double* __restrict__ emf,
+                                               double* __restrict__ F)
+{
+  const int e0 = d.n1 > 1, e1 = d.n2 > 1, e2 = d.n3 > 1;
+  const int fi = d.n1 + (DIR == 1 ? e0 : 0), fj = d.n2 + (DIR == 2 ? e1 : 0), fk = d.n3 + (DIR == 3 ? e2 : 0);
+  const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= (long)fi * fj * fk) return;
+  const int i = (int)(t % fi), j = (int)((t / fi) % fj), k = (int)(t / ((long)fi * fj));
+  const long n = (long)ni * nj * nk;
+  auto E = [&](int c, int ii, int jj, int kk) { return emf[c * n + ((long)kk * nj + jj) * ni + ii]; };
+  const long o = hd::cidx(d, i, j, k);
+  const long nc = d.ncell;
+  if (DIR == 1) {
+    F[HARM_B1 * nc + o] = 0.;
+    F[HARM_B2 * nc + o] = 0.5 * (E(2, i, j, k) + E(2, i, j + e1, k));
+    F[HARM_B3 * nc + o] = -0.5 * (E(1, i, j, k) + E(1, i, j, k + e2));
+  } else if (DIR == 2) {
+    F[HARM_B1 * nc + o] = -0.5 * (E(2, i, j, k) + E(2, i + e0, j, k));
+    F[HARM_B2 * nc + o] = 0.;
+    F[HARM_B3 * nc + o] = 0.5 * (E(0, i, j, k) + E(0, i, j, k + e2));
+  } else {
+    F[HARM_B1 * nc + o] = 0.5 * (E(1, i, j, k) + E(1, i + e0, j, k));
+    F[HARM_B2 * nc + o] = -0.5 * (E(0, i, j, k) + E(0, i, j + e1, k));
+    F[HARM_B3 * nc + o] = 0.;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// update: Ui = U(pi); dU = -div F + source(pb); Uf, ucum; source dt limit
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_update(Dims d, Params P, const double* __restrict__ pi,
+                                                   const double* __restrict__ pb, const double* __restrict__ F1,
+                                                   const double* __restrict__ F2, const double* __restrict__ F3,
+                                                   double* __restrict__ ui, double* __restrict__ uf,
+                                                   double* __restrict__ ucum, const double* __restrict__ gcent,
+                                                   const double* __restrict__ conn, double cuf0, double cuf1,
+                                                   double cuf2, double cun0, double cun1, double cun2, int timeorder,
+                                                   double* __restrict__ ds)
+{
+  const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+  const long nact = (long)d.n1 * d.n2 * d.n3;
+  double dts = kBIG;
+  if (t < nact) {
+    const int i = (int)(t % d.n1), j = (int)((t / d.n1) % d.n2), k = (int)(t / ((long)d.n1 * d.n2));
+    const long o = hd::cidx(d, i, j, k);
+    const long pp = hd::pidx(d, i, j);
+    const long nc = d.ncell;
+    const double dt = ds[DS_DT];
+    Geom G;
+    hd::load_geom(gcent, pp, d.nplane, G);
+    double pz[HARM_NPR], Ui[HARM_NPR];
+#pragma unroll
+    for (int pl = 0; pl < HARM_NPR; pl++) pz[pl] = pi[pl * nc + o];
+    State q;
+    hd::get_state(pz, G, P.gam, q);
+    hd::primtoflux(pz, q, 0, G, Ui);
+#pragma unroll
+    for (int pl = 0; pl < HARM_NPR; pl++) ui[pl * nc + o] = Ui[pl];
+    const double idx1 = 1.0 / P.dx[1], idx2 = 1.0 / P.dx[2], idx3 = 1.0 / P.dx[3];
+    const long s2 = d.ns1, s3 = (long)d.ns1 * d.ns2;
+    double dUr[HARM_NPR];
+#pragma unroll
+    for (int pl = 0; pl < HARM_NPR; pl++) {
+      double dU1 = 0, dU2 = 0, dU3 = 0;
+      if (d.n1 > 1) dU1 = (-(F1[pl * nc + o + 1] - F1[pl * nc + o]) * idx1);
+      if (d.n2 > 1) dU2 = (-(F2[pl * nc + o + s2] - F2[pl * nc + o]) * idx2);
+      if (d.n3 > 1) dU3 = (-(F3[pl * nc + o + s3] - F3[pl * nc + o]) * idx3);
+      dUr[pl] = dU1 + dU2 + dU3;
+    }
+    double pbz[HARM_NPR], dUg[HARM_NPR];
+#pragma unroll
+    for (int pl = 0; pl < HARM_NPR; pl++) pbz[pl] = pb[pl * nc + o];
+    State qb;
+    hd::get_state(pbz, G, P.gam, qb);
+    const double* cz = conn + pp;
+    const long np = d.nplane;
+    hd::source(pbz, G, qb, [&](int c) { return cz[(long)c * np]; }, dUg);
+    double dUt[HARM_NPR];
+#pragma unroll
+    for (int pl = 0; pl < HARM_NPR; pl++) {
+      const double ufo = (timeorder == 0) ? 0.0 : uf[pl * nc + o];
+      const double uco = (timeorder == 0) ? 0.0 : ucum[pl * nc + o];
+      const double ufn = (cuf0 * Ui[pl] + cuf1 * ufo + cuf2 * dt * (dUr[pl] + dUg[pl]));
+      uf[pl * nc + o] = ufn;
+      ucum[pl * nc + o] = uco + (cun0 * Ui[pl] + cun1 * dt * (dUr[pl] + dUg[pl]) + cun2 * ufn);
+      dUt[pl] = dUr[pl] + dUg[pl];
+    }
+    // dUtodt -> compute_dt_fromsource
+    double U[HARM_NPR], Ugf[HARM_NPR];
+    hd::primtoflux(pbz, qb, 0, G, U);
+    const double ig = hd::igdetnosing(G.gdet);
+#pragma unroll
+    for (int pl = 0; pl < HARM_NPR; pl++) Ugf[pl] = U[pl] * ig;
+    double dUd[4], dUu[4], mydU[4], mydx[4];
+#pragma unroll
+    for (int jj = 0; jj < 4; jj++) dUd[jj] = (dUt[HARM_UU + jj] * dt) * ig;
+    hd::raise(dUd, G, dUu);
+    mydU[0] = dUd[0];
+    mydx[0] = dt;
+#pragma unroll
+    for (int jj = 1; jj < 4; jj++) { mydU[jj] = dUu[jj]; mydx[jj] = P.dx[jj]; }
+    double bcon[4], bcov[4];
+    bcon[0] = pbz[HARM_B1] * qb.ucov[1] + pbz[HARM_B2] * qb.ucov[2] + pbz[HARM_B3] * qb.ucov[3];
+#pragma unroll
+    for (int jj = 1; jj < 4; jj++) bcon[jj] = (pbz[HARM_B1 - 1 + jj] + bcon[0] * qb.ucon[jj]) / qb.ucon[0];
+    hd::lower(bcon, G, bcov);
+    const double bsq = bcon[0] * bcov[0] + bcon[1] * bcov[1] + bcon[2] * bcov[2] + bcon[3] * bcov[3];
+    const double rho = pbz[HARM_RHO], u = pbz[HARM_UU];
+    const double Pg = (P.gam - 1.0) * u;
+    const double w = rho + u + Pg;
+    const double eta = w + bsq;
+    double dtsource[4];
+#pragma unroll
+    for (int jj = 0; jj < 4; jj++) {
+      const double rhoprime = fmax(fabs(eta), fabs(Ugf[HARM_UU]));
+      const double ag = D_SMALL + fabs(mydU[jj] / rhoprime);
+      dtsource[jj] = P.cour * (mydx[jj] / ag);
+    }
+    dts = kBIG;
+    if (dtsource[0] < dts) dts = dtsource[0];
+    if (d.n1 > 1 && dtsource[1] < dts) dts = dtsource[1];
+    if (d.n2 > 1 && dtsource[2] < dts) dts = dtsource[2];
+    if (d.n3 > 1 && dtsource[3] < dts) dts = dtsource[3];
+  }
+  block_min_to(dts, ds + DS_ACCDT);
+}
+
+// ---------------------------------------------------------------------------
+// inversion + fixup1zone (floors via ZAMO-frame mass/energy addition)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_invert(Dims d, Params P, const double* __restrict__ U,
+                                                   double* __restrict__ pf, int32_t* __restrict__ pflag,
+                                                   const double* __restrict__ gcent, const double* __restrict__ floors,
+                                                   int finalstep, unsigned long long* __restrict__ di)
+{
+  const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+  const long nact = (long)d.n1 * d.n2 * d.n3;
+  int ns = 0, nf = 0;
+  if (t < nact) {
+    const int i = (int)(t % d.n1), j = (int)((t / d.n1) % d.n2), k = (int)(t / ((long)d.n1 * d.n2));
+    const long o = hd::cidx(d, i, j, k);
+    const long pp = hd::pidx(d, i, j);
+    const long nc = d.ncell;
+    Geom G;
+    hd::load_geom(gcent, pp, d.nplane, G);
+    double Uz[HARM_NPR], pz[HARM_NPR];
+#pragma unroll
+    for (int pl = 0; pl < HARM_NPR; pl++) {
+      Uz[pl] = U[pl * nc + o];
+      pz[pl] = pf[pl * nc + o];
+    }
+    int fl = hd::utoprimgen(Uz, G, P.gam, P.gammafail, pz, ns, __builtin_nan(""));
+    if (finalstep) {
+      double prfloor[2] = {floors[pp], floors[d.nplane + pp]};
+      for (int pl = 0; pl < 2; pl++)
+        if (prfloor[pl] < D_SMALL) prfloor[pl] = D_SMALL;
+      double prnew[HARM_NPR];
+      bool didchange = false;
+#pragma unroll
+      for (int pl = 0; pl < HARM_NPR; pl++) {
+        if (pl <= HARM_UU && prfloor[pl] > pz[pl]) { didchange = true; prnew[pl] = prfloor[pl]; }
+        else prnew[pl] = pz[pl];
+      }
+      if (didchange) {
+        Geom G1 = G;
+        G1.gdet = 1.0;
+        State q, dq;
+        double Ua[HARM_NPR], dpr[HARM_NPR], dUa[HARM_NPR];
+        hd::get_state(pz, G, P.gam, q);
+        hd::primtoflux(pz, q, 0, G1, Ua);
+#pragma unroll
+        for (int pl = 0; pl < HARM_NPR; pl++) dpr[pl] = 0.0;
+        for (int pl = HARM_RHO; pl <= HARM_UU; pl++) dpr[pl] = prnew[pl] - pz[pl];
+        hd::get_state(dpr, G, P.gam, dq);
+        hd::primtoflux(dpr, dq, 0, G1, dUa);
+#pragma unroll
+        for (int pl = 0; pl < HARM_NPR; pl++) Ua[pl] += dUa[pl];
+        fl = hd::jon(Ua, G, P.gam, P.gammafail, pz, ns, __builtin_nan(""));
+        if (fl > 0) {
+#pragma unroll
+          for (int pl = 0; pl < HARM_NPR; pl++) pz[pl] = prnew[pl];
+        }
+      }
+      hd::limit_gamma(P.gammamax, pz, G);
+    }
+#pragma unroll
+    for (int pl = 0; pl < HARM_NPR; pl++) pf[pl * nc + o] = pz[pl];
+    pflag[o] = fl;
+    nf = fl > 0;
+  }
+  // integer sums are order independent
+  __shared__ unsigned long long red[2];
+  if (threadIdx.x == 0) { red[0] = 0; red[1] = 0; }
+  __syncthreads();
+  if (ns) atomicAdd(&red[0], (unsigned long long)ns);
+  if (nf) atomicAdd(&red[1], 1ull);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (red[0]) atomicAdd(&di[DI_NSTROKE], red[0]);
+    if (red[1]) atomicAdd(&di[DI_NFAIL], red[1]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// failure averaging (fixup_utoprim with the simple_average cascade)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_post_fixup(Dims d, Params P, const double* __restrict__ pa,
+                                                       double* __restrict__ pf, const int32_t* __restrict__ pflag,
+                                                       const double* __restrict__ gcent)
+{
+  const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+  const long nact = (long)d.n1 * d.n2 * d.n3;
+  if (t >= nact) return;
+  const int i = (int)(t % d.n1), j = (int)((t / d.n1) % d.n2), k = (int)(t / ((long)d.n1 * d.n2));
+  const long o = hd::cidx(d, i, j, k);
+  if (pflag[o] <= 0) return;
+  const int e1 = d.n2 > 1;
+  auto good = [&](int ii, int jj) {
+    if (ii < 0 || ii >= d.n1 || jj < 0 || jj >= d.n2) return true;
+    return pflag[hd::cidx(d, ii, jj, k)] <= 0;
+  };
+  const long nc = d.ncell;
+  auto A = [&](int pl, int ii, int jj) { return pa[pl * nc + hd::cidx(d, ii, jj, k)]; };
+  const int jp = j + e1, jm = j - e1;
+  double pz[HARM_NPR];
+#pragma unroll
+  for (int pl = 0; pl < HARM_NPR; pl++) pz[pl] = pf[pl * nc + o];
+  for (int pl = HARM_RHO; pl <= HARM_U3; pl++) {
+    double v;
+    if (good(i, jp) && good(i, jm) && good(i + 1, j) && good(i - 1, j))
+      v = 0.25 * (A(pl, i, jp) + A(pl, i, jm) + A(pl, i + 1, j) + A(pl, i - 1, j));
+    else if (good(i + 1, jp) && good(i + 1, jm) && good(i - 1, jp) && good(i - 1, jm))
+      v = 0.25 * (A(pl, i + 1, jp) + A(pl, i + 1, jm) + A(pl, i - 1, jp) + A(pl, i - 1, jm));
+    else if (good(i, jp) && good(i, jm)) v = 0.5 * (A(pl, i, jp) + A(pl, i, jm));
+    else if (good(i + 1, j) && good(i - 1, j)) v = 0.5 * (A(pl, i + 1, j) + A(pl, i - 1, j));
+    else if (good(i + 1, jp) && good(i - 1, jm)) v = 0.5 * (A(pl, i + 1, jp) + A(pl, i - 1, jm));
+    else if (good(i + 1, jm) && good(i - 1, jp)) v = 0.5 * (A(pl, i + 1, jm) + A(pl, i - 1, jp));
+    else if (good(i + 1, jp)) v = A(pl, i + 1, jp);
+    else if (good(i + 1, j)) v = A(pl, i + 1, j);
+    else v = A(pl, i, j);
+    pz[pl] = v;
+  }
+  Geom G;
+  hd::load_geom(gcent, hd::pidx(d, i, j), d.nplane, G);
+  hd::limit_gamma(P.gammamax, pz, G);
+#pragma unroll
+  for (int pl = 0; pl < HARM_NPR; pl++) pf[pl * nc + o] = pz[pl];
+}
+
+// ---------------------------------------------------------------------------
+// boundary conditions
+// ---------------------------------------------------------------------------
+__device__ void extrap_line(const Dims& d, const Params& P, double* p, const double* gcent, int up, int j, int k)
+{
+  const long nc = d.ncell;
+  const int ri = up ? d.n1 - 1 : 0;
+  const int ri2 = up ? ri - 1 : ri + 1;
+  const int ri3 = up ? ri - 2 : ri + 2;
+  const double signdq = up ? -1.0 : 1.0;
+  Geom Gr, Gr2, Gr3;
+  hd::load_geom(gcent, hd::pidx(d, ri, j), d.nplane, Gr);
+  hd::load_geom(gcent, hd::pidx(d, ri2, j), d.nplane, Gr2);
+  hd::load_geom(gcent, hd::pidx(d, ri3, j), d.nplane, Gr3);
+  const long o1 = hd::cidx(d, ri, j, k), o2 = hd::cidx(d, ri2, j, k), o3 = hd::cidx(d, ri3, j, k);
+  double pr[HARM_NPR], pr2[HARM_NPR], pr3[HARM_NPR];
+#pragma unroll
+  for (int pl = 0; pl < HARM_NPR; pl++) {
+    pr[pl] = p[pl * nc + o1];
+    pr2[pl] = p[pl * nc + o2];
+    pr3[pl] = p[pl * nc + o3];
+  }
+  double Bd3 = 0.0, Bd3ri2 = 0.0, Bd3ri3 = 0.0;
+#pragma unroll
+  for (int jj = 1; jj < 4; jj++) Bd3 += pr[HARM_B1 + jj - 1] * hd::gco(Gr, 3, jj);
+#pragma unroll
+  for (int jj = 1; jj < 4; jj++) Bd3ri2 += pr2[HARM_B1 + jj - 1] * hd::gco(Gr2, 3, jj);
+#pragma unroll
+  for (int jj = 1; jj < 4; jj++) Bd3ri3 += pr3[HARM_B1 + jj - 1] * hd::gco(Gr3, 3, jj);
+  double Dqp = Bd3ri3 - Bd3ri2, Dqm = Bd3ri2 - Bd3, Dqc = Bd3ri3 - Bd3;
+  const double dqBd3 = signdq * hd::minmod(hd::minmod(Dqp, Dqm), Dqc);
+  State qr, qr2;
+  hd::get_state(pr, Gr, P.gam, qr);
+  hd::get_state(pr2, Gr2, P.gam, qr2);
+  double dq[HARM_NPR], dql[HARM_NPR], dqg[HARM_NPR];
+#pragma unroll
+  for (int pl = 0; pl < HARM_NPR; pl++) {
+    Dqp = pr3[pl] - pr2[pl];
+    Dqm = pr2[pl] - pr[pl];
+    Dqc = pr3[pl] - pr[pl];
+    dq[pl] = signdq * hd::minmod(hd::minmod(Dqp, Dqm), Dqc);
+    Dqp = log(D_SMALL + fabs(pr3[pl])) - log(D_SMALL + fabs(pr2[pl]));
+    Dqm = log(D_SMALL + fabs(pr2[pl])) - log(D_SMALL + fabs(pr[pl]));
+    Dqc = log(D_SMALL + fabs(pr3[pl])) - log(D_SMALL + fabs(pr[pl]));
+    dql[pl] = signdq * hd::minmod(hd::minmod(Dqp, Dqm), Dqc);
+    Dqp = (Gr3.gdet) * pr3[pl] - (Gr2.gdet) * pr2[pl];
+    Dqm = (Gr2.gdet) * pr2[pl] - (Gr.gdet) * pr[pl];
+    Dqc = (Gr3.gdet) * pr3[pl] - (Gr.gdet) * pr[pl];
+    dqg[pl] = signdq * hd::minmod(hd::minmod(Dqp, Dqm), Dqc);
+  }
+  const double xdqfrac = fabs(qr.ucon[0] - qr2.ucon[0]) / qr.ucon[0];
+  double ydqfrac;
+  if (xdqfrac < 0.1) ydqfrac = 0.0;
+  else if (xdqfrac < 0.2) ydqfrac = 0.0 + (1.0 - 0.0) / (0.2 - 0.1) * (xdqfrac - 0.1);
+  else ydqfrac = 1.0;
+#pragma unroll
+  for (int jj = 0; jj < 4; jj++) {
+    dq[HARM_UU + jj] = dq[HARM_UU + jj] * (1.0 - ydqfrac);
+    dql[HARM_UU + jj] = dql[HARM_UU + jj] * (1.0 - ydqfrac);
+    dqg[HARM_UU + jj] = dqg[HARM_UU + jj] * (1.0 - ydqfrac);
+  }
+  const int is = up ? d.n1 : -d.ng1;
+  const int ie = up ? d.n1 + d.ng1 - 1 : -1;
+  for (int i = is; i <= ie; i++) {
+    double pi[HARM_NPR];
+#pragma unroll
+    for (int pl = 0; pl < HARM_NPR; pl++) pi[pl] = pr[pl];
+    Geom Gi;
+    hd::load_geom(gcent, hd::pidx(d, i, j), d.nplane, Gi);
+    const double di = (double)(i - ri);
+    for (int pl = 0; pl <= HARM_UU; pl++) {
+      const double ftemp = exp(-signdq * dql[pl]) - 1.1;
+      double mydqlog;
+      if (ftemp >= 0.0) mydqlog = log(1.1) / (-signdq);
+      else mydqlog = dql[pl];
+      pi[pl] = exp(log(D_SMALL + fabs(pr[pl])) + mydqlog * di);
+    }
+    for (int pl = HARM_U1; pl <= HARM_U3; pl++) pi[pl] = pr[pl] + dq[pl] * di;
+    for (int pl = HARM_B1; pl <= HARM_B2; pl++) {
+      const double ign = hd::igdetnosing(Gi.gdet);
+      pi[pl] = (pr[pl] * fabs((Gr.gdet) + dqg[pl] * di) * ign);
+    }
+    const double Bu1 = pi[HARM_B1], Bu2 = pi[HARM_B2];
+    const double gcon03 = hd::gcn(Gi, 0, 3), gcon13 = hd::gcn(Gi, 1, 3), gcon23 = hd::gcn(Gi, 2, 3),
+                 gcon33 = hd::gcn(Gi, 3, 3);
+    const double gcov01 = hd::gco(Gi, 0, 1), gcov02 = hd::gco(Gi, 0, 2), gcov11 = hd::gco(Gi, 1, 1);
+    const double gcov12 = hd::gco(Gi, 1, 2), gcov21 = gcov12, gcov22 = hd::gco(Gi, 2, 2);
+    const double gcov03 = hd::gco(Gi, 0, 3), gcov13 = hd::gco(Gi, 1, 3), gcov23 = hd::gco(Gi, 2, 3);
+    const double myBd3 = Bd3 + dqBd3 * di;
+    const double ftemp = (1.0 - gcon03 * gcov03 - gcon13 * gcov13 - gcon23 * gcov23);
+    const double ign = hd::igdetnosing(ftemp);
+    pi[HARM_B3] = (myBd3 * gcon33 + Bu1 * gcon03 * gcov01 + Bu2 * gcon03 * gcov02 + Bu1 * gcon13 * gcov11
+                   + Bu2 * gcon13 * gcov12 + Bu1 * gcon23 * gcov21 + Bu2 * gcon23 * gcov22) * ign;
+    if (up) {
+      hd::inflow_check_x1up(pi, Gi, P.gam, P.gammamax);
+      hd::limit_gamma(P.gammamax, pi, Gi);
+    }
+    const long oi = hd::cidx(d, i, j, k);
+#pragma unroll
+    for (int pl = 0; pl < HARM_NPR; pl++) p[pl * nc + oi] = pi[pl];
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_bound_x1(Dims d, Params P, double* __restrict__ p,
+                                                     const double* __restrict__ gcent)
+{
+  const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= 2L * d.n2 * d.n3) return;
+  const int up = (int)(t & 1);
+  const long l = t >> 1;
+  const int j = (int)(l % d.n2), k = (int)(l / d.n2);
+  extrap_line(d, P, p, gcent, up, j, k);
+}
+
+__global__ __launch_bounds__(kBlock) void k_x2_reflect(Dims d, double* __restrict__ p, int side)
+{
+  const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+  const long ncol = (long)d.ns1 * d.n3;
+  if (t >= ncol * d.ng2) return;
+  const int g = (int)(t / ncol);
+  const long col = t % ncol;
+  const int i = (int)(col % d.ns1) - d.ng1, k = (int)(col / d.ns1);
+  const int n2 = d.n2, n3 = d.n3;
+  const int rk = (n3 > 1) ? (k + n3 / 2) % n3 : k;
+  const int j = side == 0 ? -d.ng2 + g : n2 + g;
+  const int rj = side == 0 ? -j - 1 : 2 * (n2 - 1) - j + 1;
+  const long nc = d.ncell;
+  const long so = hd::cidx(d, i, rj, rk), doff = hd::cidx(d, i, j, k);
+#pragma unroll
+  for (int pl = 0; pl < HARM_NPR; pl++) {
+    double v = p[pl * nc + so];
+    if (pl == HARM_U2 || pl == HARM_B2) v *= -1.;
+    p[pl * nc + doff] = v;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_x2_poledeath(Dims d, Params P, double* __restrict__ p,
+                                                         const double* __restrict__ gcent,
+                                                         const double* __restrict__ rcent, int side)
+{
+  const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+  const long ncol = (long)d.ns1 * d.n3;
+  if (t >= ncol) return;
+  const int i = (int)(t % d.ns1) - d.ng1, k = (int)(t / d.ns1);
+  const int n2 = d.n2, n3 = d.n3;
+  const long nc = d.ncell;
+  const double x2c0 = P.startx[2];
+  const int rj = side == 0 ? 1 : n2 - 2;
+  const int js = side == 0 ? -1 : n2 - 1;
+  const int je = side == 0 ? 0 : n2;
+  const double X0 = side == 0 ? x2c0 : x2c0 + n2 * P.dx[2];
+  const double Xr = x2c0 + (rj + 0.5) * P.dx[2];
+  const long ro = hd::cidx(d, i, rj, k);
+  double pref[HARM_NPR];
+#pragma unroll
+  for (int pl = 0; pl < HARM_NPR; pl++) pref[pl] = p[pl * nc + ro];
+  for (int j = js; j <= je; j++) {
+    const long o = hd::cidx(d, i, j, k);
+    double pj[HARM_NPR];
+#pragma unroll
+    for (int pl = 0; pl < HARM_NPR; pl++) pj[pl] = p[pl * nc + o];
+    const double X = x2c0 + (j + 0.5) * P.dx[2];
+    for (int pl = HARM_RHO; pl <= HARM_U3; pl++) {
+      if (pl == HARM_U2) continue;
+      pj[pl] = pref[pl];
+    }
+    double ftemp = pref[HARM_U2];
+    if ((side == 0 && ftemp > 0.0) || (side == 1 && ftemp < 0.0)) ftemp = 0.0;
+    pj[HARM_U2] = ftemp + (X - Xr) * (ftemp - 0.0) / (Xr - X0);
+    if (n3 == 1) pj[HARM_B3] = pref[HARM_B3];
+    Geom Gj;
+    hd::load_geom(gcent, hd::pidx(d, i, j), d.nplane, Gj);
+    State q;
+    hd::get_state(pj, Gj, P.gam, q);
+    if (q.ucon[1] > 0.0) {
+      const double r = rcent[i + d.ng1];
+      double lim;
+      if (r <= 10.0) lim = 1.1;
+      else lim = 1.1 * pow(r / 10.0, 1.0);
+      hd::limit_gamma(lim, pj, Gj);
+    } else {
+      hd::limit_gamma(P.gammamax, pj, Gj);
+    }
+#pragma unroll
+    for (int pl = 0; pl < HARM_NPR; pl++) p[pl * nc + o] = pj[pl];
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_bound_x3(Dims d, double* __restrict__ p)
+{
+  const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+  const long nplane = d.nplane;
+  if (t >= nplane * 2 * d.ng3) return;
+  const long pp = t % nplane;
+  const int g = (int)(t / nplane);
+  const int k = g < d.ng3 ? -d.ng3 + g : d.n3 + (g - d.ng3);
+  const int ks = k < 0 ? k + d.n3 : k - d.n3;
+  const long nc = d.ncell;
+  const long o = (long)(k + d.ng3) * nplane + pp, so = (long)(ks + d.ng3) * nplane + pp;
+#pragma unroll
+  for (int pl = 0; pl < HARM_NPR; pl++) p[pl * nc + o] = p[pl * nc + so];
+}
+
+// ---------------------------------------------------------------------------
+// scalars: reset, per-substep dt, end of step
+// ---------------------------------------------------------------------------
+__global__ void k_scal_begin_flux(double* ds)
+{
+  ds[DS_NDT1] = kBIG;
+  ds[DS_NDT2] = kBIG;
+  ds[DS_NDT3] = kBIG;
+  ds[DS_ACCDT] = kBIG;
+}
+__global__ void k_scal_begin_step(double* ds, unsigned long long* di)
+{
+  ds[DS_LASTNDT] = kBIG;
+  di[DI_NSTROKE] = 0;
+  di[DI_NFAIL] = 0;
+}
+__global__ void k_scal_substep_dt(double* ds)
+{
+  const double wavedt = 1. / (1. / ds[DS_NDT1] + 1. / ds[DS_NDT2] + 1. / ds[DS_NDT3]);
+  const double nd = 1.0 * fmin(wavedt, ds[DS_ACCDT]);
+  if (nd < ds[DS_LASTNDT]) ds[DS_LASTNDT] = nd;
+}
+__global__ void k_scal_end_step(double* ds, double safe)
+{
+  double fullndt = ds[DS_LASTNDT];
+  const double dt = ds[DS_DT];
+  if (fullndt > safe * dt) fullndt = safe * dt;
+  ds[DS_T] = ds[DS_T] + dt;
+  ds[DS_DT] = fullndt;
+}
+
+// boundary flux sums through the inner and outer radial faces (deterministic)
+__global__ __launch_bounds__(kBlock) void k_fluxacc(Dims d, Params P, const double* __restrict__ F1,
+                                                    double* __restrict__ ds, double weight)
+{
+  __shared__ double red[4][kBlock];
+  double s[4] = {0, 0, 0, 0};
+  const long nc = d.ncell;
+  for (long t = threadIdx.x; t < (long)d.n2 * d.n3; t += kBlock) {
+    const int j = (int)(t % d.n2), k = (int)(t / d.n2);
+    const long oi = hd::cidx(d, 0, j, k), oo = hd::cidx(d, d.n1, j, k);
+    s[0] += F1[HARM_RHO * nc + oi];
+    s[1] += F1[HARM_UU * nc + oi];
+    s[2] += F1[HARM_RHO * nc + oo];
+    s[3] += F1[HARM_UU * nc + oo];
+  }
+  for (int c = 0; c < 4; c++) red[c][threadIdx.x] = s[c];
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int c = 0; c < 4; c++) red[c][threadIdx.x] += red[c][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int c = 0; c < 4; c++) ds[DS_FLUXACC + c] += red[c][0] * P.dx[2] * P.dx[3] * ds[DS_DT] * weight;
+}
+
+inline unsigned nblocks(long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+}  // namespace
+
+// ===========================================================================
+// handle
+// ===========================================================================
+struct harm_handle {
+  harm_config cfg;
+  Dims d;
+  Params P;
+  hipStream_t stream = nullptr;
+  double* arr[HARM_NARR] = {nullptr};
+  double* pa = nullptr;
+  int32_t* pflag = nullptr;
+  double* geom[HARM_NLOC] = {nullptr};
+  double* gcorn3 = nullptr;
+  double* conn = nullptr;
+  double* floors = nullptr;
+  double* rcent = nullptr;
+  double* emf = nullptr;
+  double* ds = nullptr;
+  unsigned long long* di = nullptr;
+  int emf_ni = 0, emf_nj = 0, emf_nk = 0;
+  std::vector<double> hgeom[HARM_NLOC];  // AoS host copies
+  std::vector<double> hconn;
+  std::vector<double> hrcent;
+  std::vector<double> hgc3;
+};
+
+namespace {
+
+harm_status upload(harm_handle* h, double* dst, const double* src, size_t n)
+{
+  return hh::check_hip(hipMemcpy(dst, src, n * sizeof(double), hipMemcpyHostToDevice), "upload");
+}
+
+harm_status alloc(void** p, size_t bytes) { return hh::check_hip(hipMalloc(p, bytes ? bytes : 8), "hipMalloc"); }
+
+void make_dims(const harm_config* cfg, Dims& d, Params& P)
+{
+  d.n1 = cfg->n1; d.n2 = cfg->n2; d.n3 = cfg->n3;
+  d.ng1 = d.n1 > 1 ? cfg->ng : 0;
+  d.ng2 = d.n2 > 1 ? cfg->ng : 0;
+  d.ng3 = d.n3 > 1 ? cfg->ng : 0;
+  d.ns1 = d.n1 + 2 * d.ng1; d.ns2 = d.n2 + 2 * d.ng2; d.ns3 = d.n3 + 2 * d.ng3;
+  d.nplane = (long)d.ns1 * d.ns2;
+  d.ncell = d.nplane * d.ns3;
+  P.gam = cfg->gam; P.cour = cfg->cour; P.safe = cfg->safe; P.gammamax = cfg->gammamax; P.gammafail = cfg->gammafail;
+  P.startx[0] = 0.0; P.dx[0] = 0.0;
+  P.startx[1] = log(cfg->Rin - cfg->R0);
+  P.dx[1] = (log(cfg->Rout - cfg->R0) - log(cfg->Rin - cfg->R0)) / cfg->n1;
+  P.startx[2] = 0.0; P.dx[2] = 1.0 / cfg->n2;
+  P.startx[3] = 0.0; P.dx[3] = 1.0 / cfg->n3;
+}
+
+// Host geometry tables over the 2-D storage plane (AoS per point).
+void host_tables(const harm_config& c, const Dims& d, const Params& P, std::vector<double>* geom_aos,
+                 std::vector<double>& conn_aos, std::vector<double>& gc3, std::vector<double>& rcent)
+{
+  const long np = d.nplane;
+  for (int loc = 0; loc < HARM_NLOC; loc++) {
+    geom_aos[loc].assign((size_t)np * HARM_G_NFIELD, 0.0);
+#pragma omp parallel for collapse(2)
+    for (int jj = 0; jj < d.ns2; jj++)
+      for (int ii = 0; ii < d.ns1; ii++) {
+        const int i = ii - d.ng1, j = jj - d.ng2;
+        double X[4] = {0.0, P.startx[1] + (i + (loc == HARM_LOC_FACE1 ? 0.0 : 0.5)) * P.dx[1],
+                       P.startx[2] + (j + (loc == HARM_LOC_FACE2 ? 0.0 : 0.5)) * P.dx[2], P.startx[3] + 0.5 * P.dx[3]};
+        hg_mks_record(&c, X, &geom_aos[loc][((long)jj * d.ns1 + ii) * HARM_G_NFIELD]);
+      }
+  }
+  gc3.assign(np, 0.0);
+  for (int jj = 0; jj < d.ns2; jj++)
+    for (int ii = 0; ii < d.ns1; ii++) {
+      const int i = ii - d.ng1, j = jj - d.ng2;
+      double X[4] = {0.0, P.startx[1] + i * P.dx[1], P.startx[2] + j * P.dx[2], 0.0};
+      double rec[HARM_G_NFIELD];
+      hg_mks_record(&c, X, rec);
+      gc3[(size_t)jj * d.ns1 + ii] = rec[HARM_G_GDET];
+    }
+  conn_aos.assign((size_t)np * HARM_NCONN, 0.0);
+#pragma omp parallel for collapse(2) schedule(dynamic, 16)
+  for (int jj = 0; jj < d.ns2; jj++)
+    for (int ii = 0; ii < d.ns1; ii++) {
+      const int i = ii - d.ng1, j = jj - d.ng2;
+      double X[4] = {0.0, P.startx[1] + (i + 0.5) * P.dx[1], P.startx[2] + (j + 0.5) * P.dx[2],
+                     P.startx[3] + 0.5 * P.dx[3]};
+      hg_mks_conn(&c, X, 0.5 * P.dx[1], 0.5 * P.dx[2], &conn_aos[((long)jj * d.ns1 + ii) * HARM_NCONN]);
+    }
+  rcent.assign(d.ns1, 0.0);
+  for (int ii = 0; ii < d.ns1; ii++) {
+    double X[4] = {0.0, P.startx[1] + (ii - d.ng1 + 0.5) * P.dx[1], 0.5, 0.0}, V[4];
+    hg_mks_V(&c, X, V);
+    rcent[ii] = V[1];
+  }
+}
+
+harm_status build_geometry(harm_handle* h)
+{
+  const Dims& d = h->d;
+  const harm_config& c = h->cfg;
+  const long np = d.nplane;
+  host_tables(c, d, h->P, h->hgeom, h->hconn, h->hgc3, h->hrcent);
+  std::vector<double> soa((size_t)std::max((int)HARM_G_NFIELD, (int)HARM_NCONN) * np);
+  for (int loc = 0; loc < HARM_NLOC; loc++) {
+    for (long pp = 0; pp < np; pp++)
+      for (int f = 0; f < HARM_G_NFIELD; f++) soa[(size_t)f * np + pp] = h->hgeom[loc][pp * HARM_G_NFIELD + f];
+    HH_TRY(alloc((void**)&h->geom[loc], sizeof(double) * HARM_G_NFIELD * np));
+    HH_TRY(upload(h, h->geom[loc], soa.data(), (size_t)HARM_G_NFIELD * np));
+  }
+  for (long pp = 0; pp < np; pp++)
+    for (int f = 0; f < HARM_NCONN; f++) soa[(size_t)f * np + pp] = h->hconn[pp * HARM_NCONN + f];
+  HH_TRY(alloc((void**)&h->conn, sizeof(double) * HARM_NCONN * np));
+  HH_TRY(upload(h, h->conn, soa.data(), (size_t)HARM_NCONN * np));
+  HH_TRY(alloc((void**)&h->gcorn3, sizeof(double) * np));
+  HH_TRY(upload(h, h->gcorn3, h->hgc3.data(), np));
+  std::vector<double> fl(2 * np);
+  for (int jj = 0; jj < d.ns2; jj++)
+    for (int ii = 0; ii < d.ns1; ii++) {
+      const double r = h->hrcent[ii];
+      fl[(size_t)jj * d.ns1 + ii] = c.rhomin * pow(r, -1.5);
+      fl[np + (size_t)jj * d.ns1 + ii] = c.uumin * pow(r, -2.5);
+    }
+  HH_TRY(alloc((void**)&h->floors, sizeof(double) * 2 * np));
+  HH_TRY(upload(h, h->floors, fl.data(), fl.size()));
+  HH_TRY(alloc((void**)&h->rcent, sizeof(double) * d.ns1));
+  HH_TRY(upload(h, h->rcent, h->hrcent.data(), h->hrcent.size()));
+  return HARM_OK;
+}
+
+harm_status check_handle(const harm_handle* h) { return h ? HARM_OK : hh::set_error(HARM_EINVAL, "null handle"); }
+harm_status check_which(int which)
+{
+  return (which >= 0 && which < HARM_NARR) ? HARM_OK : hh::set_error(HARM_EINVAL, "bad array index %d", which);
+}
+
+// ---- enqueue helpers (no host synchronisation) ----
+harm_status enq_fluxcalc(harm_handle* h, int which_p, int do_ct)
+{
+  const Dims& d = h->d;
+  hipStream_t s = h->stream;
+  const double* pb = h->arr[which_p];
+  hipLaunchKernelGGL(k_scal_begin_flux, dim3(1), dim3(1), 0, s, h->ds);
+  const int recon = h->cfg.recon, ft = h->cfg.flux;
+#define LAUNCH_FLUX(DIR, F, G)                                                                                      \
+  do {                                                                                                              \
+    Box b = face_box(d, DIR);                                                                                       \
+    if (recon == HARM_RECON_WENO5)                                                                                  \
+      hipLaunchKernelGGL((k_flux<DIR, HARM_RECON_WENO5>), dim3(nblocks(b.count())), dim3(kBlock), 0, s, d, b, pb, F, \
+                         G, h->P, ft, do_ct, h->ds);                                                                \
+    else if (recon == HARM_RECON_PARA)                                                                              \
+      hipLaunchKernelGGL((k_flux<DIR, HARM_RECON_PARA>), dim3(nblocks(b.count())), dim3(kBlock), 0, s, d, b, pb, F, \
+                         G, h->P, ft, do_ct, h->ds);                                                                \
+    else                                                                                                            \
+      hipLaunchKernelGGL((k_flux<DIR, HARM_RECON_DONOR>), dim3(nblocks(b.count())), dim3(kBlock), 0, s, d, b, pb,   \
+                         F, G, h->P, ft, do_ct, h->ds);                                                             \
+  } while (0)
+  if (d.n1 > 1) LAUNCH_FLUX(1, h->arr[HARM_ARR_F1], h->geom[HARM_LOC_FACE1]);
+  if (d.n2 > 1) LAUNCH_FLUX(2, h->arr[HARM_ARR_F2], h->geom[HARM_LOC_FACE2]);
+  if (d.n3 > 1) LAUNCH_FLUX(3, h->arr[HARM_ARR_F3], h->geom[HARM_LOC_FACE3]);
+#undef LAUNCH_FLUX
+  HH_TRY(hh::check_launch("k_flux"));
+  if (do_ct) {
+    const int ni = h->emf_ni, nj = h->emf_nj, nk = h->emf_nk;
+    hipLaunchKernelGGL(k_emf, dim3(nblocks((long)ni * nj * nk)), dim3(kBlock), 0, s, d, ni, nj, nk,
+                       h->arr[HARM_ARR_F1], h->arr[HARM_ARR_F2], h->arr[HARM_ARR_F3], h->geom[HARM_LOC_FACE1],
+                       h->geom[HARM_LOC_FACE2], h->gcorn3, h->emf);
+    const int e0 = d.n1 > 1, e1 = d.n2 > 1, e2 = d.n3 > 1;
+    if (e0)
+      hipLaunchKernelGGL(k_ct<1>, dim3(nblocks((long)(d.n1 + 1) * d.n2 * d.n3)), dim3(kBlock), 0, s, d, ni, nj, nk,
+                         h->emf, h->arr[HARM_ARR_F1]);
+    if (e1)
+      hipLaunchKernelGGL(k_ct<2>, dim3(nblocks((long)d.n1 * (d.n2 + 1) * d.n3)), dim3(kBlock), 0, s, d, ni, nj, nk,
+                         h->emf, h->arr[HARM_ARR_F2]);
+    if (e2)
+      hipLaunchKernelGGL(k_ct<3>, dim3(nblocks((long)d.n1 * d.n2 * (d.n3 + 1))), dim3(kBlock), 0, s, d, ni, nj, nk,
+                         h->emf, h->arr[HARM_ARR_F3]);
+    HH_TRY(hh::check_launch("k_emf/k_ct"));
+  }
+  return HARM_OK;
+}
+
+harm_status enq_update(harm_handle* h, int which_pi, int which_pb, const double* CUf, const double* Cunew,
+                       int timeorder)
+{
+  const Dims& d = h->d;
+  hipLaunchKernelGGL(k_update, dim3(nblocks((long)d.n1 * d.n2 * d.n3)), dim3(kBlock), 0, h->stream, d, h->P,
+                     h->arr[which_pi], h->arr[which_pb], h->arr[HARM_ARR_F1], h->arr[HARM_ARR_F2],
+                     h->arr[HARM_ARR_F3], h->arr[HARM_ARR_UI], h->arr[HARM_ARR_UF], h->arr[HARM_ARR_UCUM],
+                     h->geom[HARM_LOC_CENT], h->conn, CUf[0], CUf[1], CUf[2], Cunew[0], Cunew[1], Cunew[2], timeorder,
+                     h->ds);
+  return hh::check_launch("k_update");
+}
+
+harm_status enq_invert(harm_handle* h, int which_pb, int which_pf, int finalstep)
+{
+  const Dims& d = h->d;
+  const size_t bytes = sizeof(double) * HARM_NPR * d.ncell;
+  if (which_pf != which_pb)
+    HH_HIP(hipMemcpyAsync(h->arr[which_pf], h->arr[which_pb], bytes, hipMemcpyDeviceToDevice, h->stream));
+  const double* U = h->arr[finalstep ? HARM_ARR_UCUM : HARM_ARR_UF];
+  hipLaunchKernelGGL(k_invert, dim3(nblocks((long)d.n1 * d.n2 * d.n3)), dim3(kBlock), 0, h->stream, d, h->P, U,
+                     h->arr[which_pf], h->pflag, h->geom[HARM_LOC_CENT], h->floors, finalstep, h->di);
+  return hh::check_launch("k_invert");
+}
+
+harm_status enq_bound(harm_handle* h, int which)
+{
+  const Dims& d = h->d;
+  hipStream_t s = h->stream;
+  double* p = h->arr[which];
+  if (d.n1 > 1)
+    hipLaunchKernelGGL(k_bound_x1, dim3(nblocks(2L * d.n2 * d.n3)), dim3(kBlock), 0, s, d, h->P, p,
+                       h->geom[HARM_LOC_CENT]);
+  if (d.n2 > 1) {
+    for (int side = 0; side < 2; side++) {
+      hipLaunchKernelGGL(k_x2_reflect, dim3(nblocks((long)d.ns1 * d.n3 * d.ng2)), dim3(kBlock), 0, s, d, p, side);
+      hipLaunchKernelGGL(k_x2_poledeath, dim3(nblocks((long)d.ns1 * d.n3)), dim3(kBlock), 0, s, d, h->P, p,
+                         h->geom[HARM_LOC_CENT], h->rcent, side);
+    }
+  }
+  if (d.n3 > 1) hipLaunchKernelGGL(k_bound_x3, dim3(nblocks(d.nplane * 2L * d.ng3)), dim3(kBlock), 0, s, d, p);
+  return hh::check_launch("bounds");
+}
+
+harm_status enq_post_fixup(harm_handle* h, int which)
+{
+  const Dims& d = h->d;
+  HH_HIP(hipMemcpyAsync(h->pa, h->arr[which], sizeof(double) * HARM_NPR * d.ncell, hipMemcpyDeviceToDevice, h->stream));
+  hipLaunchKernelGGL(k_post_fixup, dim3(nblocks((long)d.n1 * d.n2 * d.n3)), dim3(kBlock), 0, h->stream, d, h->P,
+                     h->pa, h->arr[which], h->pflag, h->geom[HARM_LOC_CENT]);
+  return hh::check_launch("k_post_fixup");
+}
+
+int rk_tables(int timeorder, double CUf[4][4], double Cunew[4][4])
+{
+  memset(CUf, 0, sizeof(double) * 16);
+  memset(Cunew, 0, sizeof(double) * 16);
+  if (timeorder == 3) {
+    CUf[0][0] = 1.0; CUf[0][1] = 0.0; CUf[0][2] = 1.0;
+    CUf[1][0] = 3.0 / 4.0; CUf[1][1] = 1.0 / 4.0; CUf[1][2] = 1.0 / 4.0;
+    CUf[2][0] = 1.0 / 3.0; CUf[2][1] = 2.0 / 3.0; CUf[2][2] = 2.0 / 3.0;
+    Cunew[1][3] = 1.0;
+    Cunew[2][2] = 1.0; Cunew[2][3] = 1.0 / 4.0;
+    return 3;
+  }
+  CUf[0][0] = 1.0; CUf[0][1] = 0.0; CUf[0][2] = 0.5;
+  CUf[1][0] = 1.0; CUf[1][1] = 0.0; CUf[1][2] = 1.0;
+  Cunew[1][2] = 1.0; Cunew[1][3] = 0.5;
+  return 2;
+}
+
+// one RK substep of advance() + post_advance, all enqueued
+harm_status enq_substep(harm_handle* h, int to, int nto, const double CUf[4][4], const double Cunew[4][4],
+                        const double* wflux, bool diag)
+{
+  const int pii[3] = {HARM_ARR_P, HARM_ARR_P, HARM_ARR_P};
+  int pbb[3] = {HARM_ARR_P, HARM_ARR_PK0, HARM_ARR_PK1};
+  int pff[3] = {HARM_ARR_PK0, HARM_ARR_PK1, HARM_ARR_P};
+  if (nto == 2) { pbb[1] = HARM_ARR_PK0; pff[1] = HARM_ARR_P; }
+  const int finalstep = (to == nto - 1);
+  HH_TRY(enq_fluxcalc(h, pbb[to], 1));
+  if (diag && h->d.n1 > 1)
+    hipLaunchKernelGGL(k_fluxacc, dim3(1), dim3(kBlock), 0, h->stream, h->d, h->P, h->arr[HARM_ARR_F1], h->ds,
+                       wflux[to]);
+  HH_TRY(enq_update(h, pii[to], pbb[to], CUf[to], Cunew[to], to));
+  HH_TRY(enq_invert(h, pbb[to], pff[to], finalstep));
+  hipLaunchKernelGGL(k_scal_substep_dt, dim3(1), dim3(1), 0, h->stream, h->ds);
+  HH_TRY(enq_bound(h, pff[to]));
+  HH_TRY(enq_post_fixup(h, pff[to]));
+  HH_TRY(enq_bound(h, pff[to]));
+  return hh::check_launch("substep");
+}
+
+void rk_flux_weights(int timeorder, double* w)
+{
+  double CUf[4][4], Cunew[4][4];
+  const int n = rk_tables(timeorder, CUf, Cunew);
+  double C[4][4] = {{0}};
+  for (int s = 0; s < n; s++) {
+    for (int r = 0; r < s; r++) C[s][r] = CUf[s][1] * C[s - 1][r];
+    C[s][s] = CUf[s][2];
+  }
+  for (int r = 0; r < n; r++) w[r] = C[n - 1][r];
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" harm_status harm_create(const harm_config* cfg, harm_handle** out)
+{
+  if (!cfg || !out) return hh::set_error(HARM_EINVAL, "harm_create: null argument");
+  if (cfg->n1 < 1 || cfg->n2 < 1 || cfg->n3 < 1 || cfg->ng < 3)
+    return hh::set_error(HARM_EINVAL, "harm_create: bad sizes n=(%d,%d,%d) ng=%d", cfg->n1, cfg->n2, cfg->n3, cfg->ng);
+  if (cfg->n1 > 1 && cfg->n1 < 3) return hh::set_error(HARM_EINVAL, "harm_create: n1 must be >= 3");
+  if (cfg->n2 > 1 && cfg->n2 < 4) return hh::set_error(HARM_EINVAL, "harm_create: n2 must be >= 4 (pole death)");
+  if (cfg->n3 > 1 && (cfg->n3 % 2 != 0 || cfg->n3 / 2 < cfg->ng))
+    return hh::set_error(HARM_EINVAL, "harm_create: n3 must be even and n3/2 >= ng for the polar rotation");
+  if (cfg->timeorder != 2 && cfg->timeorder != 3) return hh::set_error(HARM_EINVAL, "harm_create: timeorder 2 or 3");
+  if (cfg->recon < HARM_RECON_DONOR || cfg->recon > HARM_RECON_WENO5 || cfg->recon == HARM_RECON_MC)
+    return hh::set_error(HARM_EINVAL, "harm_create: recon %d not supported", cfg->recon);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return hh::set_error(HARM_ENOGPU, "harm_create: no GPU");
+  harm_handle* h = new harm_handle();
+  h->cfg = *cfg;
+  Dims& d = h->d;
+  make_dims(cfg, h->d, h->P);
+  harm_status st = HARM_OK;
+#define TRYC(x)                    \
+  do {                             \
+    st = (x);                      \
+    if (st != HARM_OK) goto fail;  \
+  } while (0)
+  TRYC(hh::check_hip(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "hipStreamCreate"));
+  for (int a = 0; a < HARM_NARR; a++) {
+    TRYC(alloc((void**)&h->arr[a], sizeof(double) * HARM_NPR * d.ncell));
+    TRYC(hh::check_hip(hipMemset(h->arr[a], 0, sizeof(double) * HARM_NPR * d.ncell), "hipMemset"));
+  }
+  TRYC(alloc((void**)&h->pa, sizeof(double) * HARM_NPR * d.ncell));
+  TRYC(alloc((void**)&h->pflag, sizeof(int32_t) * d.ncell));
+  TRYC(hh::check_hip(hipMemset(h->pflag, 0, sizeof(int32_t) * d.ncell), "hipMemset"));
+  h->emf_ni = d.n1 + (d.n1 > 1);
+  h->emf_nj = d.n2 + (d.n2 > 1);
+  h->emf_nk = d.n3 + (d.n3 > 1);
+  TRYC(alloc((void**)&h->emf, sizeof(double) * 3 * (size_t)h->emf_ni * h->emf_nj * h->emf_nk));
+  TRYC(alloc((void**)&h->ds, sizeof(double) * DS_N));
+  TRYC(hh::check_hip(hipMemset(h->ds, 0, sizeof(double) * DS_N), "hipMemset"));
+  TRYC(alloc((void**)&h->di, sizeof(unsigned long long) * DI_N));
+  TRYC(hh::check_hip(hipMemset(h->di, 0, sizeof(unsigned long long) * DI_N), "hipMemset"));
+  TRYC(build_geometry(h));
+#undef TRYC
+  *out = h;
+  return HARM_OK;
+fail:
+  harm_destroy(h);
+  return st;
+}
+
+extern "C" harm_status harm_destroy(harm_handle* h)
+{
+  if (!h) return HARM_OK;
+  // teardown: errors here are not actionable, the handle is gone either way
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  for (int a = 0; a < HARM_NARR; a++) (void)hipFree(h->arr[a]);
+  for (int l = 0; l < HARM_NLOC; l++) (void)hipFree(h->geom[l]);
+  void* bufs[] = {h->pa, h->pflag, h->gcorn3, h->conn, h->floors, h->rcent, h->emf, h->ds, h->di};
+  for (void* b : bufs) (void)hipFree(b);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return HARM_OK;
+}
+
+extern "C" harm_status harm_sync(harm_handle* h)
+{
+  HH_TRY(check_handle(h));
+  return hh::check_hip(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
+}
+
+extern "C" harm_status harm_get_stream(harm_handle* h, void** stream)
+{
+  HH_TRY(check_handle(h));
+  *stream = (void*)h->stream;
+  return HARM_OK;
+}
+
+extern "C" harm_status harm_storage_dims(const harm_handle* h, int32_t dims[3])
+{
+  HH_TRY(check_handle(h));
+  dims[0] = h->d.ns1;
+  dims[1] = h->d.ns2;
+  dims[2] = h->d.ns3;
+  return HARM_OK;
+}
+
+extern "C" harm_status harm_get_geometry(harm_handle* h, int32_t loc, double* out)
+{
+  HH_TRY(check_handle(h));
+  if (loc < 0 || loc >= HARM_NLOC) return hh::set_error(HARM_EINVAL, "bad location %d", loc);
+  memcpy(out, h->hgeom[loc].data(), sizeof(double) * h->hgeom[loc].size());
+  return HARM_OK;
+}
+
+extern "C" harm_status harm_get_connection(harm_handle* h, double* out)
+{
+  HH_TRY(check_handle(h));
+  memcpy(out, h->hconn.data(), sizeof(double) * h->hconn.size());
+  return HARM_OK;
+}
+
+extern "C" harm_status harm_get_corner_gdet(harm_handle* h, double* out)
+{
+  HH_TRY(check_handle(h));
+  memcpy(out, h->hgc3.data(), sizeof(double) * h->hgc3.size());
+  return HARM_OK;
+}
+
+extern "C" harm_status harm_get_coords(harm_handle* h, int32_t i, int32_t j, int32_t k, double X[4], double V[4])
+{
+  HH_TRY(check_handle(h));
+  X[0] = 0.0;
+  X[1] = h->P.startx[1] + (i + 0.5) * h->P.dx[1];
+  X[2] = h->P.startx[2] + (j + 0.5) * h->P.dx[2];
+  X[3] = h->P.startx[3] + (k + 0.5) * h->P.dx[3];
+  hg_mks_V(&h->cfg, X, V);
+  return HARM_OK;
+}
+
+extern "C" harm_status harm_set_array(harm_handle* h, int32_t which, const double* host_aos)
+{
+  HH_TRY(check_handle(h));
+  HH_TRY(check_which(which));
+  const long nc = h->d.ncell;
+  std::vector<double> soa((size_t)HARM_NPR * nc);
+  for (long c = 0; c < nc; c++)
+    for (int pl = 0; pl < HARM_NPR; pl++) soa[(size_t)pl * nc + c] = host_aos[c * HARM_NPR + pl];
+  HH_HIP(hipStreamSynchronize(h->stream));
+  return upload(h, h->arr[which], soa.data(), soa.size());
+}
+
+extern "C" harm_status harm_get_array(harm_handle* h, int32_t which, double* host_aos)
+{
+  HH_TRY(check_handle(h));
+  HH_TRY(check_which(which));
+  const long nc = h->d.ncell;
+  std::vector<double> soa((size_t)HARM_NPR * nc);
+  HH_HIP(hipStreamSynchronize(h->stream));
+  HH_HIP(hipMemcpy(soa.data(), h->arr[which], sizeof(double) * soa.size(), hipMemcpyDeviceToHost));
+  for (long c = 0; c < nc; c++)
+    for (int pl = 0; pl < HARM_NPR; pl++) host_aos[c * HARM_NPR + pl] = soa[(size_t)pl * nc + c];
+  return HARM_OK;
+}
+
+extern "C" harm_status harm_get_pflag(harm_handle* h, int32_t* host_flags)
+{
+  HH_TRY(check_handle(h));
+  HH_HIP(hipStreamSynchronize(h->stream));
+  return hh::check_hip(hipMemcpy(host_flags, h->pflag, sizeof(int32_t) * h->d.ncell, hipMemcpyDeviceToHost),
+                       "get_pflag");
+}
+
+extern "C" harm_status harm_set_time(harm_handle* h, double t, double dt, int64_t nstep)
+{
+  HH_TRY(check_handle(h));
+  (void)nstep;
+  double v[2] = {dt, t};
+  HH_HIP(hipStreamSynchronize(h->stream));
+  return hh::check_hip(hipMemcpy(h->ds, v, sizeof(v), hipMemcpyHostToDevice), "set_time");
+}
+
+extern "C" harm_status harm_get_time(harm_handle* h, double* t, double* dt, int64_t* nstep)
+{
+  HH_TRY(check_handle(h));
+  double v[DS_N];
+  HH_HIP(hipStreamSynchronize(h->stream));
+  HH_HIP(hipMemcpy(v, h->ds, sizeof(v), hipMemcpyDeviceToHost));
+  if (t) *t = v[DS_T];
+  if (dt) *dt = v[DS_DT];
+  if (nstep) *nstep = 0;
+  return HARM_OK;
+}
+
+extern "C" harm_status harm_init_torus(harm_handle* h, double rin, double rmax, double kappa, double beta,
+                                       double randfact, uint32_t seed)
+{
+  HH_TRY(check_handle(h));
+  const Dims& d = h->d;
+  std::vector<double> p((size_t)HARM_NPR * d.ncell, 0.0);
+  hi_grid G;
+  G.ng1 = d.ng1; G.ng2 = d.ng2; G.ng3 = d.ng3;
+  for (int a = 0; a < 4; a++) { G.startx[a] = h->P.startx[a]; G.dx[a] = h->P.dx[a]; }
+  if (hi_init_torus(&h->cfg, &G, rin, rmax, kappa, beta, randfact, seed, p.data()) != 0)
+    return hh::set_error(HARM_EINVAL, "harm_init_torus: no torus zones on this grid");
+  HH_TRY(harm_set_array(h, HARM_ARR_P, p.data()));
+  HH_TRY(enq_bound(h, HARM_ARR_P));
+  return harm_sync(h);
+}
+
+extern "C" harm_status harm_fluxcalc(harm_handle* h, int32_t which_p, int32_t do_ct, double ndt[3])
+{
+  HH_TRY(check_handle(h));
+  HH_TRY(check_which(which_p));
+  HH_TRY(enq_fluxcalc(h, which_p, do_ct));
+  if (ndt) {
+    double v[DS_N];
+    HH_HIP(hipStreamSynchronize(h->stream));
+    HH_HIP(hipMemcpy(v, h->ds, sizeof(v), hipMemcpyDeviceToHost));
+    ndt[0] = v[DS_NDT1];
+    ndt[1] = v[DS_NDT2];
+    ndt[2] = v[DS_NDT3];
+  }
+  return HARM_OK;
+}
+
+extern "C" harm_status harm_update(harm_handle* h, int32_t which_pi, int32_t which_pb, const double CUf[4],
+                                   const double Cunew[4], double dt, int32_t timeorder, double* accdt)
+{
+  HH_TRY(check_handle(h));
+  HH_TRY(check_which(which_pi));
+  HH_TRY(check_which(which_pb));
+  double v[DS_N];
+  HH_HIP(hipStreamSynchronize(h->stream));
+  HH_HIP(hipMemcpy(v, h->ds, sizeof(v), hipMemcpyDeviceToHost));
+  v[DS_DT] = dt;
+  v[DS_ACCDT] = kBIG;
+  HH_HIP(hipMemcpy(h->ds, v, sizeof(v), hipMemcpyHostToDevice));
+  HH_TRY(enq_update(h, which_pi, which_pb, CUf, Cunew, timeorder));
+  if (accdt) {
+    HH_HIP(hipStreamSynchronize(h->stream));
+    HH_HIP(hipMemcpy(v, h->ds, sizeof(v), hipMemcpyDeviceToHost));
+    *accdt = v[DS_ACCDT];
+  }
+  return HARM_OK;
+}
+
+extern "C" harm_status harm_invert_grid(harm_handle* h, int32_t which_pb, int32_t which_pf, int32_t finalstep,
+                                        harm_newton_stats* stats)
+{
+  HH_TRY(check_handle(h));
+  HH_TRY(check_which(which_pb));
+  HH_TRY(check_which(which_pf));
+  unsigned long long z[DI_N] = {0};
+  HH_HIP(hipMemcpyAsync(h->di, z, sizeof(z), hipMemcpyHostToDevice, h->stream));
+  HH_TRY(enq_invert(h, which_pb, which_pf, finalstep));
+  if (stats) {
+    HH_HIP(hipStreamSynchronize(h->stream));
+    HH_HIP(hipMemcpy(z, h->di, sizeof(z), hipMemcpyDeviceToHost));
+    stats->nstroke = (int64_t)z[DI_NSTROKE];
+    stats->nfail = (int64_t)z[DI_NFAIL];
+  }
+  return HARM_OK;
+}
+
+extern "C" harm_status harm_bound(harm_handle* h, int32_t which)
+{
+  HH_TRY(check_handle(h));
+  HH_TRY(check_which(which));
+  return enq_bound(h, which);
+}
+
+extern "C" harm_status harm_post_fixup(harm_handle* h, int32_t which)
+{
+  HH_TRY(check_handle(h));
+  HH_TRY(check_which(which));
+  return enq_post_fixup(h, which);
+}
+
+extern "C" harm_status harm_advance(harm_handle* h, int32_t timeorder, double* ndt)
+{
+  HH_TRY(check_handle(h));
+  double CUf[4][4], Cunew[4][4], w[4];
+  const int nto = rk_tables(h->cfg.timeorder, CUf, Cunew);
+  if (timeorder < 0 || timeorder >= nto) return hh::set_error(HARM_EINVAL, "harm_advance: bad timeorder");
+  rk_flux_weights(h->cfg.timeorder, w);
+  if (timeorder == 0) hipLaunchKernelGGL(k_scal_begin_step, dim3(1), dim3(1), 0, h->stream, h->ds, h->di);
+  HH_TRY(enq_substep(h, timeorder, nto, CUf, Cunew, w, false));
+  if (ndt) {
+    double v[DS_N];
+    HH_HIP(hipStreamSynchronize(h->stream));
+    HH_HIP(hipMemcpy(v, h->ds, sizeof(v), hipMemcpyDeviceToHost));
+    *ndt = v[DS_LASTNDT];
+  }
+  return HARM_OK;
+}
+
+extern "C" harm_status harm_step(harm_handle* h, int32_t nsteps)
+{
+  HH_TRY(check_handle(h));
+  double CUf[4][4], Cunew[4][4], w[4];
+  const int nto = rk_tables(h->cfg.timeorder, CUf, Cunew);
+  rk_flux_weights(h->cfg.timeorder, w);
+  for (int n = 0; n < nsteps; n++) {
+    hipLaunchKernelGGL(k_scal_begin_step, dim3(1), dim3(1), 0, h->stream, h->ds, h->di);
+    for (int to = 0; to < nto; to++) HH_TRY(enq_substep(h, to, nto, CUf, Cunew, w, true));
+    hipLaunchKernelGGL(k_scal_end_step, dim3(1), dim3(1), 0, h->stream, h->ds, h->P.safe);
+  }
+  return hh::check_launch("harm_step");
+}
+
+extern "C" harm_status harm_get_boundary_fluxes(harm_handle* h, double out[4])
+{
+  HH_TRY(check_handle(h));
+  double v[DS_N];
+  HH_HIP(hipStreamSynchronize(h->stream));
+  HH_HIP(hipMemcpy(v, h->ds, sizeof(v), hipMemcpyDeviceToHost));
+  for (int c = 0; c < 4; c++) out[c] = v[DS_FLUXACC + c];
+  return HARM_OK;
+}
+
+extern "C" harm_status harm_host_tables(const harm_config* cfg, double* geom_aos, double* conn_aos,
+                                        double* corner_gdet, double* rcent, double startx[4], double dx[4])
+{
+  if (!cfg) return hh::set_error(HARM_EINVAL, "harm_host_tables: null config");
+  Dims d;
+  Params P;
+  make_dims(cfg, d, P);
+  std::vector<double> g[HARM_NLOC], c, gc3, rc;
+  host_tables(*cfg, d, P, g, c, gc3, rc);
+  if (geom_aos)
+    for (int loc = 0; loc < HARM_NLOC; loc++)
+      memcpy(geom_aos + (size_t)loc * d.nplane * HARM_G_NFIELD, g[loc].data(), sizeof(double) * g[loc].size());
+  if (conn_aos) memcpy(conn_aos, c.data(), sizeof(double) * c.size());
+  if (corner_gdet) memcpy(corner_gdet, gc3.data(), sizeof(double) * gc3.size());
+  if (rcent) memcpy(rcent, rc.data(), sizeof(double) * rc.size());
+  for (int a = 0; a < 4; a++) {
+    if (startx) startx[a] = P.startx[a];
+    if (dx) dx[a] = P.dx[a];
+  }
+  return HARM_OK;
+}
+
+extern "C" harm_status harm_host_init_torus(const harm_config* cfg, double rin, double rmax, double kappa,
+                                            double beta, double randfact, uint32_t seed, double* p_aos)
+{
+  if (!cfg || !p_aos) return hh::set_error(HARM_EINVAL, "harm_host_init_torus: null argument");
+  Dims d;
+  Params P;
+  make_dims(cfg, d, P);
+  hi_grid G;
+  G.ng1 = d.ng1; G.ng2 = d.ng2; G.ng3 = d.ng3;
+  for (int a = 0; a < 4; a++) { G.startx[a] = P.startx[a]; G.dx[a] = P.dx[a]; }
+  memset(p_aos, 0, sizeof(double) * HARM_NPR * d.ncell);
+  if (hi_init_torus(cfg, &G, rin, rmax, kappa, beta, randfact, seed, p_aos) != 0)
+    return hh::set_error(HARM_EINVAL, "harm_host_init_torus: no torus zones on this grid");
+  return HARM_OK;
+}

@@ -1,0 +1,146 @@
+"""Host-side mirror of HARM's grid driver over the C ABI.
+
+Grid owns one harm_handle (the whole domain resident on one GPU).  Arrays cross
+the boundary in HARM's own layout, numpy shape (ns3, ns2, ns1, 8) for
+p[k][j][i][pl] over the ghosted storage box (global.storage.h:320-356).
+"""
+import ctypes
+
+import numpy as np
+
+from . import lib, check, HarmConfig, NewtonStats, ARR, G_NFIELD, NCONN, default_config
+
+
+class Grid:
+    def __init__(self, cfg: HarmConfig):
+        self.cfg = cfg
+        h = ctypes.c_void_p()
+        check(lib().harm_create(ctypes.byref(cfg), ctypes.byref(h)), "harm_create")
+        self.h = h
+        dims = (ctypes.c_int32 * 3)()
+        check(lib().harm_storage_dims(self.h, dims), "harm_storage_dims")
+        self.ns1, self.ns2, self.ns3 = dims[0], dims[1], dims[2]
+        self.ng1 = cfg.ng if cfg.n1 > 1 else 0
+        self.ng2 = cfg.ng if cfg.n2 > 1 else 0
+        self.ng3 = cfg.ng if cfg.n3 > 1 else 0
+
+    @classmethod
+    def torus(cls, n1, n2, n3, **kw):
+        return cls(default_config(n1, n2, n3, **kw))
+
+    # ---- layout helpers ----
+    @property
+    def shape(self):
+        return (self.ns3, self.ns2, self.ns1, 8)
+
+    def active(self, a):
+        """View of the active zones of a (ns3, ns2, ns1, ...) array."""
+        return a[self.ng3:self.ng3 + self.cfg.n3, self.ng2:self.ng2 + self.cfg.n2, self.ng1:self.ng1 + self.cfg.n1]
+
+    def _which(self, which):
+        return ARR[which] if isinstance(which, str) else int(which)
+
+    # ---- state exchange ----
+    def set(self, which, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        assert a.shape == self.shape, (a.shape, self.shape)
+        check(lib().harm_set_array(self.h, self._which(which), a.ctypes.data), "harm_set_array")
+
+    def get(self, which):
+        out = np.empty(self.shape)
+        check(lib().harm_get_array(self.h, self._which(which), out.ctypes.data), "harm_get_array")
+        return out
+
+    def pflag(self):
+        out = np.empty(self.shape[:3], dtype=np.int32)
+        check(lib().harm_get_pflag(self.h, out.ctypes.data), "harm_get_pflag")
+        return out
+
+    def geometry(self, loc):
+        out = np.empty((self.ns2, self.ns1, G_NFIELD))
+        check(lib().harm_get_geometry(self.h, loc, out.ctypes.data), "harm_get_geometry")
+        return out
+
+    def connection(self):
+        out = np.empty((self.ns2, self.ns1, NCONN))
+        check(lib().harm_get_connection(self.h, out.ctypes.data), "harm_get_connection")
+        return out
+
+    def corner_gdet(self):
+        out = np.empty((self.ns2, self.ns1))
+        check(lib().harm_get_corner_gdet(self.h, out.ctypes.data), "harm_get_corner_gdet")
+        return out
+
+    def rcent(self):
+        r = np.empty(self.ns1)
+        X = (ctypes.c_double * 4)()
+        V = (ctypes.c_double * 4)()
+        for ii in range(self.ns1):
+            check(lib().harm_get_coords(self.h, ii - self.ng1, 0, 0, X, V), "harm_get_coords")
+            r[ii] = V[1]
+        return r
+
+    def time(self):
+        t, dt, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
+        check(lib().harm_get_time(self.h, ctypes.byref(t), ctypes.byref(dt), ctypes.byref(n)), "harm_get_time")
+        return t.value, dt.value
+
+    def set_time(self, t, dt):
+        check(lib().harm_set_time(self.h, t, dt, 0), "harm_set_time")
+
+    # ---- problem setup ----
+    def init_torus(self, rin=6.0, rmax=12.0, kappa=1e-3, beta=100.0, randfact=4e-2, seed=1):
+        check(lib().harm_init_torus(self.h, rin, rmax, kappa, beta, randfact, seed), "harm_init_torus")
+
+    # ---- hot path ----
+    def fluxcalc(self, which="p", do_ct=True):
+        ndt = (ctypes.c_double * 3)()
+        check(lib().harm_fluxcalc(self.h, self._which(which), int(do_ct), ndt), "harm_fluxcalc")
+        return list(ndt)
+
+    def update(self, which_pi, which_pb, CUf, Cunew, dt, timeorder):
+        c1 = (ctypes.c_double * 4)(*CUf)
+        c2 = (ctypes.c_double * 4)(*Cunew)
+        acc = ctypes.c_double()
+        check(lib().harm_update(self.h, self._which(which_pi), self._which(which_pb), c1, c2, dt, timeorder,
+                                ctypes.byref(acc)), "harm_update")
+        return acc.value
+
+    def invert(self, which_pb, which_pf, finalstep):
+        st = NewtonStats()
+        check(lib().harm_invert_grid(self.h, self._which(which_pb), self._which(which_pf), int(finalstep),
+                                     ctypes.byref(st)), "harm_invert_grid")
+        return st.nstroke, st.nfail
+
+    def bound(self, which="p"):
+        check(lib().harm_bound(self.h, self._which(which)), "harm_bound")
+
+    def post_fixup(self, which="p"):
+        check(lib().harm_post_fixup(self.h, self._which(which)), "harm_post_fixup")
+
+    def step(self, nsteps=1):
+        check(lib().harm_step(self.h, nsteps), "harm_step")
+
+    def sync(self):
+        check(lib().harm_sync(self.h), "harm_sync")
+
+    def stream(self):
+        s = ctypes.c_void_p()
+        check(lib().harm_get_stream(self.h, ctypes.byref(s)), "harm_get_stream")
+        return s.value
+
+    def boundary_fluxes(self):
+        out = (ctypes.c_double * 4)()
+        check(lib().harm_get_boundary_fluxes(self.h, out), "harm_get_boundary_fluxes")
+        return list(out)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().harm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

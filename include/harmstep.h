@@ -117,6 +117,9 @@ harm_status harm_storage_dims(const harm_handle* h, int32_t dims[3]);
  * over the 2-D storage plane (ns1 x ns2); conn out[(j*ns1+i)*40 + c]. */
 harm_status harm_get_geometry(harm_handle* h, int32_t loc, double* out);
 harm_status harm_get_connection(harm_handle* h, double* out);
+/* gdet at the (i-1/2, j-1/2) corners (CORN3), out[j*ns1+i]; the Toth EMFs are
+ * multiplied by the corner gdet (fluxct.c:186, CORNGDETVERSION). */
+harm_status harm_get_corner_gdet(harm_handle* h, double* out);
 /* MKS coordinates (X1..X3) and BL radius/theta of a zone centre. */
 harm_status harm_get_coords(harm_handle* h, int32_t i, int32_t j, int32_t k, double X[4], double V[4]);
 
@@ -146,6 +149,16 @@ harm_status harm_get_time(harm_handle* h, double* t, double* dt, int64_t* nstep)
  * init.tools.c:295-894).  Leaves p bounded and sets dt from set_dt(). */
 harm_status harm_init_torus(harm_handle* h, double rin, double rmax, double kappa,
                             double beta, double randfact, uint32_t seed);
+
+/* Host-only forms (no GPU needed): the geometry tables harm_create uploads
+ * (geom_aos[loc][j][i][27] for the 4 locations, conn_aos[j][i][40],
+ * corner_gdet[j][i], rcent[i], grid origin and spacing in X), and the torus
+ * initial data on the active zones of a HARM-layout host array (ghosts zero,
+ * unbounded). */
+harm_status harm_host_tables(const harm_config* cfg, double* geom_aos, double* conn_aos, double* corner_gdet,
+                             double* rcent, double startx[4], double dx[4]);
+harm_status harm_host_init_torus(const harm_config* cfg, double rin, double rmax, double kappa, double beta,
+                                 double randfact, uint32_t seed, double* p_aos);
 
 /* ------------------------------------------------------------------------ */
 /* Hot path                                                                  */

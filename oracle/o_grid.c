@@ -1,0 +1,867 @@
+/*
+ * o_grid.c -- oracle restatement of HARM's advance() hot path on a grid:
+ * boundary conditions, reconstruction (PARA, WENO5 core, donor), face Riemann
+ * fluxes (LAX-F, HLL), Toth flux-CT, the conserved update with connection
+ * source terms, the inversion with floors, failure averaging, the timestep
+ * and the RK2/RK3 step driver.  TEST INFRASTRUCTURE ONLY (see oracle.h).
+ *
+ * Reference files restated (HARM tree): flux.c:852-1140 (fluxcalc_standard),
+ * flux.c:1952-2110 (getplpr, check_plpr), interppoint.para.c:568-1200 (para4),
+ * reconstructeno.c:1693-2030, 2211-2330, 3015 (WENO weights), fluxcompute.c
+ * (flux_compute, HLL/LAXF), wavespeeds.c:397 (cminmax_calc), fluxct.c:186-453,
+ * 1106-1161 (Toth), advance.c:140-865, 1866-2075, 2331-2620 (update, dt),
+ * fixup.c:561-830, 1061-1330, 2087-2200, 2723-2850, 3255-3390, bounds.tools.c
+ * :172-1215, 1417-2010, 2004-2560 (extrapolation, polar axis, pole death),
+ * step_ch.c:140-235, 524-845, 1171-1310 (RK driver).
+ */
+#include <math.h>
+#include <float.h>
+#include <stdlib.h>
+#include <string.h>
+#include "o_grid.h"
+
+#define SMALL 1.e-300
+#define BIG 1.e+100
+#define NPR 8
+
+
+
+static inline double minmod(double a, double b) { return ((a) * (b) <= 0) ? 0.0 : ((fabs(a) < fabs(b)) ? (a) : (b)); }
+
+void o_dims(const o_grid* g, int* ng1, int* ng2, int* ng3, int* ns1, int* ns2, int* ns3)
+{
+  *ng1 = g->n1 > 1 ? g->ng : 0;
+  *ng2 = g->n2 > 1 ? g->ng : 0;
+  *ng3 = g->n3 > 1 ? g->ng : 0;
+  *ns1 = g->n1 + 2 * (*ng1);
+  *ns2 = g->n2 + 2 * (*ng2);
+  *ns3 = g->n3 + 2 * (*ng3);
+}
+
+typedef struct {
+  int n1, n2, n3, ng1, ng2, ng3, ns1, ns2, ns3;
+} dims_t;
+
+static dims_t D(const o_grid* g)
+{
+  dims_t d;
+  d.n1 = g->n1; d.n2 = g->n2; d.n3 = g->n3;
+  o_dims(g, &d.ng1, &d.ng2, &d.ng3, &d.ns1, &d.ns2, &d.ns3);
+  return d;
+}
+
+static inline long ix(const dims_t* d, int i, int j, int k)
+{
+  return (((long)(k + d->ng3) * d->ns2 + (j + d->ng2)) * d->ns1 + (i + d->ng1)) * NPR;
+}
+static inline long ig2(const dims_t* d, int i, int j) { return (long)(j + d->ng2) * d->ns1 + (i + d->ng1); }
+
+static void geom_at(const o_grid* g, const dims_t* d, int loc, int i, int j, o_geom* G)
+{
+  o_geom_from_rec(g->geom[loc] + ig2(d, i, j) * 27, G);
+}
+
+static inline double igdetnosing(double gdet) { return (gdet >= 0.0 ? 1.0 : -1.0) / (fabs(gdet) + SMALL); }
+
+/* gamma_calc (phys.tools.c:1975) for VELREL4 */
+static double gamma_of(const double* pr, const o_geom* G)
+{
+  const double u1 = pr[O_U1], u2 = pr[O_U2], u3 = pr[O_U3];
+  double qsq = G->gcov[1][1] * u1 * u1 + G->gcov[2][2] * u2 * u2 + G->gcov[3][3] * u3 * u3
+               + 2. * (G->gcov[1][2] * u1 * u2 + G->gcov[1][3] * u1 * u3);
+  qsq = qsq * (qsq >= 0.0);
+  return sqrt(1. + qsq);
+}
+
+/* limit_gamma (fixup.c:2723), DO_CONSERVE_D=0 */
+static void limit_gamma(double gammamax, double* pr, const o_geom* G)
+{
+  if (gammamax <= 1.0) return;
+  const double gamma = gamma_of(pr, G);
+  if ((gamma > gammamax && (gamma != 1.0))) {
+    const double pref = (gammamax * gammamax - 1.) / (gamma * gamma - 1.);
+    const double f = sqrt(pref);
+    pr[O_U1] *= f;
+    pr[O_U2] *= f;
+    pr[O_U3] *= f;
+  }
+}
+
+/* ---------------------------------------------------------------------- */
+/* Boundary conditions                                                    */
+/* ---------------------------------------------------------------------- */
+
+/* extrapfunc (bounds.tools.c:1417, HORIZONEXTRAP=OUTEREXTRAP=3) for one (j,k) line */
+static void extrap_line(const o_grid* g, const dims_t* d, double* p, int up, int j, int k)
+{
+  const int ri = up ? d->n1 - 1 : 0;
+  const int ri2 = up ? ri - 1 : ri + 1;
+  const int ri3 = up ? ri - 2 : ri + 2;
+  const double signdq = up ? -1.0 : 1.0;
+  o_geom Gr, Gr2, Gr3;
+  geom_at(g, d, 0, ri, j, &Gr);
+  geom_at(g, d, 0, ri2, j, &Gr2);
+  geom_at(g, d, 0, ri3, j, &Gr3);
+  const double* pr = p + ix(d, ri, j, k);
+  const double* pr2 = p + ix(d, ri2, j, k);
+  const double* pr3 = p + ix(d, ri3, j, k);
+  double Bd3 = 0.0, Bd3ri2 = 0.0, Bd3ri3 = 0.0;
+  for (int jj = 1; jj < 4; jj++) Bd3 += pr[O_B1 + jj - 1] * Gr.gcov[3][jj];
+  for (int jj = 1; jj < 4; jj++) Bd3ri2 += pr2[O_B1 + jj - 1] * Gr2.gcov[3][jj];
+  for (int jj = 1; jj < 4; jj++) Bd3ri3 += pr3[O_B1 + jj - 1] * Gr3.gcov[3][jj];
+  double Dqp = Bd3ri3 - Bd3ri2, Dqm = Bd3ri2 - Bd3, Dqc = Bd3ri3 - Bd3;
+  const double dqBd3 = signdq * minmod(minmod(Dqp, Dqm), Dqc);
+  o_state qr, qr2;
+  o_get_state(pr, &Gr, g->gam, &qr);
+  o_get_state(pr2, &Gr2, g->gam, &qr2);
+  double dq[NPR], dql[NPR], dqg[NPR];
+  for (int pl = 0; pl < NPR; pl++) {
+    Dqp = pr3[pl] - pr2[pl];
+    Dqm = pr2[pl] - pr[pl];
+    Dqc = pr3[pl] - pr[pl];
+    dq[pl] = signdq * minmod(minmod(Dqp, Dqm), Dqc);
+  }
+  for (int pl = 0; pl < NPR; pl++) {
+    Dqp = log(SMALL + fabs(pr3[pl])) - log(SMALL + fabs(pr2[pl]));
+    Dqm = log(SMALL + fabs(pr2[pl])) - log(SMALL + fabs(pr[pl]));
+    Dqc = log(SMALL + fabs(pr3[pl])) - log(SMALL + fabs(pr[pl]));
+    dql[pl] = signdq * minmod(minmod(Dqp, Dqm), Dqc);
+  }
+  for (int pl = 0; pl < NPR; pl++) {
+    Dqp = (Gr3.gdet) * pr3[pl] - (Gr2.gdet) * pr2[pl];
+    Dqm = (Gr2.gdet) * pr2[pl] - (Gr.gdet) * pr[pl];
+    Dqc = (Gr3.gdet) * pr3[pl] - (Gr.gdet) * pr[pl];
+    dqg[pl] = signdq * minmod(minmod(Dqp, Dqm), Dqc);
+  }
+  const double xdqfrac = fabs(qr.ucon[0] - qr2.ucon[0]) / qr.ucon[0];
+  double ydqfrac;
+  if (xdqfrac < 0.1) ydqfrac = 0.0;
+  else if (xdqfrac < 0.2) ydqfrac = 0.0 + (1.0 - 0.0) / (0.2 - 0.1) * (xdqfrac - 0.1);
+  else ydqfrac = 1.0;
+  for (int jj = 0; jj < 4; jj++) {
+    dq[O_UU + jj] = dq[O_UU + jj] * (1.0 - ydqfrac);
+    dql[O_UU + jj] = dql[O_UU + jj] * (1.0 - ydqfrac);
+    dqg[O_UU + jj] = dqg[O_UU + jj] * (1.0 - ydqfrac);
+  }
+  const int is = up ? d->n1 : -d->ng1;
+  const int ie = up ? d->n1 + d->ng1 - 1 : -1;
+  for (int i = is; i <= ie; i++) {
+    double* pi = p + ix(d, i, j, k);
+    for (int pl = 0; pl < NPR; pl++) pi[pl] = pr[pl];
+    o_geom Gi;
+    geom_at(g, d, 0, i, j, &Gi);
+    const double di = (double)(i - ri);
+    for (int pl = 0; pl <= O_UU; pl++) {
+      const double ftemp = exp(-signdq * dql[pl]) - 1.1;
+      double mydqlog;
+      if (ftemp >= 0.0) mydqlog = log(1.1) / (-signdq);
+      else mydqlog = dql[pl];
+      pi[pl] = exp(log(SMALL + fabs(pr[pl])) + mydqlog * di);
+    }
+    for (int pl = O_U1; pl <= O_U3; pl++) pi[pl] = pr[pl] + dq[pl] * di;
+    for (int pl = O_B1; pl <= O_B2; pl++) {
+      const double ign = igdetnosing(Gi.gdet);
+      pi[pl] = (pr[pl] * fabs((Gr.gdet) + dqg[pl] * di) * ign);
+    }
+    const double Bu1 = pi[O_B1], Bu2 = pi[O_B2];
+    const double gcon03 = Gi.gcon[0][3], gcon13 = Gi.gcon[1][3], gcon23 = Gi.gcon[2][3], gcon33 = Gi.gcon[3][3];
+    const double gcov01 = Gi.gcov[0][1], gcov02 = Gi.gcov[0][2], gcov11 = Gi.gcov[1][1];
+    const double gcov12 = Gi.gcov[1][2], gcov21 = gcov12, gcov22 = Gi.gcov[2][2];
+    const double gcov03 = Gi.gcov[0][3], gcov13 = Gi.gcov[1][3], gcov23 = Gi.gcov[2][3];
+    const double myBd3 = Bd3 + dqBd3 * di;
+    const double ftemp = (1.0 - gcon03 * gcov03 - gcon13 * gcov13 - gcon23 * gcov23);
+    const double ign = igdetnosing(ftemp);
+    pi[O_B3] = (myBd3 * gcon33 + Bu1 * gcon03 * gcov01 + Bu2 * gcon03 * gcov02 + Bu1 * gcon13 * gcov11
+                + Bu2 * gcon13 * gcov12 + Bu1 * gcon23 * gcov21 + Bu2 * gcon23 * gcov22) * ign;
+  }
+}
+
+/* inflow_check_rel4vel (fixup.c:3255) for dir 1 at the outer boundary */
+static void inflow_check_x1up(double* pr, const o_geom* G, double gam, double gammamax)
+{
+  o_state q;
+  o_get_state(pr, G, gam, &q);
+  if (!(q.ucon[1] < 0.)) return;
+  const double gamma = gamma_of(pr, G);
+  pr[O_U1] /= gamma;
+  pr[O_U2] /= gamma;
+  pr[O_U3] /= gamma;
+  const double alpha = 1. / sqrt(-G->gcon[0][0]);
+  const double betacon = G->gcon[0][1] * alpha * alpha;
+  pr[O_U1] = betacon / alpha;
+  double vsq = 0.;
+  for (int j = 1; j < 4; j++)
+    for (int k = 1; k < 4; k++) vsq += G->gcov[j][k] * pr[O_U1 + j - 1] * pr[O_U1 + k - 1];
+  if (vsq < 0.0) vsq = 0.0;
+  if (vsq >= 1.0) vsq = 1.0 - 1.0 / (gammamax * gammamax);
+  const double gnew = 1. / sqrt(1. - vsq);
+  pr[O_U1] *= gnew;
+  pr[O_U2] *= gnew;
+  pr[O_U3] *= gnew;
+}
+
+static void bound_x1(const o_grid* g, const dims_t* d, double* p)
+{
+#pragma omp parallel for collapse(2) num_threads(g->nthreads > 0 ? g->nthreads : 1)
+  for (int k = 0; k < d->n3; k++)
+    for (int j = 0; j < d->n2; j++) {
+      extrap_line(g, d, p, 0, j, k); /* X1DN: FREEOUTFLOW */
+      extrap_line(g, d, p, 1, j, k); /* X1UP: OUTFLOW + inflow check */
+      for (int i = d->n1; i < d->n1 + d->ng1; i++) {
+        o_geom Gi;
+        geom_at(g, d, 0, i, j, &Gi);
+        double* pi = p + ix(d, i, j, k);
+        inflow_check_x1up(pi, &Gi, g->gam, g->gammamax);
+        limit_gamma(g->gammamax, pi, &Gi);
+      }
+    }
+}
+
+/* bound_x2dn/x2up_polaraxis (bounds.tools.c:751, :996 and the _full3d forms
+ * with the pi rotation in phi when N3>1): each pole is a reflection loop over
+ * all (i,k) columns followed by poledeath (:2004) over all columns, with
+ * POLEDEATH=1, POLEINTERPTYPE=3, POLEGAMMADEATH=1. */
+static void x2_reflect(const o_grid* g, const dims_t* d, double* p, int side)
+{
+  const int n2 = d->n2, n3 = d->n3;
+#pragma omp parallel for collapse(2) num_threads(g->nthreads > 0 ? g->nthreads : 1)
+  for (int k = 0; k < n3; k++)
+    for (int i = -d->ng1; i < d->n1 + d->ng1; i++) {
+      const int rk = (n3 > 1) ? (k + n3 / 2) % n3 : k;
+      const int js = side == 0 ? -d->ng2 : n2, je = side == 0 ? -1 : n2 + d->ng2 - 1;
+      for (int j = js; j <= je; j++) {
+        const int rj = side == 0 ? -j - 1 : 2 * (n2 - 1) - j + 1;
+        const double* src = p + ix(d, i, rj, rk);
+        double* dst = p + ix(d, i, j, k);
+        for (int pl = 0; pl < NPR; pl++) dst[pl] = src[pl];
+        dst[O_U2] *= -1.;
+        dst[O_B2] *= -1.;
+      }
+    }
+}
+
+static void x2_poledeath(const o_grid* g, const dims_t* d, double* p, int side)
+{
+  const int n2 = d->n2, n3 = d->n3;
+  const double x2c0 = g->startx[2];
+#pragma omp parallel for collapse(2) num_threads(g->nthreads > 0 ? g->nthreads : 1)
+  for (int k = 0; k < n3; k++)
+    for (int i = -d->ng1; i < d->n1 + d->ng1; i++) {
+      const int rj = side == 0 ? 1 : n2 - 2;
+      const int js = side == 0 ? -1 : n2 - 1;
+      const int je = side == 0 ? 0 : n2;
+      const double X0 = side == 0 ? x2c0 : x2c0 + n2 * g->dx[2];
+      const double Xr = x2c0 + (rj + 0.5) * g->dx[2];
+      const double* pref = p + ix(d, i, rj, k);
+      for (int j = js; j <= je; j++) {
+        double* pj = p + ix(d, i, j, k);
+        const double X = x2c0 + (j + 0.5) * g->dx[2];
+        for (int pl = O_RHO; pl <= O_U3; pl++) {
+          if (pl == O_U2) continue;
+          pj[pl] = pref[pl];
+        }
+        double ftemp = pref[O_U2];
+        if ((side == 0 && ftemp > 0.0) || (side == 1 && ftemp < 0.0)) ftemp = 0.0;
+        pj[O_U2] = ftemp + (X - Xr) * (ftemp - 0.0) / (Xr - X0);
+        if (n3 == 1) pj[O_B3] = pref[O_B3];
+      }
+      for (int j = js; j <= je; j++) {
+        double* pj = p + ix(d, i, j, k);
+        o_geom Gj;
+        geom_at(g, d, 0, i, j, &Gj);
+        o_state q;
+        o_get_state(pj, &Gj, g->gam, &q);
+        if (q.ucon[1] > 0.0) {
+          const double r = g->rcent[i + d->ng1];
+          double lim;
+          if (r <= 10.0) lim = 1.1;
+          else lim = 1.1 * pow(r / 10.0, 1.0);
+          limit_gamma(lim, pj, &Gj);
+        } else {
+          limit_gamma(g->gammamax, pj, &Gj);
+        }
+      }
+    }
+}
+
+static void bound_x2(const o_grid* g, const dims_t* d, double* p)
+{
+  x2_reflect(g, d, p, 0);
+  x2_poledeath(g, d, p, 0);
+  x2_reflect(g, d, p, 1);
+  x2_poledeath(g, d, p, 1);
+}
+
+static void bound_x3(const o_grid* g, const dims_t* d, double* p)
+{
+  if (d->n3 == 1) return;
+#pragma omp parallel for collapse(2) num_threads(g->nthreads > 0 ? g->nthreads : 1)
+  for (int j = -d->ng2; j < d->n2 + d->ng2; j++)
+    for (int i = -d->ng1; i < d->n1 + d->ng1; i++) {
+      for (int k = -d->ng3; k < 0; k++) memcpy(p + ix(d, i, j, k), p + ix(d, i, j, k + d->n3), sizeof(double) * NPR);
+      for (int k = d->n3; k < d->n3 + d->ng3; k++)
+        memcpy(p + ix(d, i, j, k), p + ix(d, i, j, k - d->n3), sizeof(double) * NPR);
+    }
+}
+
+void o_bound(const o_grid* g, double* p)
+{
+  dims_t d = D(g);
+  if (d.n1 > 1) bound_x1(g, &d, p);
+  if (d.n2 > 1) bound_x2(g, &d, p);
+  bound_x3(g, &d, p);
+}
+
+/* ---------------------------------------------------------------------- */
+/* Reconstruction                                                          */
+/* ---------------------------------------------------------------------- */
+
+/* para4 (interppoint.para.c:568) with PARA2LIM=MC, no smoothing/steepening */
+static void para4(const double* y, double* lout, double* rout)
+{
+  double dq1[4], dq2[3], dq[3];
+  /* dq1[m+1] = y[m]-y[m-1], m=-1..2 ; dq2[m+1] = (y[m+1]-y[m-1])/2, m=-1..1 */
+  for (int m = -1; m <= 1; m++) {
+    dq1[m + 1] = (y[m] - y[m - 1]);
+    dq2[m + 1] = 0.5 * (y[m + 1] - y[m - 1]);
+  }
+  dq1[3] = (y[2] - y[1]);
+  for (int m = -1; m <= 1; m++) {
+    const double Dqm = 2.0 * dq1[m + 1];
+    const double Dqp = 2.0 * dq1[m + 2];
+    const double Dqc = dq2[m + 1];
+    dq[m + 1] = minmod(Dqc, minmod(Dqm, Dqp));
+  }
+  const double ddql = (dq[1] - dq[0]);
+  const double ddqr = (dq[2] - dq[1]);
+  double l = 0.5 * (y[0] + y[-1]) - ddql * (1.0 / 8.0);
+  double r = 0.5 * (y[1] + y[0]) - ddqr * (1.0 / 8.0);
+  /* checkparamonotonicity, a6COEF = 4 */
+  const double qa = (r - y[0]) * (y[0] - l);
+  const double qd = (r - l);
+  const double qe = 4.0 * (y[0] - 0.5 * (l + r));
+  if (qa <= 0.0) {
+    l = y[0];
+    r = y[0];
+  } else {
+    if (qd * (qd - qe) < 0.0) l = (-(2.0 + 4.0) * r + 2.0 * 4.0 * y[0]) / (4.0 - 2.0);
+    else if (qd * (qd + qe) < 0.0) r = (-(2.0 + 4.0) * l + 2.0 * 4.0 * y[0]) / (4.0 - 2.0);
+  }
+  *lout = l;
+  *rout = r;
+}
+
+/* symmetric pairwise sum (get_sum_of_elements, SYMMETRIZE_WENO) */
+static double symsum(int n, const double* a)
+{
+  double s = 0.0;
+  int i;
+  for (i = 0; i < n / 2; i++) s += a[i] + a[n - 1 - i];
+  s += a[i] * (n % 2);
+  return s;
+}
+
+/* limit_weight (reconstructeno.c:3015) for order 3 */
+static double limit_weight3(double w)
+{
+  const double avg = 1. / 3.;
+  const double w1 = 0.2 * avg, w2 = (1.0 - 0.2) * avg, w3 = (1.0 + 0.2) * avg, w4 = 1.0;
+  if (w < w1) return 0.0;
+  if (w < w2) return avg * (w - w1) / (w2 - w1);
+  if (w < w3) return avg;
+  return (w - w3) * (w4 - avg) / (w4 - w3) + avg;
+}
+
+/* WENO5 c2e core: Jiang-Shu indicators on the three 3-point sub-stencils,
+ * value-proportional epsilon, normalised indicators, limited weights, point
+ * value sub-stencil interpolants at x -+ 1/2. */
+static void weno5(const double* u, double* lout, double* rout)
+{
+  const double EPS = (500.0 * DBL_EPSILON) * (500.0 * DBL_EPSILON);
+  double b[3], d1[3], d2[3];
+  d1[0] = -(3 * u[0] + u[2]) + 4 * u[1];
+  d1[1] = u[1] - u[-1];
+  d1[2] = (3 * u[0] + u[-2]) - 4 * u[-1];
+  d2[0] = (u[0] + u[2]) - 2 * u[1];
+  d2[1] = (u[-1] + u[1]) - 2 * u[0];
+  d2[2] = (u[0] + u[-2]) - 2 * u[-1];
+  for (int s = 0; s < 3; s++) b[s] = 13. / 12. * d2[s] * d2[s] + 1. / 4. * d1[s] * d1[s];
+  double u2[5];
+  for (int i = -2; i <= 2; i++) u2[i + 2] = u[i] * u[i];
+  const double err = symsum(5, u2) * EPS;
+  for (int s = 0; s < 3; s++) b[s] += err;
+  double sb = symsum(3, b);
+  for (int s = 0; s < 3; s++) b[s] /= (sb + DBL_MIN);
+  double w[3];
+  for (int s = 0; s < 3; s++) {
+    double t = b[s] + EPS / 1.;
+    t *= t;
+    w[s] = 1. / t;
+  }
+  double sw = symsum(3, w);
+  for (int s = 0; s < 3; s++) w[s] /= (sw + 0);
+  for (int s = 0; s < 3; s++) w[s] = limit_weight3(w[s]);
+  sw = symsum(3, w);
+  for (int s = 0; s < 3; s++) w[s] /= (sw + 0);
+  /* optimal weights: right face {5/16, 5/8, 1/16}, left face mirrored */
+  double wr[3] = {w[0] * (5.0 / 16.0), w[1] * (5.0 / 8.0), w[2] * (1.0 / 16.0)};
+  double wl[3] = {w[0] * (1.0 / 16.0), w[1] * (5.0 / 8.0), w[2] * (5.0 / 16.0)};
+  sw = symsum(3, wr);
+  for (int s = 0; s < 3; s++) wr[s] /= (sw + DBL_MIN);
+  sw = symsum(3, wl);
+  for (int s = 0; s < 3; s++) wl[s] /= (sw + DBL_MIN);
+  const double qr0 = (3 * u[0] + 6 * u[1] - u[2]) / 8.0;
+  const double qr1 = (-u[-1] + 6 * u[0] + 3 * u[1]) / 8.0;
+  const double qr2 = (3 * u[-2] - 10 * u[-1] + 15 * u[0]) / 8.0;
+  const double ql0 = (15 * u[0] - 10 * u[1] + 3 * u[2]) / 8.0;
+  const double ql1 = (3 * u[-1] + 6 * u[0] - u[1]) / 8.0;
+  const double ql2 = (-u[-2] + 6 * u[-1] + 3 * u[0]) / 8.0;
+  *rout = (wr[0] * qr0 + wr[2] * qr2) + wr[1] * qr1;
+  *lout = (wl[0] * ql0 + wl[2] * ql2) + wl[1] * ql1;
+}
+
+static void recon_cell(int recon, const double* y, double* l, double* r)
+{
+  if (recon == 2) para4(y, l, r);
+  else if (recon == 3) weno5(y, l, r);
+  else { *l = y[0]; *r = y[0]; }
+}
+
+/* ---------------------------------------------------------------------- */
+/* Fluxes                                                                  */
+/* ---------------------------------------------------------------------- */
+
+static void face_flux(const o_grid* g, int dir, const o_geom* G, double* p_l, double* p_r, double* F, double* ctop)
+{
+  /* getplpr: BDIRCONT, then check_plpr: limit_gamma on both states */
+  const int pb = O_B1 + dir - 1;
+  p_l[pb] = p_r[pb] = 0.5 * (p_l[pb] + p_r[pb]);
+  limit_gamma(g->gammamax, p_l, G);
+  limit_gamma(g->gammamax, p_r, G);
+  o_state ql, qr;
+  double F_l[NPR], F_r[NPR], U_l[NPR], U_r[NPR];
+  o_get_state(p_l, G, g->gam, &ql);
+  o_primtoflux(p_l, &ql, dir, G, F_l);
+  o_primtoflux(p_l, &ql, 0, G, U_l);
+  o_get_state(p_r, G, g->gam, &qr);
+  o_primtoflux(p_r, &qr, dir, G, F_r);
+  o_primtoflux(p_r, &qr, 0, G, U_r);
+  double cmaxl, cminl, cmaxr, cminr;
+  o_vchar(p_l, &ql, dir, G, g->gam, &cmaxl, &cminl);
+  o_vchar(p_r, &qr, dir, G, g->gam, &cmaxr, &cminr);
+  double lmin = fmin(cminl, cminr), lmax = fmax(cmaxl, cmaxr);
+  lmin = fabs(fmax(0., -lmin));
+  lmax = fabs(fmax(0., lmax));
+  const double cmin = lmin, cmax = lmax, ct = fmax(lmax, lmin);
+  *ctop = ct;
+  if (g->flux == 1 && cmax + cmin != 0.0) {
+    for (int pl = 0; pl < NPR; pl++)
+      F[pl] = ((cmax * F_l[pl] + cmin * F_r[pl]) - (cmax * cmin) * (U_r[pl] - U_l[pl])) / (cmax + cmin);
+  } else {
+    for (int pl = 0; pl < NPR; pl++) F[pl] = (0.5 * ((F_l[pl] + F_r[pl]) - ct * (U_r[pl] - U_l[pl])));
+  }
+}
+
+static void flux_dir(const o_grid* g, const dims_t* d, int dir, const double* pb, double* F, double* ndt)
+{
+  const int e1 = d->n2 > 1, e2 = d->n3 > 1, e0 = d->n1 > 1;
+  int is = 0, ie = d->n1 - 1, js = 0, je = d->n2 - 1, ks = 0, ke = d->n3 - 1;
+  if (dir == 1) { ie = d->n1; js -= e1; je += e1; ks -= e2; ke += e2; }
+  if (dir == 2) { je = d->n2; is -= e0; ie += e0; ks -= e2; ke += e2; }
+  if (dir == 3) { ke = d->n3; is -= e0; ie += e0; js -= e1; je += e1; }
+  const int di = dir == 1, dj = dir == 2, dk = dir == 3;
+  const long st = (dir == 1) ? NPR : (dir == 2) ? (long)d->ns1 * NPR : (long)d->ns1 * d->ns2 * NPR;
+  double mindt = BIG;
+#pragma omp parallel for collapse(3) reduction(min : mindt) num_threads(g->nthreads > 0 ? g->nthreads : 1)
+  for (int k = ks; k <= ke; k++)
+    for (int j = js; j <= je; j++)
+      for (int i = is; i <= ie; i++) {
+        o_geom G;
+        geom_at(g, d, dir, i, j, &G);
+        const double* c = pb + ix(d, i, j, k);          /* cell on the right of the face */
+        const double* cm = pb + ix(d, i - di, j - dj, k - dk); /* cell on the left */
+        double p_l[NPR], p_r[NPR];
+        for (int pl = 0; pl < NPR; pl++) {
+          double y[5], l, r;
+          for (int m = -2; m <= 2; m++) y[m + 2] = cm[pl + m * st];
+          recon_cell(g->recon, y + 2, &l, &r);
+          p_l[pl] = r; /* right edge of the left cell */
+          for (int m = -2; m <= 2; m++) y[m + 2] = c[pl + m * st];
+          recon_cell(g->recon, y + 2, &l, &r);
+          p_r[pl] = l; /* left edge of the right cell */
+        }
+        double ctop;
+        double* Fo = F + ix(d, i, j, k);
+        face_flux(g, dir, &G, p_l, p_r, Fo, &ctop);
+        const double dtij = g->cour * g->dx[dir] / ctop;
+        if (dtij < mindt) mindt = dtij;
+      }
+  *ndt = mindt;
+}
+
+/* flux_ct (fluxct.c:186) with FLUXB=FLUXCTTOTH, CORNGDETVERSION=1 */
+static void flux_ct(const o_grid* g, const dims_t* d, double* F1, double* F2, double* F3)
+{
+  const int N1 = d->n1, N2 = d->n2, N3 = d->n3;
+  const int e0 = N1 > 1, e1 = N2 > 1, e2 = N3 > 1;
+  double* Fs[4] = {NULL, F1, F2, F3};
+  /* divide face B fluxes by the face gdet over all computed faces */
+  for (int dir = 1; dir <= 3; dir++) {
+    if ((dir == 1 && !e0) || (dir == 2 && !e1) || (dir == 3 && !e2)) continue;
+    int is = 0, ie = N1 - 1, js = 0, je = N2 - 1, ks = 0, ke = N3 - 1;
+    if (dir == 1) { ie = N1; js -= e1; je += e1; ks -= e2; ke += e2; }
+    if (dir == 2) { je = N2; is -= e0; ie += e0; ks -= e2; ke += e2; }
+    if (dir == 3) { ke = N3; is -= e0; ie += e0; js -= e1; je += e1; }
+    for (int k = ks; k <= ke; k++)
+      for (int j = js; j <= je; j++)
+        for (int i = is; i <= ie; i++) {
+          const double ign = igdetnosing(g->geom[dir][ig2(d, i, j) * 27 + 21]);
+          double* F = Fs[dir] + ix(d, i, j, k);
+          F[O_B1] *= ign;
+          F[O_B2] *= ign;
+          F[O_B3] *= ign;
+        }
+  }
+  const double c1 = (e1 && e2) ? 0.25 : 0.5, c2 = (e0 && e2) ? 0.25 : 0.5, c3 = (e0 && e1) ? 0.25 : 0.5;
+  const int ni = N1 + e0, nj = N2 + e1, nk = N3 + e2;
+  double* emf = (double*)malloc(sizeof(double) * 3 * (size_t)ni * nj * nk);
+#define EMF(c, i, j, k) emf[(((size_t)(c) * nk + (k)) * nj + (j)) * ni + (i)]
+  for (int k = 0; k < nk; k++)
+    for (int j = 0; j < nj; j++)
+      for (int i = 0; i < ni; i++) {
+        const int km = k - e2, jm = j - e1, im = i - e0;
+        double e;
+        if (e1 || e2) {
+          if (e1 && e2) e = c1 * ((+F2[ix(d, i, j, k) + O_B3] + F2[ix(d, i, j, km) + O_B3]) - F3[ix(d, i, j, k) + O_B2] - F3[ix(d, i, jm, k) + O_B2]);
+          else if (e1) e = c1 * (+F2[ix(d, i, j, k) + O_B3] + F2[ix(d, i, j, km) + O_B3]);
+          else e = c1 * (-F3[ix(d, i, j, k) + O_B2] - F3[ix(d, i, jm, k) + O_B2]);
+        } else e = 0.0;
+        EMF(0, i, j, k) = e * g->geom[2][ig2(d, i, j) * 27 + 21]; /* CORN1 gdet = FACE2 gdet */
+        if (e0 || e2) {
+          if (e0 && e2) e = c2 * ((+F3[ix(d, i, j, k) + O_B1] + F3[ix(d, im, j, k) + O_B1]) - F1[ix(d, i, j, k) + O_B3] - F1[ix(d, i, j, km) + O_B3]);
+          else if (e0) e = c2 * (-F1[ix(d, i, j, k) + O_B3] - F1[ix(d, i, j, km) + O_B3]);
+          else e = c2 * (+F3[ix(d, i, j, k) + O_B1] + F3[ix(d, im, j, k) + O_B1]);
+        } else e = 0.0;
+        EMF(1, i, j, k) = e * g->geom[1][ig2(d, i, j) * 27 + 21]; /* CORN2 gdet = FACE1 gdet */
+        if (e0 || e1) {
+          if (e0 && e1) e = c3 * ((+F1[ix(d, i, j, k) + O_B2] + F1[ix(d, i, jm, k) + O_B2]) - F2[ix(d, i, j, k) + O_B1] - F2[ix(d, im, j, k) + O_B1]);
+          else if (e0) e = c3 * (+F1[ix(d, i, j, k) + O_B2] + F1[ix(d, i, jm, k) + O_B2]);
+          else e = c3 * (-F2[ix(d, i, j, k) + O_B1] - F2[ix(d, im, j, k) + O_B1]);
+        } else e = 0.0;
+        EMF(2, i, j, k) = e * g->gdet_corn3[ig2(d, i, j)];
+      }
+  /* rewrite face fluxes on active faces */
+  for (int k = 0; k < N3; k++)
+    for (int j = 0; j < N2; j++)
+      for (int i = 0; i <= N1 - 1 + e0; i++) {
+        if (!e0) break;
+        double* F = F1 + ix(d, i, j, k);
+        F[O_B1] = 0.;
+        F[O_B2] = 0.5 * (EMF(2, i, j, k) + EMF(2, i, j + e1, k));
+        F[O_B3] = -0.5 * (EMF(1, i, j, k) + EMF(1, i, j, k + e2));
+      }
+  for (int k = 0; k < N3; k++)
+    for (int j = 0; j <= N2 - 1 + e1; j++)
+      for (int i = 0; i < N1; i++) {
+        if (!e1) break;
+        double* F = F2 + ix(d, i, j, k);
+        F[O_B1] = -0.5 * (EMF(2, i, j, k) + EMF(2, i + e0, j, k));
+        F[O_B2] = 0.;
+        F[O_B3] = 0.5 * (EMF(0, i, j, k) + EMF(0, i, j, k + e2));
+      }
+  for (int k = 0; k <= N3 - 1 + e2; k++)
+    for (int j = 0; j < N2; j++)
+      for (int i = 0; i < N1; i++) {
+        if (!e2) break;
+        double* F = F3 + ix(d, i, j, k);
+        F[O_B1] = 0.5 * (EMF(1, i, j, k) + EMF(1, i + e0, j, k));
+        F[O_B2] = -0.5 * (EMF(0, i, j, k) + EMF(0, i, j + e1, k));
+        F[O_B3] = 0.;
+      }
+#undef EMF
+  free(emf);
+}
+
+void o_fluxcalc(const o_grid* g, const double* pb, double* F1, double* F2, double* F3, double* ndt, int do_ct)
+{
+  dims_t d = D(g);
+  ndt[1] = ndt[2] = ndt[3] = BIG;
+  if (d.n1 > 1) flux_dir(g, &d, 1, pb, F1, &ndt[1]);
+  if (d.n2 > 1) flux_dir(g, &d, 2, pb, F2, &ndt[2]);
+  if (d.n3 > 1) flux_dir(g, &d, 3, pb, F3, &ndt[3]);
+  if (do_ct) flux_ct(g, &d, F1, F2, F3);
+}
+
+/* ---------------------------------------------------------------------- */
+/* Update                                                                  */
+/* ---------------------------------------------------------------------- */
+
+/* dUtodt -> compute_dt_fromsource (advance.c:2581, :2331; the dUevolve
+ * arrays sized NPR, which is the intended reading of the indexing) */
+static double source_dt(const o_grid* g, const dims_t* d, const o_geom* G, const double* pr, const o_state* q,
+                        const double* dUtot, double dt)
+{
+  double U[NPR], Ugf[NPR];
+  o_primtoflux(pr, q, 0, G, U);
+  const double ig = igdetnosing(G->gdet);
+  for (int pl = 0; pl < NPR; pl++) Ugf[pl] = U[pl] * ig;
+  double dUd[4], dUu[4], mydU[4], mydx[4];
+  for (int jj = 0; jj < 4; jj++) dUd[jj] = (dUtot[O_UU + jj] * dt) * ig;
+  for (int a = 0; a < 4; a++) dUu[a] = G->gcon[a][0] * dUd[0] + G->gcon[a][1] * dUd[1] + G->gcon[a][2] * dUd[2] + G->gcon[a][3] * dUd[3];
+  mydU[0] = dUd[0];
+  mydx[0] = dt;
+  for (int jj = 1; jj < 4; jj++) { mydU[jj] = dUu[jj]; mydx[jj] = g->dx[jj]; }
+  /* b^mu from primitive B (bcon_calc), b^2 = b.b */
+  double bcon[4], bcov[4];
+  bcon[0] = pr[O_B1] * q->ucov[1] + pr[O_B2] * q->ucov[2] + pr[O_B3] * q->ucov[3];
+  for (int j = 1; j < 4; j++) bcon[j] = (pr[O_B1 - 1 + j] + bcon[0] * q->ucon[j]) / q->ucon[0];
+  for (int a = 0; a < 4; a++) bcov[a] = G->gcov[a][0] * bcon[0] + G->gcov[a][1] * bcon[1] + G->gcov[a][2] * bcon[2] + G->gcov[a][3] * bcon[3];
+  const double bsq = bcon[0] * bcov[0] + bcon[1] * bcov[1] + bcon[2] * bcov[2] + bcon[3] * bcov[3];
+  const double rho = pr[O_RHO], u = pr[O_UU];
+  const double P = (g->gam - 1.0) * u;
+  const double w = rho + u + P;
+  const double eta = w + bsq;
+  double dtsource[4];
+  for (int jj = 0; jj < 4; jj++) {
+    const double rhoprime = fmax(fabs(eta), fabs(Ugf[O_UU]));
+    const double ag = SMALL + fabs(mydU[jj] / rhoprime);
+    dtsource[jj] = g->cour * (mydx[jj] / ag);
+  }
+  double dtij = BIG;
+  if (dtsource[0] < dtij) dtij = dtsource[0];
+  if (d->n1 > 1 && dtsource[1] < dtij) dtij = dtsource[1];
+  if (d->n2 > 1 && dtsource[2] < dtij) dtij = dtsource[2];
+  if (d->n3 > 1 && dtsource[3] < dtij) dtij = dtsource[3];
+  return dtij;
+}
+
+void o_update(const o_grid* g, const double* pi, const double* pb, const double* F1, const double* F2,
+              const double* F3, double* ui, double* uf, double* ucum, const double* CUf, const double* Cunew,
+              double dt, int timeorder, double* accdt)
+{
+  dims_t d = D(g);
+  const double idx1 = 1.0 / g->dx[1], idx2 = 1.0 / g->dx[2], idx3 = 1.0 / g->dx[3];
+  const long s1 = NPR, s2 = (long)d.ns1 * NPR, s3 = (long)d.ns1 * d.ns2 * NPR;
+  double mindt = BIG;
+#pragma omp parallel for collapse(3) reduction(min : mindt) num_threads(g->nthreads > 0 ? g->nthreads : 1)
+  for (int k = 0; k < d.n3; k++)
+    for (int j = 0; j < d.n2; j++)
+      for (int i = 0; i < d.n1; i++) {
+        const long o = ix(&d, i, j, k);
+        o_geom G;
+        geom_at(g, &d, 0, i, j, &G);
+        o_state q;
+        double Ui[NPR], dU1[NPR], dU2[NPR], dU3[NPR], dUr[NPR], dUg[NPR];
+        o_get_state(pi + o, &G, g->gam, &q);
+        o_primtoflux(pi + o, &q, 0, &G, Ui);
+        for (int pl = 0; pl < NPR; pl++) ui[o + pl] = Ui[pl];
+        for (int pl = 0; pl < NPR; pl++) {
+          dU1[pl] = 0; dU2[pl] = 0; dU3[pl] = 0;
+          if (d.n1 > 1) dU1[pl] = (-(F1[o + s1 + pl] - F1[o + pl]) * idx1);
+          if (d.n2 > 1) dU2[pl] = (-(F2[o + s2 + pl] - F2[o + pl]) * idx2);
+          if (d.n3 > 1) dU3[pl] = (-(F3[o + s3 + pl] - F3[o + pl]) * idx3);
+          dUr[pl] = dU1[pl] + dU2[pl] + dU3[pl];
+        }
+        o_state qb;
+        o_get_state(pb + o, &G, g->gam, &qb);
+        o_source(pb + o, &G, &qb, g->conn + ig2(&d, i, j) * 40, dUg);
+        if (timeorder == 0)
+          for (int pl = 0; pl < NPR; pl++) { uf[o + pl] = 0.0; ucum[o + pl] = 0.0; }
+        for (int pl = 0; pl < NPR; pl++)
+          uf[o + pl] = (CUf[0] * Ui[pl] + CUf[1] * uf[o + pl] + CUf[2] * dt * (dUr[pl] + dUg[pl]));
+        for (int pl = 0; pl < NPR; pl++)
+          ucum[o + pl] += (Cunew[0] * Ui[pl] + Cunew[1] * dt * (dUr[pl] + dUg[pl]) + Cunew[2] * uf[o + pl]);
+        double dUt[NPR];
+        for (int pl = 0; pl < NPR; pl++) dUt[pl] = dUr[pl] + dUg[pl];
+        const double dts = source_dt(g, &d, &G, pb + o, &qb, dUt, dt);
+        if (dts < mindt) mindt = dts;
+      }
+  *accdt = mindt;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Inversion, floors, failure averaging                                    */
+/* ---------------------------------------------------------------------- */
+
+/* fixup1zone (fixup.c:561), FIXUPTYPE=1, rescaletype 0 floors */
+static void fixup1zone(const o_grid* g, const o_geom* G, double r, double* pr, int32_t* pflag, int* nstroke)
+{
+  o_params P = {g->gam, g->gammafail, g->gammamax};
+  double prfloor[2], prnew[NPR];
+  prfloor[0] = g->rhomin * pow(r, -1.5);
+  prfloor[1] = g->uumin * pow(r, -2.5);
+  for (int pl = 0; pl < 2; pl++)
+    if (prfloor[pl] < SMALL) prfloor[pl] = SMALL;
+  int didchange = 0;
+  for (int pl = 0; pl < NPR; pl++) {
+    if (pl <= O_UU && prfloor[pl] > pr[pl]) { didchange = 1; prnew[pl] = prfloor[pl]; }
+    else prnew[pl] = pr[pl];
+  }
+  if (didchange) {
+    o_state q, dq;
+    o_geom G1 = *G;
+    G1.gdet = 1.0; /* primtoU(UNOTHING): no gdet factor */
+    double U[NPR], dpr[NPR], dU[NPR];
+    o_get_state(pr, G, g->gam, &q);
+    o_primtoflux(pr, &q, 0, &G1, U);
+    for (int pl = 0; pl < NPR; pl++) dpr[pl] = 0.0;
+    for (int pl = O_RHO; pl <= O_UU; pl++) dpr[pl] = prnew[pl] - pr[pl];
+    o_get_state(dpr, G, g->gam, &dq);
+    o_primtoflux(dpr, &dq, 0, &G1, dU);
+    for (int pl = 0; pl < NPR; pl++) U[pl] += dU[pl];
+    const int fl = o_utoprimgen_unothing(U, G, &P, pr, nstroke);
+    *pflag = fl;
+    if (fl > 0)
+      for (int pl = 0; pl < NPR; pl++) pr[pl] = prnew[pl];
+  }
+  limit_gamma(g->gammamax, pr, G);
+}
+
+void o_invert(const o_grid* g, const double* pb, const double* U, double* pf, int32_t* pflag, int finalstep,
+              int64_t* nstroke)
+{
+  dims_t d = D(g);
+  const long ntot = (long)d.ns1 * d.ns2 * d.ns3 * NPR;
+  memcpy(pf, pb, sizeof(double) * ntot);
+  int64_t ns = 0;
+  o_params P = {g->gam, g->gammafail, g->gammamax};
+#pragma omp parallel for collapse(3) reduction(+ : ns) schedule(dynamic, 64) num_threads(g->nthreads > 0 ? g->nthreads : 1)
+  for (int k = 0; k < d.n3; k++)
+    for (int j = 0; j < d.n2; j++)
+      for (int i = 0; i < d.n1; i++) {
+        const long o = ix(&d, i, j, k);
+        const long f = ((long)(k + d.ng3) * d.ns2 + (j + d.ng2)) * d.ns1 + (i + d.ng1);
+        o_geom G;
+        geom_at(g, &d, 0, i, j, &G);
+        int nst = 0;
+        pflag[f] = o_utoprimgen(U + o, &G, &P, pf + o, &nst);
+        if (finalstep) fixup1zone(g, &G, g->rcent[i + d.ng1], pf + o, &pflag[f], &nst);
+        ns += nst;
+      }
+  if (nstroke) *nstroke = ns;
+}
+
+/* fixup_utoprim (fixup.c:1061) with the simple_average cascade (:2087); every
+ * failure averages rho..U3 (HANDLE*NEG=0); averages read a copy of pf and the
+ * original flags; ghost flags count as good. */
+void o_post_fixup(const o_grid* g, double* pf, const int32_t* pflag)
+{
+  dims_t d = D(g);
+  const long ntot = (long)d.ns1 * d.ns2 * d.ns3 * NPR;
+  double* pa = (double*)malloc(sizeof(double) * ntot);
+  memcpy(pa, pf, sizeof(double) * ntot);
+  const int e1 = d.n2 > 1;
+#define FL(i, j, k) (((i) < 0 || (i) >= d.n1 || (j) < 0 || (j) >= d.n2) ? 0 : pflag[((long)((k) + d.ng3) * d.ns2 + ((j) + d.ng2)) * d.ns1 + ((i) + d.ng1)])
+#define GOOD(i, j, k) (FL(i, j, k) <= 0)
+  for (int k = 0; k < d.n3; k++)
+    for (int j = 0; j < d.n2; j++)
+      for (int i = 0; i < d.n1; i++) {
+        if (FL(i, j, k) <= 0) continue;
+        const int jp = j + e1, jm = j - e1;
+        double* p = pf + ix(&d, i, j, k);
+        const double* a = pa;
+        for (int pl = O_RHO; pl <= O_U3; pl++) {
+          double v;
+          if (GOOD(i, jp, k) && GOOD(i, jm, k) && GOOD(i + 1, j, k) && GOOD(i - 1, j, k))
+            v = 0.25 * (a[ix(&d, i, jp, k) + pl] + a[ix(&d, i, jm, k) + pl] + a[ix(&d, i + 1, j, k) + pl] + a[ix(&d, i - 1, j, k) + pl]);
+          else if (GOOD(i + 1, jp, k) && GOOD(i + 1, jm, k) && GOOD(i - 1, jp, k) && GOOD(i - 1, jm, k))
+            v = 0.25 * (a[ix(&d, i + 1, jp, k) + pl] + a[ix(&d, i + 1, jm, k) + pl] + a[ix(&d, i - 1, jp, k) + pl] + a[ix(&d, i - 1, jm, k) + pl]);
+          else if (GOOD(i, jp, k) && GOOD(i, jm, k))
+            v = 0.5 * (a[ix(&d, i, jp, k) + pl] + a[ix(&d, i, jm, k) + pl]);
+          else if (GOOD(i + 1, j, k) && GOOD(i - 1, j, k))
+            v = 0.5 * (a[ix(&d, i + 1, j, k) + pl] + a[ix(&d, i - 1, j, k) + pl]);
+          else if (GOOD(i + 1, jp, k) && GOOD(i - 1, jm, k))
+            v = 0.5 * (a[ix(&d, i + 1, jp, k) + pl] + a[ix(&d, i - 1, jm, k) + pl]);
+          else if (GOOD(i + 1, jm, k) && GOOD(i - 1, jp, k))
+            v = 0.5 * (a[ix(&d, i + 1, jm, k) + pl] + a[ix(&d, i - 1, jp, k) + pl]);
+          else if (GOOD(i + 1, jp, k)) v = a[ix(&d, i + 1, jp, k) + pl];
+          else if (GOOD(i + 1, j, k)) v = a[ix(&d, i + 1, j, k) + pl];
+          else v = a[ix(&d, i, j, k) + pl];
+          p[pl] = v;
+        }
+        o_geom G;
+        geom_at(g, &d, 0, i, j, &G);
+        limit_gamma(g->gammamax, p, &G);
+      }
+#undef GOOD
+#undef FL
+  free(pa);
+}
+
+/* ---------------------------------------------------------------------- */
+/* Step driver                                                             */
+/* ---------------------------------------------------------------------- */
+
+int o_rk_tables(int timeorder, double CUf[4][4], double Cunew[4][4])
+{
+  memset(CUf, 0, sizeof(double) * 16);
+  memset(Cunew, 0, sizeof(double) * 16);
+  if (timeorder == 3) {
+    CUf[0][0] = 1.0; CUf[0][1] = 0.0; CUf[0][2] = 1.0;
+    CUf[1][0] = 3.0 / 4.0; CUf[1][1] = 1.0 / 4.0; CUf[1][2] = 1.0 / 4.0;
+    CUf[2][0] = 1.0 / 3.0; CUf[2][1] = 2.0 / 3.0; CUf[2][2] = 2.0 / 3.0;
+    Cunew[1][3] = 1.0;
+    Cunew[2][2] = 1.0; Cunew[2][3] = 1.0 / 4.0;
+    return 3;
+  }
+  CUf[0][0] = 1.0; CUf[0][1] = 0.0; CUf[0][2] = 0.5;
+  CUf[1][0] = 1.0; CUf[1][1] = 0.0; CUf[1][2] = 1.0;
+  Cunew[1][2] = 1.0; Cunew[1][3] = 0.5;
+  return 2;
+}
+
+/* effective weight of substep s's flux in the final state */
+static void rk_flux_weights(int timeorder, double* w)
+{
+  double CUf[4][4], Cunew[4][4];
+  const int n = o_rk_tables(timeorder, CUf, Cunew);
+  double C[4][4] = {{0}};
+  for (int s = 0; s < n; s++) {
+    for (int r = 0; r < s; r++) C[s][r] = CUf[s][1] * C[s - 1][r];
+    C[s][s] = CUf[s][2];
+  }
+  for (int r = 0; r < n; r++) w[r] = C[n - 1][r];
+}
+
+void o_step(const o_grid* g, double* p, double* pk0, double* pk1, double* ui, double* uf, double* ucum, double* F1,
+            double* F2, double* F3, int32_t* pflag, double* dt, double* fluxacc)
+{
+  dims_t d = D(g);
+  double CUf[4][4], Cunew[4][4], wflux[4];
+  const int nto = o_rk_tables(g->timeorder, CUf, Cunew);
+  rk_flux_weights(g->timeorder, wflux);
+  double* pii[3] = {p, p, p};
+  double* pbb[3] = {p, pk0, pk1};
+  double* pff[3] = {pk0, pk1, p};
+  if (nto == 2) { pbb[1] = pk0; pff[1] = p; }
+  double lastndt = BIG;
+  for (int to = 0; to < nto; to++) {
+    const int finalstep = (to == nto - 1);
+    double ndt[4], accdt;
+    o_fluxcalc(g, pbb[to], F1, F2, F3, ndt, 1);
+    if (fluxacc && d.n1 > 1) {
+      /* diag_flux_pureflux: F1[RHO], F1[UU] through the radial boundaries */
+      double s[4] = {0, 0, 0, 0};
+      for (int k = 0; k < d.n3; k++)
+        for (int j = 0; j < d.n2; j++) {
+          s[0] += F1[ix(&d, 0, j, k) + O_RHO];
+          s[1] += F1[ix(&d, 0, j, k) + O_UU];
+          s[2] += F1[ix(&d, d.n1, j, k) + O_RHO];
+          s[3] += F1[ix(&d, d.n1, j, k) + O_UU];
+        }
+      for (int c = 0; c < 4; c++) fluxacc[c] += s[c] * g->dx[2] * g->dx[3] * (*dt) * wflux[to];
+    }
+    o_update(g, pii[to], pbb[to], F1, F2, F3, ui, uf, ucum, CUf[to], Cunew[to], *dt, to, &accdt);
+    int64_t ns;
+    o_invert(g, pbb[to], finalstep ? ucum : uf, pff[to], pflag, finalstep, &ns);
+    const double wavedt = 1. / (1. / ndt[1] + 1. / ndt[2] + 1. / ndt[3]);
+    const double nd = 1.0 * fmin(wavedt, accdt);
+    if (nd < lastndt) lastndt = nd;
+    o_bound(g, pff[to]);
+    o_post_fixup(g, pff[to], pflag);
+    o_bound(g, pff[to]);
+  }
+  double fullndt = lastndt;
+  if (fullndt > g->safe * (*dt)) fullndt = g->safe * (*dt);
+  *dt = fullndt;
+}

@@ -1,0 +1,58 @@
+/*
+ * o_grid.h -- oracle restatement of HARM's advance() substep on a grid.
+ * TEST INFRASTRUCTURE ONLY (see oracle.h).  Arrays are HARM layout
+ * a[((k*ns2+j)*ns1+i)*8+pl] over the ghosted storage box.
+ */
+#ifndef HARM_O_GRID_H
+#define HARM_O_GRID_H
+
+#include <stdint.h>
+#include "oracle.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct o_grid {
+  int n1, n2, n3, ng;
+  int recon, flux, timeorder;
+  int nthreads;
+  double gam, cour, safe, gammamax, gammafail;
+  double dx[4];      /* dX1..dX3 (index 1..3)                      */
+  double startx[4];  /* X of the left edge of zone 0               */
+  double rhomin, uumin;
+  /* geometry, AoS per 2-D point: (j*ns1+i)*27 + f */
+  const double* geom[4]; /* CENT, FACE1, FACE2, FACE3 */
+  const double* gdet_corn3; /* (j*ns1+i) */
+  const double* conn;       /* (j*ns1+i)*40 + c */
+  const double* rcent;      /* r at zone centre per i (ns1) */
+} o_grid;
+
+/* storage helpers */
+void o_dims(const o_grid* g, int* ng1, int* ng2, int* ng3, int* ns1, int* ns2, int* ns3);
+
+/* bound_prim_user_dir for dirs 1..3 (bounds.fishmon.c:15, bounds.tools.c) */
+void o_bound(const o_grid* g, double* p);
+/* fluxcalc + flux_ct (flux.c:38, fluxct.c:186); ndt[1..3] */
+void o_fluxcalc(const o_grid* g, const double* pb, double* F1, double* F2, double* F3, double* ndt, int do_ct);
+/* advance_standard update loop (advance.c:420-564) */
+void o_update(const o_grid* g, const double* pi, const double* pb, const double* F1, const double* F2,
+              const double* F3, double* ui, double* uf, double* ucum, const double* CUf, const double* Cunew,
+              double dt, int timeorder, double* accdt);
+/* pf = pb; invert utoinvert into pf over active zones; fixup1zone on finalstep */
+void o_invert(const o_grid* g, const double* pb, const double* U, double* pf, int32_t* pflag, int finalstep,
+              int64_t* nstroke);
+/* fixup_utoprim restatement (simple_average cascade) */
+void o_post_fixup(const o_grid* g, double* pf, const int32_t* pflag);
+/* one full RK step (step_ch_simplempi + post_advance + post_stepch).
+ * Arrays: p (in/out), pk0, pk1, ui, uf, ucum, F1..F3 scratch, pflag.
+ * *dt in/out; fluxacc[4] accumulates boundary fluxes (may be NULL). */
+void o_step(const o_grid* g, double* p, double* pk0, double* pk1, double* ui, double* uf, double* ucum, double* F1,
+            double* F2, double* F3, int32_t* pflag, double* dt, double* fluxacc);
+/* RK coefficient tables (step_ch.c:1171 setup_rktimestep) */
+int o_rk_tables(int timeorder, double CUf[4][4], double Cunew[4][4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -89,3 +89,137 @@ def source_batch(p, geom, conn, gam=4.0 / 3.0):
     dU = np.empty_like(p)
     lib().o_source_batch(p.ctypes.data, geom.ctypes.data, conn.ctypes.data, dU.ctypes.data, n, gam)
     return dU
+
+
+# ---------------------------------------------------------------------------
+# grid-level oracle (o_grid.c)
+# ---------------------------------------------------------------------------
+class OGridStruct(ctypes.Structure):
+    _fields_ = [
+        ("n1", ctypes.c_int), ("n2", ctypes.c_int), ("n3", ctypes.c_int), ("ng", ctypes.c_int),
+        ("recon", ctypes.c_int), ("flux", ctypes.c_int), ("timeorder", ctypes.c_int), ("nthreads", ctypes.c_int),
+        ("gam", ctypes.c_double), ("cour", ctypes.c_double), ("safe", ctypes.c_double),
+        ("gammamax", ctypes.c_double), ("gammafail", ctypes.c_double),
+        ("dx", ctypes.c_double * 4), ("startx", ctypes.c_double * 4),
+        ("rhomin", ctypes.c_double), ("uumin", ctypes.c_double),
+        ("geom", ctypes.c_void_p * 4), ("gdet_corn3", ctypes.c_void_p), ("conn", ctypes.c_void_p),
+        ("rcent", ctypes.c_void_p),
+    ]
+
+
+def _grid_protos(L):
+    P = ctypes.POINTER(OGridStruct)
+    L.o_bound.argtypes = [P, _P]
+    L.o_fluxcalc.argtypes = [P, _P, _P, _P, _P, _P, _I]
+    L.o_update.argtypes = [P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _D, _I, _P]
+    L.o_invert.argtypes = [P, _P, _P, _P, _P, _I, _P]
+    L.o_post_fixup.argtypes = [P, _P, _P]
+    L.o_step.argtypes = [P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]
+    L.o_rk_tables.argtypes = [_I, _P, _P]
+    L.o_rk_tables.restype = _I
+    for n in ("o_bound", "o_fluxcalc", "o_update", "o_invert", "o_post_fixup", "o_step"):
+        getattr(L, n).restype = None
+
+
+class OGrid:
+    """The oracle's grid, built from the library's host geometry tables so that
+    both sides see identical metric and connection values."""
+
+    def __init__(self, cfg, nthreads=8):
+        import harm_harm_amd as H
+        L = lib()
+        if not hasattr(L, "_grid_ok"):
+            _grid_protos(L)
+            L._grid_ok = True
+        self.cfg = cfg
+        self.ng1 = cfg.ng if cfg.n1 > 1 else 0
+        self.ng2 = cfg.ng if cfg.n2 > 1 else 0
+        self.ng3 = cfg.ng if cfg.n3 > 1 else 0
+        self.ns1, self.ns2, self.ns3 = cfg.n1 + 2 * self.ng1, cfg.n2 + 2 * self.ng2, cfg.n3 + 2 * self.ng3
+        np_ = self.ns1 * self.ns2
+        self.geom = np.empty((4, self.ns2, self.ns1, 27))
+        self.conn = np.empty((self.ns2, self.ns1, 40))
+        self.gc3 = np.empty((self.ns2, self.ns1))
+        self.rc = np.empty(self.ns1)
+        sx = (ctypes.c_double * 4)()
+        dx = (ctypes.c_double * 4)()
+        H.check(H.lib().harm_host_tables(ctypes.byref(cfg), self.geom.ctypes.data, self.conn.ctypes.data,
+                                         self.gc3.ctypes.data, self.rc.ctypes.data, sx, dx), "host_tables")
+        g = OGridStruct()
+        g.n1, g.n2, g.n3, g.ng = cfg.n1, cfg.n2, cfg.n3, cfg.ng
+        g.recon, g.flux, g.timeorder, g.nthreads = cfg.recon, cfg.flux, cfg.timeorder, nthreads
+        g.gam, g.cour, g.safe, g.gammamax, g.gammafail = cfg.gam, cfg.cour, cfg.safe, cfg.gammamax, cfg.gammafail
+        for a in range(4):
+            g.dx[a] = dx[a]
+            g.startx[a] = sx[a]
+        g.rhomin, g.uumin = cfg.rhomin, cfg.uumin
+        for loc in range(4):
+            g.geom[loc] = self.geom[loc].ctypes.data
+        g.gdet_corn3 = self.gc3.ctypes.data
+        g.conn = self.conn.ctypes.data
+        g.rcent = self.rc.ctypes.data
+        self.g = g
+        self.dx = [dx[a] for a in range(4)]
+
+    @property
+    def shape(self):
+        return (self.ns3, self.ns2, self.ns1, 8)
+
+    def zeros(self):
+        return np.zeros(self.shape)
+
+    def init_torus(self, rin=6.0, rmax=12.0, kappa=1e-3, beta=100.0, randfact=4e-2, seed=1):
+        import harm_harm_amd as H
+        p = self.zeros()
+        H.check(H.lib().harm_host_init_torus(ctypes.byref(self.cfg), rin, rmax, kappa, beta, randfact, seed,
+                                             p.ctypes.data), "host_init_torus")
+        self.bound(p)
+        return p
+
+    def bound(self, p):
+        lib().o_bound(ctypes.byref(self.g), p.ctypes.data)
+
+    def fluxcalc(self, pb, F1, F2, F3, do_ct=True):
+        ndt = (ctypes.c_double * 4)()
+        lib().o_fluxcalc(ctypes.byref(self.g), pb.ctypes.data, F1.ctypes.data, F2.ctypes.data, F3.ctypes.data, ndt,
+                         int(do_ct))
+        return [ndt[1], ndt[2], ndt[3]]
+
+    def update(self, pi, pb, F1, F2, F3, ui, uf, ucum, CUf, Cunew, dt, timeorder):
+        c1 = (ctypes.c_double * 4)(*CUf)
+        c2 = (ctypes.c_double * 4)(*Cunew)
+        acc = ctypes.c_double()
+        lib().o_update(ctypes.byref(self.g), pi.ctypes.data, pb.ctypes.data, F1.ctypes.data, F2.ctypes.data,
+                       F3.ctypes.data, ui.ctypes.data, uf.ctypes.data, ucum.ctypes.data, c1, c2, dt, timeorder,
+                       ctypes.byref(acc))
+        return acc.value
+
+    def invert(self, pb, U, pf, pflag, finalstep):
+        ns = ctypes.c_int64()
+        lib().o_invert(ctypes.byref(self.g), pb.ctypes.data, U.ctypes.data, pf.ctypes.data, pflag.ctypes.data,
+                       int(finalstep), ctypes.byref(ns))
+        return ns.value
+
+    def post_fixup(self, pf, pflag):
+        lib().o_post_fixup(ctypes.byref(self.g), pf.ctypes.data, pflag.ctypes.data)
+
+    def rk_tables(self):
+        CUf = np.zeros((4, 4))
+        Cunew = np.zeros((4, 4))
+        n = lib().o_rk_tables(self.cfg.timeorder, CUf.ctypes.data, Cunew.ctypes.data)
+        return n, CUf, Cunew
+
+    def step(self, state, dt, fluxacc=None):
+        """state: dict of arrays p, pk0, pk1, ui, uf, ucum, F1, F2, F3, pflag; returns new dt."""
+        d = ctypes.c_double(dt)
+        fa = fluxacc if fluxacc is not None else None
+        lib().o_step(ctypes.byref(self.g), *[state[k].ctypes.data for k in
+                                             ("p", "pk0", "pk1", "ui", "uf", "ucum", "F1", "F2", "F3", "pflag")],
+                     ctypes.byref(d), fa.ctypes.data if fa is not None else None)
+        return d.value
+
+    def new_state(self, p):
+        st = {k: self.zeros() for k in ("pk0", "pk1", "ui", "uf", "ucum", "F1", "F2", "F3")}
+        st["p"] = p
+        st["pflag"] = np.zeros(self.shape[:3], dtype=np.int32)
+        return st
