@@ -1,0 +1,168 @@
+#!/usr/bin/env python3
+"""Benchmark: zone-updates/s of the HARM advance() step on one MI355X.
+
+Workload: the 3-D Fishbone-Moncrief torus in MKS Kerr-Schild coordinates at
+256x128x64 (r, theta, phi), FP64, a=0.9375, Gamma=4/3, WENO5 reconstruction,
+HLL fluxes, Toth flux-CT, RK3, Courant 0.8, seed-1 internal-energy noise,
+rescaletype-0 floors.  A "step" is one full RK3 step (3 substeps: fluxes, CT,
+update, inversion, floors, bounds, failure averaging, dt).  All state is
+resident in HBM before the timed region; the timed region is K steps between
+two device synchronisations.
+
+Prints one JSON line: the throughput, a roofline record for the dominant kernel
+(algorithmic HBM bytes per launch over its event-measured mean duration) and a
+CPU baseline (the C restatement, OpenMP, timed on this host's cores on a
+bounded sample of the same workload).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+GRID = (256, 128, 64)
+
+
+def algorithmic_bytes(g, kernel):
+    """Bytes each kernel group must move per launch at minimum (compulsory
+    traffic of its arrays; 2-D geometry tables counted once per (i,j) face or
+    zone column)."""
+    n1, n2, n3 = g.cfg.n1, g.cfg.n2, g.cfg.n3
+    e0, e1, e2 = int(n1 > 1), int(n2 > 1), int(n3 > 1)
+    geom = 27 * 8
+    if kernel.startswith("flux"):
+        d = int(kernel[-1])
+        if d == 1:
+            ni, nj, nk = n1 + 1, n2 + 2 * e1, n3 + 2 * e2
+        elif d == 2:
+            ni, nj, nk = n1 + 2 * e0, n2 + 1, n3 + 2 * e2
+        else:
+            ni, nj, nk = n1 + 2 * e0, n2 + 2 * e1, n3 + 1
+        faces = ni * nj * nk
+        # read 8 primitives per cell of the swept block, write 8 fluxes per face
+        return faces * (64 + 64) + ni * nj * geom
+    zones = n1 * n2 * n3
+    if kernel == "update":
+        # pi, pb, 3 face fluxes (8 each), ui write, uf r/w, ucum r/w
+        return zones * (64 + 64 + 3 * 64 + 64 + 128 + 128) + n1 * n2 * (geom + 40 * 8)
+    if kernel == "invert":
+        # U read, guess read, solution write, flag write (+ the pb->pf copy 128 B)
+        return zones * (64 + 64 + 64 + 4 + 128) + n1 * n2 * geom
+    if kernel == "emf_ct":
+        return zones * (6 * 8 + 3 * 8 + 3 * 8 + 6 * 8)
+    return None
+
+
+def initial_dt(g):
+    ndt = g.fluxcalc("p", do_ct=True)
+    wavedt = 1.0 / sum(1.0 / x for x in ndt)
+    acc = g.update("p", "p", [1.0, 0.0, 1.0, 0.0], [0.0, 0.0, 0.0, 0.0], wavedt, 0)
+    return min(wavedt, acc)
+
+
+def cpu_baseline(cfg, nsteps):
+    """The oracle (C restatement of the same step, OpenMP) on this host."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle
+    threads = min(16, os.cpu_count() or 1)
+    og = _oracle.OGrid(cfg, nthreads=threads)
+    p = og.init_torus(seed=1)
+    ndt = og.fluxcalc(p, og.zeros(), og.zeros(), og.zeros())
+    dt = 1.0 / sum(1.0 / x for x in ndt)
+    st = og.new_state(p)
+    t0 = time.time()
+    for _ in range(nsteps):
+        dt = og.step(st, dt)
+    el = time.time() - t0
+    zones = cfg.n1 * cfg.n2 * cfg.n3
+    return {"value": zones * nsteps / el, "unit": "zone-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{nsteps} full RK3 steps of the same {cfg.n1}x{cfg.n2}x{cfg.n3} WENO5/HLL torus "
+                      f"({zones * nsteps} zone-updates, {el:.1f} s) with the oracle's C restatement, "
+                      f"OpenMP on {threads} host threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--grid", default="x".join(map(str, GRID)))
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs the whole domain on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is a design note only)", file=sys.stderr)
+        sys.exit(2)
+
+    import harm_harm_amd as H
+    from harm_harm_amd.grid import Grid
+
+    n1, n2, n3 = (int(x) for x in args.grid.split("x"))
+    cfg = H.default_config(n1, n2, n3, recon="weno5", flux="hll", timeorder=3)
+    g = Grid(cfg)
+    g.init_torus(seed=1)
+    dt0 = initial_dt(g)
+    g.set_time(0.0, dt0)
+    g.step(args.warmup)
+    g.sync()
+
+    g.profile(True)
+    g.sync()
+    t0 = time.perf_counter()
+    g.step(args.steps)
+    g.sync()
+    el = time.perf_counter() - t0
+    prof = g.profile_read()
+    g.profile(False)
+    t_sim, dt_now = g.time()
+
+    zones = n1 * n2 * n3
+    value = zones * args.steps / el
+    # dominant kernel group by total event time
+    dom = max((k for k in prof if prof[k][1] > 0), key=lambda k: prof[k][0])
+    ms_tot, nlaunch = prof[dom]
+    avg_s = ms_tot / 1e3 / nlaunch
+    nbytes = algorithmic_bytes(g, dom)
+    achieved = nbytes / avg_s / 1e9 if nbytes else None
+    kernels = {k: {"ms_total": round(v[0], 3), "launches": v[1],
+                   "ms_avg": round(v[0] / v[1], 4) if v[1] else None,
+                   "GBs_alg": round(algorithmic_bytes(g, k) / (v[0] / 1e3 / v[1]) / 1e9, 1)
+                   if v[1] and algorithmic_bytes(g, k) else None} for k, v in prof.items() if v[1]}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(cfg, args.cpu_steps)
+
+    out = {
+        "metric": "zone-updates/s, 3D FM torus 256x128x64 FP64: plain vs fused+MLP-guess inversion",
+        "value": value,
+        "unit": "zone-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Fishbone-Moncrief torus initial data, random-free geometry)",
+        "config": {"workload": f"3D FM torus {n1}x{n2}x{n3} MKS Kerr-Schild a=0.9375, plain step",
+                   "recon": "WENO5", "flux": "HLL", "ct": "Toth", "rk": 3, "cour": cfg.cour,
+                   "form": "plain (multi-kernel step, Newton from previous primitives)",
+                   "sim_time_end": t_sim, "dt": dt_now},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": None,
+                     "bytes_per_launch_alg": nbytes, "avg_launch_ms": avg_s * 1e3},
+        "kernels": kernels,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -98,6 +98,8 @@ _OPTIONAL_PROTOS = [
     ("harm_step", _I, [_P, _I]),
     ("harm_get_boundary_fluxes", _I, [_P, _P]),
     ("harm_get_stream", _I, [_P, ctypes.POINTER(_P)]),
+    ("harm_profile", _I, [_P, _I]),
+    ("harm_profile_read", _I, [_P, _P, _P]),
 ]
 
 _lib = None

@@ -738,6 +738,13 @@ struct harm_handle {
   std::vector<double> hconn;
   std::vector<double> hrcent;
   std::vector<double> hgc3;
+  // kernel timing
+  bool prof = false;
+  std::vector<hipEvent_t> evpool;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> evrec;
+  double prof_ms[HARM_K_N] = {0};
+  long long prof_n[HARM_K_N] = {0};
+  hipEvent_t cur0 = nullptr;
 };
 
 namespace {
@@ -841,6 +848,35 @@ harm_status build_geometry(harm_handle* h)
   return HARM_OK;
 }
 
+hipEvent_t ev_get(harm_handle* h)
+{
+  if (h->evpool.empty()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+  }
+  hipEvent_t e = h->evpool.back();
+  h->evpool.pop_back();
+  return e;
+}
+
+void prof_begin(harm_handle* h)
+{
+  if (!h->prof) return;
+  h->cur0 = ev_get(h);
+  if (h->cur0) (void)hipEventRecord(h->cur0, h->stream);
+}
+
+void prof_end(harm_handle* h, int kind)
+{
+  if (!h->prof || !h->cur0) return;
+  hipEvent_t e1 = ev_get(h);
+  if (!e1) return;
+  (void)hipEventRecord(e1, h->stream);
+  h->evrec.push_back({kind, {h->cur0, e1}});
+  h->cur0 = nullptr;
+}
+
 harm_status check_handle(const harm_handle* h) { return h ? HARM_OK : hh::set_error(HARM_EINVAL, "null handle"); }
 harm_status check_which(int which)
 {
@@ -868,12 +904,13 @@ harm_status enq_fluxcalc(harm_handle* h, int which_p, int do_ct)
       hipLaunchKernelGGL((k_flux<DIR, HARM_RECON_DONOR>), dim3(nblocks(b.count())), dim3(kBlock), 0, s, d, b, pb,   \
                          F, G, h->P, ft, do_ct, h->ds);                                                             \
   } while (0)
-  if (d.n1 > 1) LAUNCH_FLUX(1, h->arr[HARM_ARR_F1], h->geom[HARM_LOC_FACE1]);
-  if (d.n2 > 1) LAUNCH_FLUX(2, h->arr[HARM_ARR_F2], h->geom[HARM_LOC_FACE2]);
-  if (d.n3 > 1) LAUNCH_FLUX(3, h->arr[HARM_ARR_F3], h->geom[HARM_LOC_FACE3]);
+  if (d.n1 > 1) { prof_begin(h); LAUNCH_FLUX(1, h->arr[HARM_ARR_F1], h->geom[HARM_LOC_FACE1]); prof_end(h, HARM_K_FLUX1); }
+  if (d.n2 > 1) { prof_begin(h); LAUNCH_FLUX(2, h->arr[HARM_ARR_F2], h->geom[HARM_LOC_FACE2]); prof_end(h, HARM_K_FLUX2); }
+  if (d.n3 > 1) { prof_begin(h); LAUNCH_FLUX(3, h->arr[HARM_ARR_F3], h->geom[HARM_LOC_FACE3]); prof_end(h, HARM_K_FLUX3); }
 #undef LAUNCH_FLUX
   HH_TRY(hh::check_launch("k_flux"));
   if (do_ct) {
+    prof_begin(h);
     const int ni = h->emf_ni, nj = h->emf_nj, nk = h->emf_nk;
     hipLaunchKernelGGL(k_emf, dim3(nblocks((long)ni * nj * nk)), dim3(kBlock), 0, s, d, ni, nj, nk,
                        h->arr[HARM_ARR_F1], h->arr[HARM_ARR_F2], h->arr[HARM_ARR_F3], h->geom[HARM_LOC_FACE1],
@@ -888,6 +925,7 @@ harm_status enq_fluxcalc(harm_handle* h, int which_p, int do_ct)
     if (e2)
       hipLaunchKernelGGL(k_ct<3>, dim3(nblocks((long)d.n1 * d.n2 * (d.n3 + 1))), dim3(kBlock), 0, s, d, ni, nj, nk,
                          h->emf, h->arr[HARM_ARR_F3]);
+    prof_end(h, HARM_K_EMFCT);
     HH_TRY(hh::check_launch("k_emf/k_ct"));
   }
   return HARM_OK;
@@ -897,11 +935,13 @@ harm_status enq_update(harm_handle* h, int which_pi, int which_pb, const double*
                        int timeorder)
 {
   const Dims& d = h->d;
+  prof_begin(h);
   hipLaunchKernelGGL(k_update, dim3(nblocks((long)d.n1 * d.n2 * d.n3)), dim3(kBlock), 0, h->stream, d, h->P,
                      h->arr[which_pi], h->arr[which_pb], h->arr[HARM_ARR_F1], h->arr[HARM_ARR_F2],
                      h->arr[HARM_ARR_F3], h->arr[HARM_ARR_UI], h->arr[HARM_ARR_UF], h->arr[HARM_ARR_UCUM],
                      h->geom[HARM_LOC_CENT], h->conn, CUf[0], CUf[1], CUf[2], Cunew[0], Cunew[1], Cunew[2], timeorder,
                      h->ds);
+  prof_end(h, HARM_K_UPDATE);
   return hh::check_launch("k_update");
 }
 
@@ -912,8 +952,10 @@ harm_status enq_invert(harm_handle* h, int which_pb, int which_pf, int finalstep
   if (which_pf != which_pb)
     HH_HIP(hipMemcpyAsync(h->arr[which_pf], h->arr[which_pb], bytes, hipMemcpyDeviceToDevice, h->stream));
   const double* U = h->arr[finalstep ? HARM_ARR_UCUM : HARM_ARR_UF];
+  prof_begin(h);
   hipLaunchKernelGGL(k_invert, dim3(nblocks((long)d.n1 * d.n2 * d.n3)), dim3(kBlock), 0, h->stream, d, h->P, U,
                      h->arr[which_pf], h->pflag, h->geom[HARM_LOC_CENT], h->floors, finalstep, h->di);
+  prof_end(h, HARM_K_INVERT);
   return hh::check_launch("k_invert");
 }
 
@@ -922,6 +964,7 @@ harm_status enq_bound(harm_handle* h, int which)
   const Dims& d = h->d;
   hipStream_t s = h->stream;
   double* p = h->arr[which];
+  prof_begin(h);
   if (d.n1 > 1)
     hipLaunchKernelGGL(k_bound_x1, dim3(nblocks(2L * d.n2 * d.n3)), dim3(kBlock), 0, s, d, h->P, p,
                        h->geom[HARM_LOC_CENT]);
@@ -933,15 +976,18 @@ harm_status enq_bound(harm_handle* h, int which)
     }
   }
   if (d.n3 > 1) hipLaunchKernelGGL(k_bound_x3, dim3(nblocks(d.nplane * 2L * d.ng3)), dim3(kBlock), 0, s, d, p);
+  prof_end(h, HARM_K_BOUND);
   return hh::check_launch("bounds");
 }
 
 harm_status enq_post_fixup(harm_handle* h, int which)
 {
   const Dims& d = h->d;
+  prof_begin(h);
   HH_HIP(hipMemcpyAsync(h->pa, h->arr[which], sizeof(double) * HARM_NPR * d.ncell, hipMemcpyDeviceToDevice, h->stream));
   hipLaunchKernelGGL(k_post_fixup, dim3(nblocks((long)d.n1 * d.n2 * d.n3)), dim3(kBlock), 0, h->stream, d, h->P,
                      h->pa, h->arr[which], h->pflag, h->geom[HARM_LOC_CENT]);
+  prof_end(h, HARM_K_FIXUP);
   return hh::check_launch("k_post_fixup");
 }
 
@@ -1058,6 +1104,11 @@ extern "C" harm_status harm_destroy(harm_handle* h)
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   for (int a = 0; a < HARM_NARR; a++) (void)hipFree(h->arr[a]);
   for (int l = 0; l < HARM_NLOC; l++) (void)hipFree(h->geom[l]);
+  for (auto& r : h->evrec) {
+    (void)hipEventDestroy(r.second.first);
+    (void)hipEventDestroy(r.second.second);
+  }
+  for (hipEvent_t e : h->evpool) (void)hipEventDestroy(e);
   void* bufs[] = {h->pa, h->pflag, h->gcorn3, h->conn, h->floors, h->rcent, h->emf, h->ds, h->di};
   for (void* b : bufs) (void)hipFree(b);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -1336,5 +1387,34 @@ extern "C" harm_status harm_host_init_torus(const harm_config* cfg, double rin, 
   memset(p_aos, 0, sizeof(double) * HARM_NPR * d.ncell);
   if (hi_init_torus(cfg, &G, rin, rmax, kappa, beta, randfact, seed, p_aos) != 0)
     return hh::set_error(HARM_EINVAL, "harm_host_init_torus: no torus zones on this grid");
+  return HARM_OK;
+}
+
+extern "C" harm_status harm_profile(harm_handle* h, int32_t enable)
+{
+  HH_TRY(check_handle(h));
+  h->prof = enable != 0;
+  return HARM_OK;
+}
+
+extern "C" harm_status harm_profile_read(harm_handle* h, double* ms_total, int64_t* count)
+{
+  HH_TRY(check_handle(h));
+  HH_HIP(hipStreamSynchronize(h->stream));
+  for (auto& r : h->evrec) {
+    float ms = 0.f;
+    HH_HIP(hipEventElapsedTime(&ms, r.second.first, r.second.second));
+    h->prof_ms[r.first] += ms;
+    h->prof_n[r.first] += 1;
+    h->evpool.push_back(r.second.first);
+    h->evpool.push_back(r.second.second);
+  }
+  h->evrec.clear();
+  for (int k = 0; k < HARM_K_N; k++) {
+    if (ms_total) ms_total[k] = h->prof_ms[k];
+    if (count) count[k] = h->prof_n[k];
+    h->prof_ms[k] = 0.0;
+    h->prof_n[k] = 0;
+  }
   return HARM_OK;
 }

@@ -134,6 +134,17 @@ class Grid:
         check(lib().harm_get_boundary_fluxes(self.h, out), "harm_get_boundary_fluxes")
         return list(out)
 
+    KERNELS = ("flux1", "flux2", "flux3", "emf_ct", "update", "invert", "bound", "fixup", "other")
+
+    def profile(self, enable=True):
+        check(lib().harm_profile(self.h, int(enable)), "harm_profile")
+
+    def profile_read(self):
+        ms = np.zeros(len(self.KERNELS))
+        n = np.zeros(len(self.KERNELS), dtype=np.int64)
+        check(lib().harm_profile_read(self.h, ms.ctypes.data, n.ctypes.data), "harm_profile_read")
+        return {k: (ms[i], int(n[i])) for i, k in enumerate(self.KERNELS)}
+
     def close(self):
         if getattr(self, "h", None):
             lib().harm_destroy(self.h)
