@@ -462,6 +462,23 @@ static void face_flux(const o_grid* g, int dir, const o_geom* G, double* p_l, do
   }
 }
 
+/* Single-cell and single-face forms of the two routines above, exported for
+ * the tests that pin them on known answers. */
+void o_recon_cell(int recon, const double* y5, double* l, double* r) { recon_cell(recon, y5 + 2, l, r); }
+
+void o_face_flux(int flux, int dir, double gam, double gammamax, const double* rec, double* p_l, double* p_r,
+                 double* F, double* ctop)
+{
+  o_grid g;
+  memset(&g, 0, sizeof(g));
+  g.flux = flux;
+  g.gam = gam;
+  g.gammamax = gammamax;
+  o_geom G;
+  o_geom_from_rec(rec, &G);
+  face_flux(&g, dir, &G, p_l, p_r, F, ctop);
+}
+
 static void flux_dir(const o_grid* g, const dims_t* d, int dir, const double* pb, double* F, double* ndt)
 {
   const int e1 = d->n2 > 1, e2 = d->n3 > 1, e0 = d->n1 > 1;

@@ -49,6 +49,13 @@ void o_post_fixup(const o_grid* g, double* pf, const int32_t* pflag);
  * *dt in/out; fluxacc[4] accumulates boundary fluxes (may be NULL). */
 void o_step(const o_grid* g, double* p, double* pk0, double* pk1, double* ui, double* uf, double* ucum, double* F1,
             double* F2, double* F3, int32_t* pflag, double* dt, double* fluxacc);
+/* one cell's (left, right) edge values from y5[0..4] = y[-2..2] (recon 0 donor,
+ * 2 PARA, 3 WENO5), and one face's Riemann flux (flux 0 LAX-F, 1 HLL) from
+ * two states at a 27-field geometry record; p_l, p_r are modified as
+ * getplpr/check_plpr do (B^dir averaged, gamma limited). */
+void o_recon_cell(int recon, const double* y5, double* l, double* r);
+void o_face_flux(int flux, int dir, double gam, double gammamax, const double* rec, double* p_l, double* p_r,
+                 double* F, double* ctop);
 /* RK coefficient tables (step_ch.c:1171 setup_rktimestep) */
 int o_rk_tables(int timeorder, double CUf[4][4], double Cunew[4][4]);
 

@@ -7,6 +7,15 @@
  * whose algorithm it restates.  Arrays are HARM layout: a[((k*ns2+j)*ns1+i)*8+pl]
  * over the ghosted storage box; geometry is the library's per-point record
  * (27 doubles) on the 2-D (i,j) storage plane.
+ *
+ * Pinning (tests/test_oracle_phys.py, tests/test_oracle_grid.py): per-zone
+ * state, fluxes, wave speeds and the inversion against textbook GRMHD and
+ * round trips; reconstruction, the face Riemann solver, RK tables, ghost
+ * zones, flux-CT and the update/source balance against known answers and
+ * invariants.  Parity unpinned: the WENO5BND extras that are not restated
+ * (SMONO, stencil reduction, shock reduction), the radial outflow
+ * extrapolation formulas, the pole-death velocity interpolation and the
+ * failure-averaging cascade, which only the GPU comparison exercises.
  */
 #ifndef HARM_ORACLE_H
 #define HARM_ORACLE_H

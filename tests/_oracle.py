@@ -91,6 +91,32 @@ def source_batch(p, geom, conn, gam=4.0 / 3.0):
     return dU
 
 
+def recon_cell(recon, y5):
+    """(left, right) edge values of the cell y5[2] from y5 = y[-2..2]."""
+    y = _c(y5)
+    l, r = _D(), _D()
+    L = lib()
+    L.o_recon_cell.argtypes = [_I, _P, _P, _P]
+    L.o_recon_cell.restype = None
+    L.o_recon_cell(int(recon), y.ctypes.data, ctypes.byref(l), ctypes.byref(r))
+    return l.value, r.value
+
+
+def face_flux(flux, dir, rec, p_l, p_r, gam=4.0 / 3.0, gammamax=25.0):
+    """One face's Riemann flux (0 LAX-F, 1 HLL) at a 27-field geometry record."""
+    L = lib()
+    L.o_face_flux.argtypes = [_I, _I, _D, _D, _P, _P, _P, _P, _P]
+    L.o_face_flux.restype = None
+    rec = _c(rec)
+    pl = _c(p_l).copy()
+    pr = _c(p_r).copy()
+    F = np.empty(8)
+    ct = _D()
+    L.o_face_flux(int(flux), int(dir), gam, gammamax, rec.ctypes.data, pl.ctypes.data, pr.ctypes.data,
+                  F.ctypes.data, ctypes.byref(ct))
+    return F, ct.value
+
+
 # ---------------------------------------------------------------------------
 # grid-level oracle (o_grid.c)
 # ---------------------------------------------------------------------------

@@ -1,0 +1,285 @@
+"""Pins the oracle's grid-level pieces (CPU, no GPU): reconstruction, the face
+Riemann solver, the RK tables, boundary zones, flux-CT and the update.
+
+Each check is a known answer or invariant that a plausible mistake breaks:
+  * PARA and WENO5 reproduce linear (and, away from extrema, quadratic) data
+    exactly, flatten a local extremum (PARA), pick the smooth sub-stencil next
+    to a jump (WENO5), are mirror-symmetric, and PARA's limited parabola is
+    monotone in the cell (Colella & Woodward 1984);
+  * the Riemann solver is consistent (equal states give the physical flux),
+    HLL is the upwind flux for supersonic flow and equals LAX-F for a
+    symmetric wave fan, and both diffuse mass down a density jump;
+  * the RK tables reproduce the Taylor amplification factor of dU/dt = z U;
+  * ghost zones: phi periodicity, polar reflection through the axis with the
+    phi rotation by pi and the theta-vector sign flips;
+  * flux-CT keeps Toth's corner divergence of B at round-off over many steps;
+  * on the Fishbone-Moncrief torus with B = 0 (an exact hydrostatic
+    equilibrium) the flux divergence and the connection source cancel in the
+    momentum equations up to truncation error;
+  * the connection obeys Gamma^j_{ij} = d_i ln gdet.
+"""
+import numpy as np
+import pytest
+
+import _oracle
+import harm_harm_amd as H
+
+FLAT = np.array([-1, 0, 0, 0, 1, 0, 0, 1, 0, 1] * 2 + [0.0, 1.0, 1.0, 0.0, 0.0, 0.0, 0.0], dtype=float)
+PARA, WENO5 = H.HARM_RECON_PARA if hasattr(H, "HARM_RECON_PARA") else 2, 3
+
+
+# ---------------------------------------------------------------------------
+# reconstruction
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("recon", [2, 3])
+def test_recon_linear_exact(recon):
+    rng = np.random.default_rng(0)
+    for _ in range(200):
+        a, b = rng.uniform(-5, 5, 2)
+        y = a + b * np.arange(-2, 3)
+        l, r = _oracle.recon_cell(recon, y)
+        assert abs(l - (a - 0.5 * b)) <= 1e-13 * (abs(a) + abs(b))
+        assert abs(r - (a + 0.5 * b)) <= 1e-13 * (abs(a) + abs(b))
+
+
+@pytest.mark.parametrize("recon", [2, 3])
+def test_recon_quadratic_exact_away_from_extremum(recon):
+    for c in (7.0, -9.0, 15.5):
+        f = lambda x: 0.3 * (x + c) ** 2 + 1.0
+        y = f(np.arange(-2, 3, dtype=float))
+        l, r = _oracle.recon_cell(recon, y)
+        assert abs(l - f(-0.5)) <= 1e-12 * abs(f(-0.5))
+        assert abs(r - f(0.5)) <= 1e-12 * abs(f(0.5))
+
+
+def test_para_flattens_extremum():
+    for y in ([1.0, 2.0, 3.0, 2.0, 1.0], [5.0, 4.0, 3.5, 4.5, 6.0]):
+        l, r = _oracle.recon_cell(2, y)
+        assert l == y[2] and r == y[2]
+
+
+def test_para_parabola_monotone():
+    """After the CW84 limiter the cell parabola P(x) = l + x (dq + a6 (1 - x)),
+    a6 = 4 (y0 - (l + r)/2), has no extremum inside the cell."""
+    rng = np.random.default_rng(1)
+    for _ in range(2000):
+        y = np.cumsum(rng.exponential(1.0, 5)) * rng.choice([-1, 1])
+        y[2] += rng.normal(0, 0.7)
+        l, r = _oracle.recon_cell(2, y)
+        dq, a6 = r - l, 4.0 * (y[2] - 0.5 * (l + r))
+        d0, d1 = dq + a6, dq - a6  # P'(0), P'(1)
+        assert d0 * d1 >= -1e-12 * (abs(d0) + abs(d1)) ** 2
+
+
+def test_weno5_picks_smooth_stencil_at_jump():
+    l, r = _oracle.recon_cell(3, [0.0, 0.0, 0.0, 1.0, 1.0])
+    assert r == 0.0 and l == 0.0
+    l, r = _oracle.recon_cell(3, [0.0, 0.0, 1.0, 1.0, 1.0])
+    assert l == 1.0 and r == 1.0
+
+
+@pytest.mark.parametrize("recon", [2, 3])
+def test_recon_mirror_symmetric(recon):
+    rng = np.random.default_rng(2)
+    for _ in range(500):
+        y = rng.normal(size=5)
+        l, r = _oracle.recon_cell(recon, y)
+        lm, rm = _oracle.recon_cell(recon, y[::-1].copy())
+        assert abs(l - rm) <= 1e-13 * (1 + abs(l)) and abs(r - lm) <= 1e-13 * (1 + abs(r))
+
+
+# ---------------------------------------------------------------------------
+# Riemann solver (flat space)
+# ---------------------------------------------------------------------------
+def prim(rho, u, v1=0.0, v2=0.0, v3=0.0, B1=0.0, B2=0.0, B3=0.0):
+    return np.array([rho, u, v1, v2, v3, B1, B2, B3], dtype=float)
+
+
+@pytest.mark.parametrize("flux", [0, 1])
+@pytest.mark.parametrize("dir", [1, 2, 3])
+def test_riemann_consistent(flux, dir):
+    rng = np.random.default_rng(3)
+    for _ in range(50):
+        p = prim(rng.uniform(0.1, 2), rng.uniform(0.01, 1), *rng.uniform(-1, 1, 3), *rng.uniform(-0.5, 0.5, 3))
+        F, ct = _oracle.face_flux(flux, dir, FLAT, p, p)
+        Fphys = _oracle.flux_batch(p[:, None], FLAT[:, None], dir)[:, 0]
+        assert np.allclose(F, Fphys, rtol=1e-13, atol=1e-13)
+        assert ct > 0
+
+
+def test_hll_upwind_supersonic():
+    # cold, fast flow in +x on both sides of a density jump: every wave moves
+    # right, so HLL must return the left state's flux exactly
+    pl = prim(1.0, 1e-4, 3.0)
+    pr = prim(0.3, 2e-4, 2.5)
+    F, _ = _oracle.face_flux(1, 1, FLAT, pl, pr)
+    Fl = _oracle.flux_batch(pl[:, None], FLAT[:, None], 1)[:, 0]
+    assert np.allclose(F, Fl, rtol=1e-14, atol=0)
+    Flax, _ = _oracle.face_flux(0, 1, FLAT, pl, pr)
+    assert not np.allclose(Flax, Fl, rtol=1e-6)
+    # and the mirror image picks the right state
+    F, _ = _oracle.face_flux(1, 1, FLAT, prim(1.0, 1e-4, -3.0), prim(0.3, 2e-4, -2.5))
+    Fr = _oracle.flux_batch(prim(0.3, 2e-4, -2.5)[:, None], FLAT[:, None], 1)[:, 0]
+    assert np.allclose(F, Fr, rtol=1e-14, atol=0)
+
+
+def test_hll_equals_laxf_for_symmetric_fan():
+    pl = prim(1.0, 0.5, 0.2)
+    pr = prim(1.0, 0.5, -0.2)
+    Fh, _ = _oracle.face_flux(1, 1, FLAT, pl, pr)
+    Fl, _ = _oracle.face_flux(0, 1, FLAT, pl, pr)
+    assert np.allclose(Fh, Fl, rtol=1e-14, atol=1e-15)
+    assert abs(Fh[0]) < 1e-15  # no net mass flux between colliding mirror streams
+    assert Fh[2] > 0  # momentum flux (pressure + ram) is positive
+
+
+@pytest.mark.parametrize("flux", [0, 1])
+def test_riemann_diffuses_mass_down_a_jump(flux):
+    F, _ = _oracle.face_flux(flux, 1, FLAT, prim(1.0, 0.1), prim(0.5, 0.1))
+    assert F[0] > 0
+    F, _ = _oracle.face_flux(flux, 1, FLAT, prim(0.5, 0.1), prim(1.0, 0.1))
+    assert F[0] < 0
+
+
+# ---------------------------------------------------------------------------
+# RK tables
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("timeorder", [2, 3])
+def test_rk_tables_taylor(timeorder):
+    """Apply UFSET/UCUMUPDATE with the tables to dU/dt = z U (U = p)."""
+    cfg = H.default_config(8, 8, 1, recon="para", flux="laxf", timeorder=timeorder)
+    n, CUf, Cunew = _oracle.OGrid(cfg).rk_tables()
+    assert n == timeorder
+    for z in (0.1, -0.37, 0.8):
+        U0 = 1.0
+        Ui, uf, ucum, pb = U0, 0.0, 0.0, U0
+        for s in range(n):
+            if s == 0:
+                uf, ucum = 0.0, 0.0
+            dU = z * pb
+            uf = CUf[s][0] * Ui + CUf[s][1] * uf + CUf[s][2] * dU
+            ucum += Cunew[s][0] * Ui + Cunew[s][1] * dU + Cunew[s][2] * uf
+            pb = ucum if s == n - 1 else uf
+        taylor = sum(z ** m / np.prod(range(1, m + 1)) for m in range(timeorder + 1))
+        assert abs(pb - taylor) <= 1e-15 * abs(taylor)
+
+
+# ---------------------------------------------------------------------------
+# grid-level: boundaries, flux-CT, equilibrium, connection
+# ---------------------------------------------------------------------------
+def grid(n, **kw):
+    cfg = H.default_config(*n, **kw)
+    return cfg, _oracle.OGrid(cfg, nthreads=4)
+
+
+def test_bound_periodic_and_polar_reflection():
+    cfg, og = grid((16, 12, 8), recon="weno5", flux="hll", timeorder=3)
+    p = og.init_torus()
+    n1, n2, n3 = cfg.n1, cfg.n2, cfg.n3
+    g1, g2, g3 = og.ng1, og.ng2, og.ng3
+    P = lambda i, j, k: p[k + g3, j + g2, i + g1]
+    for i in range(-g1, n1 + g1):
+        for j in range(-g2, n2 + g2):
+            for m in range(1, g3 + 1):
+                assert np.array_equal(P(i, j, -m), P(i, j, n3 - m))
+                assert np.array_equal(P(i, j, n3 - 1 + m), P(i, j, m - 1))
+    flip = np.array([1, 1, 1, -1, 1, 1, -1, 1.0])
+    for i in range(-g1, n1 + g1):
+        for k in range(n3):
+            rk = (k + n3 // 2) % n3
+            for m in range(1, g2 - 1):  # ghosts beyond the pole-death layer
+                assert np.array_equal(P(i, -1 - m, k), P(i, m, rk) * flip)
+                assert np.array_equal(P(i, n2 + m, k), P(i, n2 - 1 - m, rk) * flip)
+            # pole death: density and internal energy copied from the next zone out
+            for j, jr in ((-1, 1), (0, 1), (n2 - 1, n2 - 2), (n2, n2 - 2)):
+                assert np.array_equal(P(i, j, k)[:2], P(i, jr, k)[:2])
+
+
+def divb_corner(og, p):
+    """Toth's corner divergence of gdet B, normalised by |gdet B| / dx."""
+    n1, n2, n3 = og.cfg.n1, og.cfg.n2, og.cfg.n3
+    gd = og.geom[0][..., 21][None]
+    B = [p[..., 5 + m] * gd for m in range(3)]
+    dx = og.dx
+    nk = max(n3 - (n3 > 1), 1)
+    sl = lambda a, di, dj, dk: a[og.ng3 + dk:og.ng3 + dk + nk, og.ng2 + dj:og.ng2 + dj + n2 - 1,
+                                 og.ng1 + di:og.ng1 + di + n1 - 1]
+    ks = (0, 1) if n3 > 1 else (0,)
+    d = 0.0
+    for c, s in ((0, dx[1]), (1, dx[2]), (2, dx[3])):
+        if c == 2 and n3 == 1:
+            continue
+        acc = 0.0
+        for di in (0, 1):
+            for dj in (0, 1):
+                for dk in ks:
+                    acc = acc + ([di, dj, dk][c] * 2 - 1) * sl(B[c], di, dj, dk)
+        d = d + acc / (len(ks) * 2 * s)
+    scale = max(np.abs(b).max() for b in B) / min(dx[1], dx[2])
+    return np.abs(d).max() / scale
+
+
+@pytest.mark.parametrize("n,kw", [((32, 32, 1), dict(recon="para", flux="laxf", timeorder=2)),
+                                  ((16, 12, 8), dict(recon="weno5", flux="hll", timeorder=3))])
+def test_flux_ct_keeps_divb(n, kw):
+    cfg, og = grid(n, **kw)
+    p = og.init_torus()
+    d0 = divb_corner(og, p)
+    dt = 0.5 * min(og.fluxcalc(p, og.zeros(), og.zeros(), og.zeros()))
+    st = og.new_state(p)
+    for _ in range(10):
+        dt = og.step(st, dt)
+    d1 = divb_corner(og, st["p"])
+    assert d1 < 1e-12 and d1 < max(10 * d0, 1e-13)
+
+
+def test_torus_equilibrium_flux_balances_source():
+    """B = 0 torus: -div F + source = 0 analytically; numerically the residual
+    of the momentum equations is a small fraction of either term."""
+    cfg, og = grid((64, 64, 1), recon="para", flux="laxf", timeorder=2)
+    p = og.init_torus(randfact=0.0)
+    p[..., 5:8] = 0.0
+    og.bound(p)
+    F = [og.zeros() for _ in range(3)]
+    og.fluxcalc(p, *F, do_ct=True)
+    z = og.zeros
+    dU = z()
+    og.update(p, p, *F, z(), dU, z(), [0.0, 0.0, 1.0, 0.0], [0.0, 0.0, 0.0, 0.0], 1.0, 0)
+    og.conn[...] = 0.0  # same grid, connection switched off: flux divergence only
+    dUf = z()
+    og.update(p, p, *F, z(), dUf, z(), [0.0, 0.0, 1.0, 0.0], [0.0, 0.0, 0.0, 0.0], 1.0, 0)
+    a = lambda x: x[0, og.ng2:og.ng2 + cfg.n2, og.ng1:og.ng1 + cfg.n1]
+    rho = a(p)[..., 0]
+    inside = rho > 0.3 * rho.max()
+    assert inside.sum() > 50
+    for comp in (2, 3):  # U_r and U_theta momentum
+        res = np.abs(a(dU)[..., comp][inside])
+        term = np.abs(a(dUf)[..., comp][inside])
+        ratio = np.median(res) / np.median(term)
+        print("component", comp, "median |residual| / |flux term|", ratio)
+        assert ratio < 0.05
+
+
+def test_connection_trace_is_gradient_of_log_gdet():
+    """Gamma^a_{a i} = d_i ln|gdet|, with the derivative taken by a 4th-order
+    central difference over face (x -+ dx/2) and centre (x -+ dx) values, away
+    from the poles where ln sin(theta) is too steep for the difference."""
+    cfg, og = grid((64, 64, 1), recon="para", flux="laxf", timeorder=2)
+    lg = lambda loc: np.log(np.abs(og.geom[loc][..., 21]) + 1e-300)
+    lc, l1, l2 = lg(0), lg(1), lg(2)
+    conn = og.conn
+    g1, g2 = og.ng1, og.ng2
+    sym = {(0, 0): 0, (0, 1): 1, (0, 2): 2, (0, 3): 3, (1, 1): 4, (1, 2): 5, (1, 3): 6, (2, 2): 7, (2, 3): 8,
+           (3, 3): 9}
+    tr = lambda j, i, b: sum(conn[j, i, a * 10 + sym[(min(a, b), max(a, b))]] for a in range(4))
+    worst = 0.0
+    for i in range(g1 + 1, g1 + cfg.n1 - 1, 3):
+        for j in range(g2 + 8, g2 + cfg.n2 - 8, 3):
+            h1, h2 = og.dx[1] / 2, og.dx[2] / 2
+            d1 = (8 * (l1[j, i + 1] - l1[j, i]) - (lc[j, i + 1] - lc[j, i - 1])) / (12 * h1)
+            d2 = (8 * (l2[j + 1, i] - l2[j, i]) - (lc[j + 1, i] - lc[j - 1, i])) / (12 * h2)
+            e1 = abs(tr(j, i, 1) - d1) / (abs(d1) + 1.0)
+            e2 = abs(tr(j, i, 2) - d2) / (abs(d2) + 1.0)
+            worst = max(worst, e1, e2)
+    print("worst relative mismatch", worst)
+    assert worst < 1e-4
