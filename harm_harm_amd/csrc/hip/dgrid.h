@@ -83,72 +83,62 @@ __device__ __forceinline__ void para4(const double* y, double& lout, double& rou
   rout = r;
 }
 
-__device__ __forceinline__ double symsum3(const double* a) { return (0.0 + (a[0] + a[2])) + a[1] * 1; }
-__device__ __forceinline__ double symsum5(const double* a) { return ((0.0 + (a[0] + a[4])) + (a[1] + a[3])) + a[2] * 1; }
-
+// limit_weight (reconstructeno.c:3015) for three weights: zero below 0.2/3,
+// a linear ramp to 1/3, flat to 1.2/3, then linear to 1 at w = 1.  The
+// constant quotients are folded into multipliers.
 __device__ __forceinline__ double limit_weight3(double w)
 {
-  const double avg = 1. / 3.;
-  const double w1 = 0.2 * avg, w2 = (1.0 - 0.2) * avg, w3 = (1.0 + 0.2) * avg, w4 = 1.0;
+  constexpr double avg = 1. / 3.;
+  constexpr double w1 = 0.2 * avg, w2 = (1.0 - 0.2) * avg, w3 = (1.0 + 0.2) * avg, w4 = 1.0;
+  constexpr double k12 = 1.0 / (w2 - w1), k34 = (w4 - avg) / (w4 - w3);
   if (w < w1) return 0.0;
-  if (w < w2) return avg * (w - w1) / (w2 - w1);
+  if (w < w2) return (avg * (w - w1)) * k12;
   if (w < w3) return avg;
-  return (w - w3) * (w4 - avg) / (w4 - w3) + avg;
+  return (w - w3) * k34 + avg;
 }
 
-// WENO5 centre-to-edge core (point values).
+// WENO5 centre-to-edge reconstruction of one cell from point values
+// u[-2..2] (reconstructeno.c:1693-2030 weights, :2211-2330 indicators, :3015
+// limiter): Jiang-Shu indicators of the three 3-point sub-stencils, a
+// value-proportional epsilon, indicators normalised by their sum, nonlinear
+// weights 1/(beta+eps)^2 normalised and limited, then combined with the
+// optimal edge weights {5/16, 5/8, 1/16} (mirrored for the left edge).
+// Both edges share the indicators, so a cell is reconstructed once.  The
+// arithmetic is arranged for few divisions: one reciprocal per
+// normalisation, the three weights 1/t_s normalised through the pairwise
+// products t_a t_b, and the second weight normalisation folded into the
+// edge normalisation (it cancels there).  Results differ from the
+// divide-per-element form by a few ulps.
 __device__ __forceinline__ void weno5(const double* u, double& lout, double& rout)
 {
-  const double EPS = (500.0 * DBL_EPSILON) * (500.0 * DBL_EPSILON);
-  double b[3], d1[3], d2[3];
-  d1[0] = -(3 * u[0] + u[2]) + 4 * u[1];
-  d1[1] = u[1] - u[-1];
-  d1[2] = (3 * u[0] + u[-2]) - 4 * u[-1];
-  d2[0] = (u[0] + u[2]) - 2 * u[1];
-  d2[1] = (u[-1] + u[1]) - 2 * u[0];
-  d2[2] = (u[0] + u[-2]) - 2 * u[-1];
-#pragma unroll
-  for (int s = 0; s < 3; s++) b[s] = 13. / 12. * d2[s] * d2[s] + 1. / 4. * d1[s] * d1[s];
-  double u2[5];
-#pragma unroll
-  for (int i = -2; i <= 2; i++) u2[i + 2] = u[i] * u[i];
-  const double err = symsum5(u2) * EPS;
-#pragma unroll
-  for (int s = 0; s < 3; s++) b[s] += err;
-  const double sb = symsum3(b);
-#pragma unroll
-  for (int s = 0; s < 3; s++) b[s] /= (sb + DBL_MIN);
-  double w[3];
-#pragma unroll
-  for (int s = 0; s < 3; s++) {
-    double t = b[s] + EPS / 1.;
-    t *= t;
-    w[s] = 1. / t;
-  }
-  double sw = symsum3(w);
-#pragma unroll
-  for (int s = 0; s < 3; s++) w[s] /= (sw + 0);
-#pragma unroll
-  for (int s = 0; s < 3; s++) w[s] = limit_weight3(w[s]);
-  sw = symsum3(w);
-#pragma unroll
-  for (int s = 0; s < 3; s++) w[s] /= (sw + 0);
-  double wr[3] = {w[0] * (5.0 / 16.0), w[1] * (5.0 / 8.0), w[2] * (1.0 / 16.0)};
-  double wl[3] = {w[0] * (1.0 / 16.0), w[1] * (5.0 / 8.0), w[2] * (5.0 / 16.0)};
-  sw = symsum3(wr);
-#pragma unroll
-  for (int s = 0; s < 3; s++) wr[s] /= (sw + DBL_MIN);
-  sw = symsum3(wl);
-#pragma unroll
-  for (int s = 0; s < 3; s++) wl[s] /= (sw + DBL_MIN);
-  const double qr0 = (3 * u[0] + 6 * u[1] - u[2]) / 8.0;
-  const double qr1 = (-u[-1] + 6 * u[0] + 3 * u[1]) / 8.0;
-  const double qr2 = (3 * u[-2] - 10 * u[-1] + 15 * u[0]) / 8.0;
-  const double ql0 = (15 * u[0] - 10 * u[1] + 3 * u[2]) / 8.0;
-  const double ql1 = (3 * u[-1] + 6 * u[0] - u[1]) / 8.0;
-  const double ql2 = (-u[-2] + 6 * u[-1] + 3 * u[0]) / 8.0;
-  rout = (wr[0] * qr0 + wr[2] * qr2) + wr[1] * qr1;
-  lout = (wl[0] * ql0 + wl[2] * ql2) + wl[1] * ql1;
+  constexpr double EPS = (500.0 * DBL_EPSILON) * (500.0 * DBL_EPSILON);
+  const double d10 = -(3 * u[0] + u[2]) + 4 * u[1];
+  const double d11 = u[1] - u[-1];
+  const double d12 = (3 * u[0] + u[-2]) - 4 * u[-1];
+  const double d20 = (u[0] + u[2]) - 2 * u[1];
+  const double d21 = (u[-1] + u[1]) - 2 * u[0];
+  const double d22 = (u[0] + u[-2]) - 2 * u[-1];
+  const double err = (((u[-2] * u[-2] + u[2] * u[2]) + (u[-1] * u[-1] + u[1] * u[1])) + u[0] * u[0]) * EPS;
+  double b0 = (13. / 12. * d20 * d20 + 1. / 4. * d10 * d10) + err;
+  double b1 = (13. / 12. * d21 * d21 + 1. / 4. * d11 * d11) + err;
+  double b2 = (13. / 12. * d22 * d22 + 1. / 4. * d12 * d12) + err;
+  const double isb = 1.0 / ((b0 + b2) + b1 + DBL_MIN);
+  const double t0 = (b0 * isb + EPS) * (b0 * isb + EPS);
+  const double t1 = (b1 * isb + EPS) * (b1 * isb + EPS);
+  const double t2 = (b2 * isb + EPS) * (b2 * isb + EPS);
+  const double q0 = t1 * t2, q1 = t0 * t2, q2 = t0 * t1;
+  const double iq = 1.0 / ((q0 + q2) + q1);
+  const double w0 = limit_weight3(q0 * iq), w1 = limit_weight3(q1 * iq), w2 = limit_weight3(q2 * iq);
+  const double wr0 = w0 * (5.0 / 16.0), wr1 = w1 * (5.0 / 8.0), wr2 = w2 * (1.0 / 16.0);
+  const double wl0 = w0 * (1.0 / 16.0), wl1 = w1 * (5.0 / 8.0), wl2 = w2 * (5.0 / 16.0);
+  const double qr0 = (3 * u[0] + 6 * u[1] - u[2]) * 0.125;
+  const double qr1 = (-u[-1] + 6 * u[0] + 3 * u[1]) * 0.125;
+  const double qr2 = (3 * u[-2] - 10 * u[-1] + 15 * u[0]) * 0.125;
+  const double ql0 = (15 * u[0] - 10 * u[1] + 3 * u[2]) * 0.125;
+  const double ql1 = (3 * u[-1] + 6 * u[0] - u[1]) * 0.125;
+  const double ql2 = (-u[-2] + 6 * u[-1] + 3 * u[0]) * 0.125;
+  rout = ((wr0 * qr0 + wr2 * qr2) + wr1 * qr1) / ((wr0 + wr2) + wr1);
+  lout = ((wl0 * ql0 + wl2 * ql2) + wl1 * ql1) / ((wl0 + wl2) + wl1);
 }
 
 template <int RECON>

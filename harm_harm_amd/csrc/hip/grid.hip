@@ -96,66 +96,180 @@ Box face_box(const Dims& d, int dir)
   return b;
 }
 
-template <int DIR, int RECON>
-__global__ __launch_bounds__(kBlock) void k_flux(Dims d, Box box, const double* __restrict__ pb, double* __restrict__ F,
-                                                 const double* __restrict__ gface, Params P, int fluxtype, int do_ct,
-                                                 double* __restrict__ ds)
+// Face tiles.  A block first reconstructs every cell its faces touch, one
+// (cell, variable) pair per lane, into LDS (each cell once, both edges), then
+// solves one face per lane.  Direction 1 runs along i, the contiguous
+// dimension: a block takes `fpb` consecutive faces of the flattened face box
+// (rows of ni faces own ni+1 cells, so a block needs fpb + rows + 1 cells).
+// Directions 2 and 3 take a tile of tc columns (consecutive i, then the other
+// transverse index) by td faces along the sweep, i.e. (td+1)*tc cells.
+constexpr int kCellMax = 320;
+
+struct FaceTile {
+  Box b;
+  int ncol;      // dir 1: rows (nj*nk); dirs 2,3: columns (ni*nk or ni*nj)
+  int nd;        // faces along the sweep
+  int fpb;       // dir 1: faces per block
+  int tc, td;    // dirs 2,3: tile shape
+  int ntc, ntd;  // dirs 2,3: tiles along columns / along the sweep
+  long nblk;
+};
+
+// Blocks are dealt to the 8 XCDs round-robin; remap so that each XCD works on
+// a contiguous range of tiles (neighbouring tiles share stencil halos in L2).
+__device__ __forceinline__ int xcd_tile(int b, int n)
 {
-  int i, j, k;
+  const int x = b & 7, loc = b >> 3, q = n >> 3, r = n & 7;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + loc;
+}
+
+// Riemann flux of one face from its two reconstructed states
+// (getplpr/check_plpr flux.c:1952-2110, flux_compute fluxcompute.c:19-402,
+// cminmax_calc wavespeeds.c:397).  Returns ctop.
+template <int DIR>
+__device__ __forceinline__ double face_riemann(double* p_l, double* p_r, const Geom& G, const Params& P, int fluxtype,
+                                               int do_ct, double* Fo)
+{
+  const int pbd = HARM_B1 + DIR - 1;
+  p_l[pbd] = p_r[pbd] = 0.5 * (p_l[pbd] + p_r[pbd]);
+  hd::limit_gamma(P.gammamax, p_l, G);
+  hd::limit_gamma(P.gammamax, p_r, G);
+  double F_l[HARM_NPR], U_l[HARM_NPR], F_r[HARM_NPR], U_r[HARM_NPR];
+  double cmaxl, cminl, cmaxr, cminr;
+  {
+    State q;
+    hd::get_state(p_l, G, P.gam, q);
+    hd::primtoflux(p_l, q, DIR, G, F_l);
+    hd::primtoflux(p_l, q, 0, G, U_l);
+    hd::vchar(p_l, q, DIR, G, P.gam, cmaxl, cminl);
+  }
+  {
+    State q;
+    hd::get_state(p_r, G, P.gam, q);
+    hd::primtoflux(p_r, q, DIR, G, F_r);
+    hd::primtoflux(p_r, q, 0, G, U_r);
+    hd::vchar(p_r, q, DIR, G, P.gam, cmaxr, cminr);
+  }
+  double lmin = fmin(cminl, cminr), lmax = fmax(cmaxl, cmaxr);
+  lmin = fabs(fmax(0., -lmin));
+  lmax = fabs(fmax(0., lmax));
+  const double cmin = lmin, cmax = lmax, ct = fmax(lmax, lmin);
+  if (fluxtype == HARM_FLUX_HLL && cmax + cmin != 0.0) {
+    const double ic = 1.0 / (cmax + cmin);
+#pragma unroll
+    for (int pl = 0; pl < HARM_NPR; pl++)
+      Fo[pl] = ((cmax * F_l[pl] + cmin * F_r[pl]) - (cmax * cmin) * (U_r[pl] - U_l[pl])) * ic;
+  } else {
+#pragma unroll
+    for (int pl = 0; pl < HARM_NPR; pl++) Fo[pl] = (0.5 * ((F_l[pl] + F_r[pl]) - ct * (U_r[pl] - U_l[pl])));
+  }
+  if (do_ct) {
+    const double ign = hd::igdetnosing(G.gdet);
+    Fo[HARM_B1] *= ign;
+    Fo[HARM_B2] *= ign;
+    Fo[HARM_B3] *= ign;
+  }
+  return ct;
+}
+
+template <int DIR, int RECON>
+__global__ __launch_bounds__(kBlock) void k_flux(Dims d, FaceTile T, const double* __restrict__ pb,
+                                                 double* __restrict__ F, const double* __restrict__ gface, Params P,
+                                                 int fluxtype, int do_ct, double* __restrict__ ds)
+{
+  __shared__ double s_l[HARM_NPR * kCellMax], s_r[HARM_NPR * kCellMax];
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int st = (DIR == 1) ? 1 : (DIR == 2) ? d.ns1 : d.ns1 * d.ns2;
+  const Box& b = T.b;
+
+  // ---- phase 1: reconstruct the tile's cells ----
+  // (32-bit index arithmetic: a face box stays far below 2^31 faces)
+  int f0 = 0, g_lo = 0, ncell, bc = 0, bd = 0;
+  const int rowc = T.nd + 1;  // dir 1: cells per row
+  if (DIR == 1) {
+    const int nf = T.nd * T.ncol;
+    f0 = tile * T.fpb;
+    const int fl = min(f0 + T.fpb, nf) - 1;
+    g_lo = (f0 / T.nd) * rowc + f0 % T.nd;
+    ncell = (fl / T.nd) * rowc + fl % T.nd + 1 - g_lo + 1;
+  } else {
+    bd = tile % T.ntd;
+    bc = tile / T.ntd;
+    ncell = (T.td + 1) * T.tc;
+  }
+  for (int it = threadIdx.x; it < ncell * HARM_NPR; it += kBlock) {
+    const int v = it / ncell, slot = it - v * ncell;
+    int i, j, k;
+    if (DIR == 1) {
+      const int g = g_lo + slot;
+      const int row = g / rowc;
+      i = b.is - 1 + (g - row * rowc);
+      const int rk = row / b.nj;
+      j = b.js + (row - rk * b.nj);
+      k = b.ks + rk;
+    } else {
+      const int dd = slot / T.tc, cc = slot - dd * T.tc;
+      const int c = bc * T.tc + cc;
+      const int pd = bd * T.td - 1 + dd;  // sweep index of the cell relative to the first face
+      if (c >= T.ncol || pd > T.nd - 1) continue;
+      const int co = c / b.ni;
+      i = b.is + (c - co * b.ni);
+      if (DIR == 2) { k = b.ks + co; j = b.js + pd; }
+      else { j = b.js + co; k = b.ks + pd; }
+    }
+    const double* a = pb + (long)v * d.ncell + hd::cidx(d, i, j, k);
+    double y[5];
+#pragma unroll
+    for (int m = 0; m < 5; m++) y[m] = a[(m - 2) * st];
+    double l, r;
+    hd::recon_cell<RECON>(y + 2, l, r);
+    s_l[v * kCellMax + slot] = l;
+    s_r[v * kCellMax + slot] = r;
+  }
+  __syncthreads();
+
+  // ---- phase 2: one face per lane ----
   double dtij = kBIG;
-  const long t = (long)blockIdx.x * kBlock + threadIdx.x;
-  if (box.at(t, i, j, k)) {
-    Geom G;
-    hd::load_geom(gface, hd::pidx(d, i, j), d.nplane, G);
-    const long c = hd::cidx(d, i, j, k);
-    const long st = (DIR == 1) ? 1 : (DIR == 2) ? (long)d.ns1 : (long)d.ns1 * d.ns2;
+  int i, j, k, sl = 0, sr = 0;
+  bool ok;
+  if (DIR == 1) {
+    const int f = f0 + (int)threadIdx.x;
+    ok = (int)threadIdx.x < T.fpb && f < T.nd * T.ncol;
+    if (ok) {
+      const int row = f / T.nd;
+      const int fi = f - row * T.nd;
+      sr = row * rowc + fi + 1 - g_lo;
+      sl = sr - 1;
+      const int rk = row / b.nj;
+      i = b.is + fi;
+      j = b.js + (row - rk * b.nj);
+      k = b.ks + rk;
+    }
+  } else {
+    const int dd = (int)threadIdx.x / T.tc, cc = (int)threadIdx.x - dd * T.tc;
+    const int c = bc * T.tc + cc, fd = bd * T.td + dd;
+    ok = dd < T.td && c < T.ncol && fd < T.nd;
+    if (ok) {
+      sl = dd * T.tc + cc;
+      sr = sl + T.tc;
+      const int co = c / b.ni;
+      i = b.is + (c - co * b.ni);
+      if (DIR == 2) { k = b.ks + co; j = b.js + fd; }
+      else { j = b.js + co; k = b.ks + fd; }
+    }
+  }
+  if (ok) {
     double p_l[HARM_NPR], p_r[HARM_NPR];
 #pragma unroll
-    for (int pl = 0; pl < HARM_NPR; pl++) {
-      const double* a = pb + (long)pl * d.ncell + c;
-      double y[6];
-#pragma unroll
-      for (int m = 0; m < 6; m++) y[m] = a[(m - 3) * st];
-      double l, r;
-      hd::recon_cell<RECON>(y + 2, l, r);  // cell c-1
-      p_l[pl] = r;
-      hd::recon_cell<RECON>(y + 3, l, r);  // cell c
-      p_r[pl] = l;
+    for (int v = 0; v < HARM_NPR; v++) {
+      p_l[v] = s_r[v * kCellMax + sl];
+      p_r[v] = s_l[v * kCellMax + sr];
     }
-    const int pbd = HARM_B1 + DIR - 1;
-    p_l[pbd] = p_r[pbd] = 0.5 * (p_l[pbd] + p_r[pbd]);
-    hd::limit_gamma(P.gammamax, p_l, G);
-    hd::limit_gamma(P.gammamax, p_r, G);
-    State ql, qr;
-    double F_l[HARM_NPR], F_r[HARM_NPR], U_l[HARM_NPR], U_r[HARM_NPR];
-    hd::get_state(p_l, G, P.gam, ql);
-    hd::primtoflux(p_l, ql, DIR, G, F_l);
-    hd::primtoflux(p_l, ql, 0, G, U_l);
-    hd::get_state(p_r, G, P.gam, qr);
-    hd::primtoflux(p_r, qr, DIR, G, F_r);
-    hd::primtoflux(p_r, qr, 0, G, U_r);
-    double cmaxl, cminl, cmaxr, cminr;
-    hd::vchar(p_l, ql, DIR, G, P.gam, cmaxl, cminl);
-    hd::vchar(p_r, qr, DIR, G, P.gam, cmaxr, cminr);
-    double lmin = fmin(cminl, cminr), lmax = fmax(cmaxl, cmaxr);
-    lmin = fabs(fmax(0., -lmin));
-    lmax = fabs(fmax(0., lmax));
-    const double cmin = lmin, cmax = lmax, ct = fmax(lmax, lmin);
+    Geom G;
+    hd::load_geom(gface, hd::pidx(d, i, j), d.nplane, G);
     double Fo[HARM_NPR];
-    if (fluxtype == HARM_FLUX_HLL && cmax + cmin != 0.0) {
-#pragma unroll
-      for (int pl = 0; pl < HARM_NPR; pl++)
-        Fo[pl] = ((cmax * F_l[pl] + cmin * F_r[pl]) - (cmax * cmin) * (U_r[pl] - U_l[pl])) / (cmax + cmin);
-    } else {
-#pragma unroll
-      for (int pl = 0; pl < HARM_NPR; pl++) Fo[pl] = (0.5 * ((F_l[pl] + F_r[pl]) - ct * (U_r[pl] - U_l[pl])));
-    }
-    if (do_ct) {
-      const double ign = hd::igdetnosing(G.gdet);
-      Fo[HARM_B1] *= ign;
-      Fo[HARM_B2] *= ign;
-      Fo[HARM_B3] *= ign;
-    }
+    const double ct = face_riemann<DIR>(p_l, p_r, G, P, fluxtype, do_ct, Fo);
+    const long c = hd::cidx(d, i, j, k);
 #pragma unroll
     for (int pl = 0; pl < HARM_NPR; pl++) F[(long)pl * d.ncell + c] = Fo[pl];
     dtij = P.cour * P.dx[DIR] / ct;
@@ -884,6 +998,45 @@ harm_status check_which(int which)
 }
 
 // ---- enqueue helpers (no host synchronisation) ----
+// Tile shape of the face kernels.  Direction 1: the largest face count per
+// block whose cells fit the LDS tile.  Directions 2 and 3: td in {4, 8, 16}
+// (tc = 256/td) minimising blocks x (recon passes x 1 + face pass x 12), the
+// rough cost ratio of one (cell, variable) reconstruction to one face solve.
+FaceTile plan_face_tile(const Dims& d, int dir)
+{
+  FaceTile T{};
+  T.b = face_box(d, dir);
+  const Box& b = T.b;
+  if (dir == 1) {
+    T.nd = b.ni;
+    T.ncol = b.nj * b.nk;
+    int fpb = kBlock;
+    while (fpb > 1 && fpb + (fpb + T.nd - 1) / T.nd + 1 > kCellMax) fpb--;
+    T.fpb = fpb;
+    T.nblk = ((long)T.nd * T.ncol + fpb - 1) / fpb;
+    return T;
+  }
+  T.nd = dir == 2 ? b.nj : b.nk;
+  T.ncol = dir == 2 ? b.ni * b.nk : b.ni * b.nj;
+  double best = 1e300;
+  for (int td : {4, 8, 16}) {
+    const int tc = kBlock / td;
+    if ((td + 1) * tc > kCellMax) continue;
+    const long ntc = (T.ncol + tc - 1) / tc, ntd = (T.nd + td - 1) / td;
+    const int passes = ((td + 1) * tc * HARM_NPR + kBlock - 1) / kBlock;
+    const double cost = (double)(ntc * ntd) * (passes + 12.0);
+    if (cost < best) {
+      best = cost;
+      T.td = td;
+      T.tc = tc;
+      T.ntc = (int)ntc;
+      T.ntd = (int)ntd;
+    }
+  }
+  T.nblk = (long)T.ntc * T.ntd;
+  return T;
+}
+
 harm_status enq_fluxcalc(harm_handle* h, int which_p, int do_ct)
 {
   const Dims& d = h->d;
@@ -891,18 +1044,16 @@ harm_status enq_fluxcalc(harm_handle* h, int which_p, int do_ct)
   const double* pb = h->arr[which_p];
   hipLaunchKernelGGL(k_scal_begin_flux, dim3(1), dim3(1), 0, s, h->ds);
   const int recon = h->cfg.recon, ft = h->cfg.flux;
-#define LAUNCH_FLUX(DIR, F, G)                                                                                      \
-  do {                                                                                                              \
-    Box b = face_box(d, DIR);                                                                                       \
-    if (recon == HARM_RECON_WENO5)                                                                                  \
-      hipLaunchKernelGGL((k_flux<DIR, HARM_RECON_WENO5>), dim3(nblocks(b.count())), dim3(kBlock), 0, s, d, b, pb, F, \
-                         G, h->P, ft, do_ct, h->ds);                                                                \
-    else if (recon == HARM_RECON_PARA)                                                                              \
-      hipLaunchKernelGGL((k_flux<DIR, HARM_RECON_PARA>), dim3(nblocks(b.count())), dim3(kBlock), 0, s, d, b, pb, F, \
-                         G, h->P, ft, do_ct, h->ds);                                                                \
-    else                                                                                                            \
-      hipLaunchKernelGGL((k_flux<DIR, HARM_RECON_DONOR>), dim3(nblocks(b.count())), dim3(kBlock), 0, s, d, b, pb,   \
-                         F, G, h->P, ft, do_ct, h->ds);                                                             \
+#define LAUNCH_FLUX(DIR, F, G)                                                                                     \
+  do {                                                                                                             \
+    const FaceTile T = plan_face_tile(d, DIR);                                                                     \
+    const dim3 grid((unsigned)T.nblk), blk(kBlock);                                                                \
+    if (recon == HARM_RECON_WENO5)                                                                                 \
+      hipLaunchKernelGGL((k_flux<DIR, HARM_RECON_WENO5>), grid, blk, 0, s, d, T, pb, F, G, h->P, ft, do_ct, h->ds); \
+    else if (recon == HARM_RECON_PARA)                                                                             \
+      hipLaunchKernelGGL((k_flux<DIR, HARM_RECON_PARA>), grid, blk, 0, s, d, T, pb, F, G, h->P, ft, do_ct, h->ds);  \
+    else                                                                                                           \
+      hipLaunchKernelGGL((k_flux<DIR, HARM_RECON_DONOR>), grid, blk, 0, s, d, T, pb, F, G, h->P, ft, do_ct, h->ds); \
   } while (0)
   if (d.n1 > 1) { prof_begin(h); LAUNCH_FLUX(1, h->arr[HARM_ARR_F1], h->geom[HARM_LOC_FACE1]); prof_end(h, HARM_K_FLUX1); }
   if (d.n2 > 1) { prof_begin(h); LAUNCH_FLUX(2, h->arr[HARM_ARR_F2], h->geom[HARM_LOC_FACE2]); prof_end(h, HARM_K_FLUX2); }
