@@ -50,8 +50,8 @@ def algorithmic_bytes(g, kernel):
         # pi, pb, 3 face fluxes (8 each), ui write, uf r/w, ucum r/w
         return zones * (64 + 64 + 3 * 64 + 64 + 128 + 128) + n1 * n2 * (geom + 40 * 8)
     if kernel == "invert":
-        # U read, guess read, solution write, flag write (+ the pb->pf copy 128 B)
-        return zones * (64 + 64 + 64 + 4 + 128) + n1 * n2 * geom
+        # U read, guess read, solution write, flag write
+        return zones * (64 + 64 + 64 + 4) + n1 * n2 * geom
     if kernel == "emf_ct":
         return zones * (6 * 8 + 3 * 8 + 3 * 8 + 6 * 8)
     return None

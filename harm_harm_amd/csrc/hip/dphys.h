@@ -245,6 +245,38 @@ __device__ __forceinline__ void source(const double* pr, const Geom& g, const St
   dU[HARM_B3] = 0.0 + g.gdet * 0.0;
 }
 
+// The same connection source, evaluated one stress-tensor row at a time so
+// that only four T^j_k values are live: for each l the terms are added in the
+// same (j, k) order as in source(), so the sums are identical.  Returns the
+// four energy-momentum components (without the gdet factor applied to the
+// zero rest-mass and field entries) times gdet, and T^0_0 (rest mass removed,
+// the UU entry of U before the gdet factor).
+template <class ConnF>
+__device__ __forceinline__ void source_rows(const double* pr, const Geom& g, const State& q, ConnF conn,
+                                            double* dUl, double& T00)
+{
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    double ma[4], em[4], row[4];
+    mhd_parts(pr, j, g, q, ma, em);
+#pragma unroll
+    for (int k = 0; k < 4; k++) row[k] = (ma[k] + 0.0) + em[k];
+    if (j == 0) T00 = ma[0] + em[0];
+#pragma unroll
+    for (int l = 0; l < 4; l++)
+#pragma unroll
+      for (int k = 0; k < 4; k++) s[l] += row[k] * conn(k * 10 + sym(l, j));
+  }
+#pragma unroll
+  for (int l = 0; l < 4; l++) {
+    double v = s[l];
+#pragma unroll
+    for (int j = 0; j < 4; j++) v += -pr[HARM_RHO] * q.ucon[j] * conn(0 * 10 + sym(l, j));
+    dUl[l] = 0.0 + g.gdet * v;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Inversion: Jon's 1-D W' Newton (WHICHHOTINVERTER 3), no line search.
 // ---------------------------------------------------------------------------

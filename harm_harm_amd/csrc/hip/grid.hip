@@ -172,8 +172,9 @@ __device__ __forceinline__ double face_riemann(double* p_l, double* p_r, const G
   return ct;
 }
 
+// Three waves per SIMD: the LDS tile allows three blocks per CU.
 template <int DIR, int RECON>
-__global__ __launch_bounds__(kBlock) void k_flux(Dims d, FaceTile T, const double* __restrict__ pb,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_flux(Dims d, FaceTile T, const double* __restrict__ pb,
                                                  double* __restrict__ F, const double* __restrict__ gface, Params P,
                                                  int fluxtype, int do_ct, double* __restrict__ ds)
 {
@@ -345,6 +346,10 @@ __global__ __launch_bounds__(kBlock) void k_ct(Dims d, int ni, int nj, int nk, c
 // ---------------------------------------------------------------------------
 // update: Ui = U(pi); dU = -div F + source(pb); Uf, ucum; source dt limit
 // ---------------------------------------------------------------------------
+// DM: bit m-1 set when dimension m has more than one zone; a compile-time
+// mask keeps the flux-divergence loads free of branches so they issue
+// together.
+template <int DM>
 __global__ __launch_bounds__(kBlock) void k_update(Dims d, Params P, const double* __restrict__ pi,
                                                    const double* __restrict__ pb, const double* __restrict__ F1,
                                                    const double* __restrict__ F2, const double* __restrict__ F3,
@@ -354,90 +359,103 @@ __global__ __launch_bounds__(kBlock) void k_update(Dims d, Params P, const doubl
                                                    double cuf2, double cun0, double cun1, double cun2, int timeorder,
                                                    double* __restrict__ ds)
 {
-  const long t = (long)blockIdx.x * kBlock + threadIdx.x;
-  const long nact = (long)d.n1 * d.n2 * d.n3;
+  // Work is ordered so that few values are live at once: flux divergence,
+  // then everything that needs the state of pb (source, source dt), then the
+  // state of pi (Ui) and the RK combination.
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  const int nact = d.n1 * d.n2 * d.n3;
   double dts = kBIG;
   if (t < nact) {
-    const int i = (int)(t % d.n1), j = (int)((t / d.n1) % d.n2), k = (int)(t / ((long)d.n1 * d.n2));
+    const int r = t / d.n1, i = t - r * d.n1;
+    const int k = r / d.n2, j = r - k * d.n2;
     const long o = hd::cidx(d, i, j, k);
     const long pp = hd::pidx(d, i, j);
     const long nc = d.ncell;
     const double dt = ds[DS_DT];
     Geom G;
     hd::load_geom(gcent, pp, d.nplane, G);
-    double pz[HARM_NPR], Ui[HARM_NPR];
-#pragma unroll
-    for (int pl = 0; pl < HARM_NPR; pl++) pz[pl] = pi[pl * nc + o];
-    State q;
-    hd::get_state(pz, G, P.gam, q);
-    hd::primtoflux(pz, q, 0, G, Ui);
-#pragma unroll
-    for (int pl = 0; pl < HARM_NPR; pl++) ui[pl * nc + o] = Ui[pl];
+    // flux divergence (flux2dUavg)
     const double idx1 = 1.0 / P.dx[1], idx2 = 1.0 / P.dx[2], idx3 = 1.0 / P.dx[3];
     const long s2 = d.ns1, s3 = (long)d.ns1 * d.ns2;
-    double dUr[HARM_NPR];
-#pragma unroll
-    for (int pl = 0; pl < HARM_NPR; pl++) {
-      double dU1 = 0, dU2 = 0, dU3 = 0;
-      if (d.n1 > 1) dU1 = (-(F1[pl * nc + o + 1] - F1[pl * nc + o]) * idx1);
-      if (d.n2 > 1) dU2 = (-(F2[pl * nc + o + s2] - F2[pl * nc + o]) * idx2);
-      if (d.n3 > 1) dU3 = (-(F3[pl * nc + o + s3] - F3[pl * nc + o]) * idx3);
-      dUr[pl] = dU1 + dU2 + dU3;
-    }
-    double pbz[HARM_NPR], dUg[HARM_NPR];
-#pragma unroll
-    for (int pl = 0; pl < HARM_NPR; pl++) pbz[pl] = pb[pl * nc + o];
-    State qb;
-    hd::get_state(pbz, G, P.gam, qb);
-    const double* cz = conn + pp;
-    const long np = d.nplane;
-    hd::source(pbz, G, qb, [&](int c) { return cz[(long)c * np]; }, dUg);
     double dUt[HARM_NPR];
 #pragma unroll
     for (int pl = 0; pl < HARM_NPR; pl++) {
+      double dU1 = 0, dU2 = 0, dU3 = 0;
+      if (DM & 1) dU1 = (-(F1[pl * nc + o + 1] - F1[pl * nc + o]) * idx1);
+      if (DM & 2) dU2 = (-(F2[pl * nc + o + s2] - F2[pl * nc + o]) * idx2);
+      if (DM & 4) dU3 = (-(F3[pl * nc + o + s3] - F3[pl * nc + o]) * idx3);
+      dUt[pl] = dU1 + dU2 + dU3;
+    }
+    // keep the phases apart: the scheduler would otherwise hoist every load of
+    // the kernel to the top and run out of registers
+    __builtin_amdgcn_sched_barrier(0);
+    {
+      double pbz[HARM_NPR];
+#pragma unroll
+      for (int pl = 0; pl < HARM_NPR; pl++) pbz[pl] = pb[pl * nc + o];
+      State qb;
+      hd::get_state(pbz, G, P.gam, qb);
+      const double* cz = conn + pp;
+      const long np = d.nplane;
+      double T00;
+      double dUg[4];
+      hd::source_rows(pbz, G, qb, [&](int c) { return cz[(long)c * np]; }, dUg, T00);
+#pragma unroll
+      for (int l = 0; l < 4; l++) dUt[HARM_UU + l] = dUt[HARM_UU + l] + dUg[l];
+#pragma unroll
+      for (int pl = 0; pl < HARM_NPR; pl++)
+        if (pl < HARM_UU || pl > HARM_U3) dUt[pl] = dUt[pl] + (0.0 + G.gdet * 0.0);
+      // dUtodt -> compute_dt_fromsource
+      const double ig = hd::igdetnosing(G.gdet);
+      const double UggUU = (T00 * G.gdet) * ig;
+      double dUd[4], dUu[4], mydU[4], mydx[4];
+#pragma unroll
+      for (int jj = 0; jj < 4; jj++) dUd[jj] = (dUt[HARM_UU + jj] * dt) * ig;
+      hd::raise(dUd, G, dUu);
+      mydU[0] = dUd[0];
+      mydx[0] = dt;
+#pragma unroll
+      for (int jj = 1; jj < 4; jj++) { mydU[jj] = dUu[jj]; mydx[jj] = P.dx[jj]; }
+      double bcon[4], bcov[4];
+      bcon[0] = pbz[HARM_B1] * qb.ucov[1] + pbz[HARM_B2] * qb.ucov[2] + pbz[HARM_B3] * qb.ucov[3];
+#pragma unroll
+      for (int jj = 1; jj < 4; jj++) bcon[jj] = (pbz[HARM_B1 - 1 + jj] + bcon[0] * qb.ucon[jj]) / qb.ucon[0];
+      hd::lower(bcon, G, bcov);
+      const double bsq = bcon[0] * bcov[0] + bcon[1] * bcov[1] + bcon[2] * bcov[2] + bcon[3] * bcov[3];
+      const double rho = pbz[HARM_RHO], u = pbz[HARM_UU];
+      const double Pg = (P.gam - 1.0) * u;
+      const double w = rho + u + Pg;
+      const double eta = w + bsq;
+      const double rhoprime = fmax(fabs(eta), fabs(UggUU));
+      double dtsource[4];
+#pragma unroll
+      for (int jj = 0; jj < 4; jj++) {
+        const double ag = D_SMALL + fabs(mydU[jj] / rhoprime);
+        dtsource[jj] = P.cour * (mydx[jj] / ag);
+      }
+      if (dtsource[0] < dts) dts = dtsource[0];
+      if ((DM & 1) && dtsource[1] < dts) dts = dtsource[1];
+      if ((DM & 2) && dtsource[2] < dts) dts = dtsource[2];
+      if ((DM & 4) && dtsource[3] < dts) dts = dtsource[3];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    double pz[HARM_NPR], Ui[HARM_NPR];
+#pragma unroll
+    for (int pl = 0; pl < HARM_NPR; pl++) pz[pl] = pi[pl * nc + o];
+    {
+      State q;
+      hd::get_state(pz, G, P.gam, q);
+      hd::primtoflux(pz, q, 0, G, Ui);
+    }
+#pragma unroll
+    for (int pl = 0; pl < HARM_NPR; pl++) {
+      ui[pl * nc + o] = Ui[pl];
       const double ufo = (timeorder == 0) ? 0.0 : uf[pl * nc + o];
       const double uco = (timeorder == 0) ? 0.0 : ucum[pl * nc + o];
-      const double ufn = (cuf0 * Ui[pl] + cuf1 * ufo + cuf2 * dt * (dUr[pl] + dUg[pl]));
+      const double ufn = (cuf0 * Ui[pl] + cuf1 * ufo + cuf2 * dt * dUt[pl]);
       uf[pl * nc + o] = ufn;
-      ucum[pl * nc + o] = uco + (cun0 * Ui[pl] + cun1 * dt * (dUr[pl] + dUg[pl]) + cun2 * ufn);
-      dUt[pl] = dUr[pl] + dUg[pl];
+      ucum[pl * nc + o] = uco + (cun0 * Ui[pl] + cun1 * dt * dUt[pl] + cun2 * ufn);
     }
-    // dUtodt -> compute_dt_fromsource
-    double U[HARM_NPR], Ugf[HARM_NPR];
-    hd::primtoflux(pbz, qb, 0, G, U);
-    const double ig = hd::igdetnosing(G.gdet);
-#pragma unroll
-    for (int pl = 0; pl < HARM_NPR; pl++) Ugf[pl] = U[pl] * ig;
-    double dUd[4], dUu[4], mydU[4], mydx[4];
-#pragma unroll
-    for (int jj = 0; jj < 4; jj++) dUd[jj] = (dUt[HARM_UU + jj] * dt) * ig;
-    hd::raise(dUd, G, dUu);
-    mydU[0] = dUd[0];
-    mydx[0] = dt;
-#pragma unroll
-    for (int jj = 1; jj < 4; jj++) { mydU[jj] = dUu[jj]; mydx[jj] = P.dx[jj]; }
-    double bcon[4], bcov[4];
-    bcon[0] = pbz[HARM_B1] * qb.ucov[1] + pbz[HARM_B2] * qb.ucov[2] + pbz[HARM_B3] * qb.ucov[3];
-#pragma unroll
-    for (int jj = 1; jj < 4; jj++) bcon[jj] = (pbz[HARM_B1 - 1 + jj] + bcon[0] * qb.ucon[jj]) / qb.ucon[0];
-    hd::lower(bcon, G, bcov);
-    const double bsq = bcon[0] * bcov[0] + bcon[1] * bcov[1] + bcon[2] * bcov[2] + bcon[3] * bcov[3];
-    const double rho = pbz[HARM_RHO], u = pbz[HARM_UU];
-    const double Pg = (P.gam - 1.0) * u;
-    const double w = rho + u + Pg;
-    const double eta = w + bsq;
-    double dtsource[4];
-#pragma unroll
-    for (int jj = 0; jj < 4; jj++) {
-      const double rhoprime = fmax(fabs(eta), fabs(Ugf[HARM_UU]));
-      const double ag = D_SMALL + fabs(mydU[jj] / rhoprime);
-      dtsource[jj] = P.cour * (mydx[jj] / ag);
-    }
-    dts = kBIG;
-    if (dtsource[0] < dts) dts = dtsource[0];
-    if (d.n1 > 1 && dtsource[1] < dts) dts = dtsource[1];
-    if (d.n2 > 1 && dtsource[2] < dts) dts = dtsource[2];
-    if (d.n3 > 1 && dtsource[3] < dts) dts = dtsource[3];
   }
   block_min_to(dts, ds + DS_ACCDT);
 }
@@ -446,15 +464,17 @@ __global__ __launch_bounds__(kBlock) void k_update(Dims d, Params P, const doubl
 // inversion + fixup1zone (floors via ZAMO-frame mass/energy addition)
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_invert(Dims d, Params P, const double* __restrict__ U,
-                                                   double* __restrict__ pf, int32_t* __restrict__ pflag,
+                                                   const double* __restrict__ pg, double* __restrict__ pf,
+                                                   int32_t* __restrict__ pflag,
                                                    const double* __restrict__ gcent, const double* __restrict__ floors,
                                                    int finalstep, unsigned long long* __restrict__ di)
 {
-  const long t = (long)blockIdx.x * kBlock + threadIdx.x;
-  const long nact = (long)d.n1 * d.n2 * d.n3;
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  const int nact = d.n1 * d.n2 * d.n3;
   int ns = 0, nf = 0;
   if (t < nact) {
-    const int i = (int)(t % d.n1), j = (int)((t / d.n1) % d.n2), k = (int)(t / ((long)d.n1 * d.n2));
+    const int r = t / d.n1, i = t - r * d.n1;
+    const int k = r / d.n2, j = r - k * d.n2;
     const long o = hd::cidx(d, i, j, k);
     const long pp = hd::pidx(d, i, j);
     const long nc = d.ncell;
@@ -464,7 +484,7 @@ __global__ __launch_bounds__(kBlock) void k_invert(Dims d, Params P, const doubl
 #pragma unroll
     for (int pl = 0; pl < HARM_NPR; pl++) {
       Uz[pl] = U[pl * nc + o];
-      pz[pl] = pf[pl * nc + o];
+      pz[pl] = pg[pl * nc + o];  // guess: the substep's starting primitives
     }
     int fl = hd::utoprimgen(Uz, G, P.gam, P.gammafail, pz, ns, __builtin_nan(""));
     if (finalstep) {
@@ -1087,25 +1107,43 @@ harm_status enq_update(harm_handle* h, int which_pi, int which_pb, const double*
 {
   const Dims& d = h->d;
   prof_begin(h);
-  hipLaunchKernelGGL(k_update, dim3(nblocks((long)d.n1 * d.n2 * d.n3)), dim3(kBlock), 0, h->stream, d, h->P,
-                     h->arr[which_pi], h->arr[which_pb], h->arr[HARM_ARR_F1], h->arr[HARM_ARR_F2],
-                     h->arr[HARM_ARR_F3], h->arr[HARM_ARR_UI], h->arr[HARM_ARR_UF], h->arr[HARM_ARR_UCUM],
-                     h->geom[HARM_LOC_CENT], h->conn, CUf[0], CUf[1], CUf[2], Cunew[0], Cunew[1], Cunew[2], timeorder,
-                     h->ds);
+  const int dm = (d.n1 > 1) | ((d.n2 > 1) << 1) | ((d.n3 > 1) << 2);
+#define LAUNCH_UPDATE(DM)                                                                                           \
+  hipLaunchKernelGGL(k_update<DM>, dim3(nblocks((long)d.n1 * d.n2 * d.n3)), dim3(kBlock), 0, h->stream, d, h->P,    \
+                     h->arr[which_pi], h->arr[which_pb], h->arr[HARM_ARR_F1], h->arr[HARM_ARR_F2],                  \
+                     h->arr[HARM_ARR_F3], h->arr[HARM_ARR_UI], h->arr[HARM_ARR_UF], h->arr[HARM_ARR_UCUM],          \
+                     h->geom[HARM_LOC_CENT], h->conn, CUf[0], CUf[1], CUf[2], Cunew[0], Cunew[1], Cunew[2],         \
+                     timeorder, h->ds)
+  switch (dm) {
+    case 1: LAUNCH_UPDATE(1); break;
+    case 2: LAUNCH_UPDATE(2); break;
+    case 3: LAUNCH_UPDATE(3); break;
+    case 4: LAUNCH_UPDATE(4); break;
+    case 5: LAUNCH_UPDATE(5); break;
+    case 6: LAUNCH_UPDATE(6); break;
+    case 7: LAUNCH_UPDATE(7); break;
+    default: LAUNCH_UPDATE(0); break;
+  }
+#undef LAUNCH_UPDATE
   prof_end(h, HARM_K_UPDATE);
   return hh::check_launch("k_update");
 }
 
-harm_status enq_invert(harm_handle* h, int which_pb, int which_pf, int finalstep)
+// copy_all: pf = pb over the whole storage box first, as advance() does
+// (advance.c:654).  Inside a step the copy is skipped: the inversion reads its
+// guess from pb and writes every active zone of pf, and the boundary pass that
+// follows rewrites every ghost zone of pf before anything reads it.
+harm_status enq_invert(harm_handle* h, int which_pb, int which_pf, int finalstep, bool copy_all)
 {
   const Dims& d = h->d;
   const size_t bytes = sizeof(double) * HARM_NPR * d.ncell;
-  if (which_pf != which_pb)
+  if (copy_all && which_pf != which_pb)
     HH_HIP(hipMemcpyAsync(h->arr[which_pf], h->arr[which_pb], bytes, hipMemcpyDeviceToDevice, h->stream));
   const double* U = h->arr[finalstep ? HARM_ARR_UCUM : HARM_ARR_UF];
   prof_begin(h);
   hipLaunchKernelGGL(k_invert, dim3(nblocks((long)d.n1 * d.n2 * d.n3)), dim3(kBlock), 0, h->stream, d, h->P, U,
-                     h->arr[which_pf], h->pflag, h->geom[HARM_LOC_CENT], h->floors, finalstep, h->di);
+                     h->arr[which_pb], h->arr[which_pf], h->pflag, h->geom[HARM_LOC_CENT], h->floors, finalstep,
+                     h->di);
   prof_end(h, HARM_K_INVERT);
   return hh::check_launch("k_invert");
 }
@@ -1174,7 +1212,7 @@ harm_status enq_substep(harm_handle* h, int to, int nto, const double CUf[4][4],
     hipLaunchKernelGGL(k_fluxacc, dim3(1), dim3(kBlock), 0, h->stream, h->d, h->P, h->arr[HARM_ARR_F1], h->ds,
                        wflux[to]);
   HH_TRY(enq_update(h, pii[to], pbb[to], CUf[to], Cunew[to], to));
-  HH_TRY(enq_invert(h, pbb[to], pff[to], finalstep));
+  HH_TRY(enq_invert(h, pbb[to], pff[to], finalstep, false));
   hipLaunchKernelGGL(k_scal_substep_dt, dim3(1), dim3(1), 0, h->stream, h->ds);
   HH_TRY(enq_bound(h, pff[to]));
   HH_TRY(enq_post_fixup(h, pff[to]));
@@ -1437,7 +1475,7 @@ extern "C" harm_status harm_invert_grid(harm_handle* h, int32_t which_pb, int32_
   HH_TRY(check_which(which_pf));
   unsigned long long z[DI_N] = {0};
   HH_HIP(hipMemcpyAsync(h->di, z, sizeof(z), hipMemcpyHostToDevice, h->stream));
-  HH_TRY(enq_invert(h, which_pb, which_pf, finalstep));
+  HH_TRY(enq_invert(h, which_pb, which_pf, finalstep, true));
   if (stats) {
     HH_HIP(hipStreamSynchronize(h->stream));
     HH_HIP(hipMemcpy(z, h->di, sizeof(z), hipMemcpyDeviceToHost));
