@@ -93,6 +93,9 @@ def main():
     ap.add_argument("--grid", default="x".join(map(str, GRID)))
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="before the run, one device-to-device copy of this many MiB (the reference dispatch "
+                         "for scaling HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs the whole domain on one GPU; --gpus {args.gpus} is not supported "
@@ -101,6 +104,14 @@ def main():
 
     import harm_harm_amd as H
     from harm_harm_amd.grid import Grid
+
+    if args.calibrate_copy_mib > 0:
+        nb = args.calibrate_copy_mib << 20
+        src, dst = H.DeviceArray(nb, "u1"), H.DeviceArray(nb, "u1")
+        dst.copy_from_device(src)
+        H.synchronize()
+        src.free()
+        dst.free()
 
     n1, n2, n3 = (int(x) for x in args.grid.split("x"))
     cfg = H.default_config(n1, n2, n3, recon="weno5", flux="hll", timeorder=3)

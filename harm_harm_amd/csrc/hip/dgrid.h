@@ -9,16 +9,19 @@
 
 namespace hd {
 
+// Active sizes n, ghost widths ng, storage box ns = n + 2 ng, and the device
+// row pitch ps1 >= ns1 (a multiple of 16 doubles, so every storage row
+// starts on a 128-byte line).  ncell and nplane count pitched positions.
 struct Dims {
-  int n1, n2, n3, ng1, ng2, ng3, ns1, ns2, ns3;
+  int n1, n2, n3, ng1, ng2, ng3, ns1, ns2, ns3, ps1;
   long ncell, nplane;
 };
 
 __device__ __forceinline__ long cidx(const Dims& d, int i, int j, int k)
 {
-  return ((long)(k + d.ng3) * d.ns2 + (j + d.ng2)) * d.ns1 + (i + d.ng1);
+  return ((long)(k + d.ng3) * d.ns2 + (j + d.ng2)) * d.ps1 + (i + d.ng1);
 }
-__device__ __forceinline__ long pidx(const Dims& d, int i, int j) { return (long)(j + d.ng2) * d.ns1 + (i + d.ng1); }
+__device__ __forceinline__ long pidx(const Dims& d, int i, int j) { return (long)(j + d.ng2) * d.ps1 + (i + d.ng1); }
 
 __device__ __forceinline__ double minmod(double a, double b)
 {

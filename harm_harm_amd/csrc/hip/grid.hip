@@ -99,19 +99,24 @@ Box face_box(const Dims& d, int dir)
 // Face tiles.  A block first reconstructs every cell its faces touch, one
 // (cell, variable) pair per lane, into LDS (each cell once, both edges), then
 // solves one face per lane.  Direction 1 runs along i, the contiguous
-// dimension: a block takes `fpb` consecutive faces of the flattened face box
-// (rows of ni faces own ni+1 cells, so a block needs fpb + rows + 1 cells).
-// Directions 2 and 3 take a tile of tc columns (consecutive i, then the other
-// transverse index) by td faces along the sweep, i.e. (td+1)*tc cells.
+// dimension: a block takes `fpb` consecutive faces of the face box flattened
+// as (i, k, j) (rows of ni faces own ni+1 cells, so a block needs
+// fpb + rows + 1 cells).  Directions 2 and 3 take tc consecutive i by td
+// faces along the sweep in one transverse plane (k for direction 2, j for
+// direction 3), i.e. (td+1)*tc cells.  Tiles are ordered so that neighbours
+// in the launch reuse the same (i, j) rows of the 2-D face geometry: phi
+// fastest for directions 1 and 2, the sweep fastest for direction 3.
 constexpr int kCellMax = 320;
 
 struct FaceTile {
   Box b;
-  int ncol;      // dir 1: rows (nj*nk); dirs 2,3: columns (ni*nk or ni*nj)
-  int nd;        // faces along the sweep
-  int fpb;       // dir 1: faces per block
-  int tc, td;    // dirs 2,3: tile shape
-  int ntc, ntd;  // dirs 2,3: tiles along columns / along the sweep
+  int ncol;       // dir 1: rows (nj*nk)
+  int nd;         // faces along the sweep
+  int fpb;        // dir 1: faces per block
+  int tc, td;     // dirs 2,3: tile shape
+  int nti, ntd;   // dirs 2,3: tiles along i / along the sweep
+  int nplane;     // dirs 2,3: transverse planes (nk for dir 2, nj for dir 3)
+  int i0;         // dirs 2,3: i of the first tile's first column (-ng1: tiles start on 128-byte lines)
   long nblk;
 };
 
@@ -121,6 +126,19 @@ __device__ __forceinline__ int xcd_tile(int b, int n)
 {
   const int x = b & 7, loc = b >> 3, q = n >> 3, r = n & 7;
   return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + loc;
+}
+
+// Active-zone order for the per-zone kernels: i fastest (coalesced), then k,
+// then j.  Consecutive rows are consecutive phi planes of the same (i, j)
+// row, so the 2-D geometry and connection rows stay in L2 across phi; blocks
+// are remapped so each XCD works on a contiguous range of rows.
+__device__ __forceinline__ int zone_thread(void) { return xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x; }
+__device__ __forceinline__ void zone_ijk(const Dims& d, int t, int& i, int& j, int& k)
+{
+  const int r = t / d.n1;
+  i = t - r * d.n1;
+  j = r / d.n3;
+  k = r - j * d.n3;
 }
 
 // Riemann flux of one face from its two reconstructed states
@@ -180,12 +198,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) 
 {
   __shared__ double s_l[HARM_NPR * kCellMax], s_r[HARM_NPR * kCellMax];
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
-  const int st = (DIR == 1) ? 1 : (DIR == 2) ? d.ns1 : d.ns1 * d.ns2;
+  const int st = (DIR == 1) ? 1 : (DIR == 2) ? d.ps1 : d.ps1 * d.ns2;
   const Box& b = T.b;
 
   // ---- phase 1: reconstruct the tile's cells ----
   // (32-bit index arithmetic: a face box stays far below 2^31 faces)
-  int f0 = 0, g_lo = 0, ncell, bc = 0, bd = 0;
+  int f0 = 0, g_lo = 0, ncell, bc = 0, bd = 0, pl = 0;
   const int rowc = T.nd + 1;  // dir 1: cells per row
   if (DIR == 1) {
     const int nf = T.nd * T.ncol;
@@ -194,8 +212,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) 
     g_lo = (f0 / T.nd) * rowc + f0 % T.nd;
     ncell = (fl / T.nd) * rowc + fl % T.nd + 1 - g_lo + 1;
   } else {
-    bd = tile % T.ntd;
-    bc = tile / T.ntd;
+    // bc: i tile, bd: sweep tile, pl: transverse plane
+    if (DIR == 2) {
+      pl = tile % T.nplane;
+      const int r = tile / T.nplane;
+      bd = r % T.ntd;
+      bc = r / T.ntd;
+    } else {
+      bd = tile % T.ntd;
+      const int r = tile / T.ntd;
+      pl = r % T.nplane;
+      bc = r / T.nplane;
+    }
     ncell = (T.td + 1) * T.tc;
   }
   for (int it = threadIdx.x; it < ncell * HARM_NPR; it += kBlock) {
@@ -205,18 +233,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) 
       const int g = g_lo + slot;
       const int row = g / rowc;
       i = b.is - 1 + (g - row * rowc);
-      const int rk = row / b.nj;
-      j = b.js + (row - rk * b.nj);
-      k = b.ks + rk;
+      const int rj = row / b.nk;
+      k = b.ks + (row - rj * b.nk);
+      j = b.js + rj;
     } else {
       const int dd = slot / T.tc, cc = slot - dd * T.tc;
-      const int c = bc * T.tc + cc;
+      i = T.i0 + bc * T.tc + cc;
       const int pd = bd * T.td - 1 + dd;  // sweep index of the cell relative to the first face
-      if (c >= T.ncol || pd > T.nd - 1) continue;
-      const int co = c / b.ni;
-      i = b.is + (c - co * b.ni);
-      if (DIR == 2) { k = b.ks + co; j = b.js + pd; }
-      else { j = b.js + co; k = b.ks + pd; }
+      if (i < b.is || i >= b.is + b.ni || pd > T.nd - 1) continue;
+      if (DIR == 2) { k = b.ks + pl; j = b.js + pd; }
+      else { j = b.js + pl; k = b.ks + pd; }
     }
     const double* a = pb + (long)v * d.ncell + hd::cidx(d, i, j, k);
     double y[5];
@@ -241,22 +267,21 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) 
       const int fi = f - row * T.nd;
       sr = row * rowc + fi + 1 - g_lo;
       sl = sr - 1;
-      const int rk = row / b.nj;
+      const int rj = row / b.nk;
       i = b.is + fi;
-      j = b.js + (row - rk * b.nj);
-      k = b.ks + rk;
+      k = b.ks + (row - rj * b.nk);
+      j = b.js + rj;
     }
   } else {
     const int dd = (int)threadIdx.x / T.tc, cc = (int)threadIdx.x - dd * T.tc;
-    const int c = bc * T.tc + cc, fd = bd * T.td + dd;
-    ok = dd < T.td && c < T.ncol && fd < T.nd;
+    const int fd = bd * T.td + dd;
+    i = T.i0 + bc * T.tc + cc;
+    ok = dd < T.td && i >= b.is && i < b.is + b.ni && fd < T.nd;
     if (ok) {
       sl = dd * T.tc + cc;
       sr = sl + T.tc;
-      const int co = c / b.ni;
-      i = b.is + (c - co * b.ni);
-      if (DIR == 2) { k = b.ks + co; j = b.js + fd; }
-      else { j = b.js + co; k = b.ks + fd; }
+      if (DIR == 2) { k = b.ks + pl; j = b.js + fd; }
+      else { j = b.js + pl; k = b.ks + fd; }
     }
   }
   if (ok) {
@@ -362,12 +387,12 @@ __global__ __launch_bounds__(kBlock) void k_update(Dims d, Params P, const doubl
   // Work is ordered so that few values are live at once: flux divergence,
   // then everything that needs the state of pb (source, source dt), then the
   // state of pi (Ui) and the RK combination.
-  const int t = blockIdx.x * kBlock + threadIdx.x;
+  const int t = zone_thread();
   const int nact = d.n1 * d.n2 * d.n3;
   double dts = kBIG;
   if (t < nact) {
-    const int r = t / d.n1, i = t - r * d.n1;
-    const int k = r / d.n2, j = r - k * d.n2;
+    int i, j, k;
+    zone_ijk(d, t, i, j, k);
     const long o = hd::cidx(d, i, j, k);
     const long pp = hd::pidx(d, i, j);
     const long nc = d.ncell;
@@ -376,7 +401,7 @@ __global__ __launch_bounds__(kBlock) void k_update(Dims d, Params P, const doubl
     hd::load_geom(gcent, pp, d.nplane, G);
     // flux divergence (flux2dUavg)
     const double idx1 = 1.0 / P.dx[1], idx2 = 1.0 / P.dx[2], idx3 = 1.0 / P.dx[3];
-    const long s2 = d.ns1, s3 = (long)d.ns1 * d.ns2;
+    const long s2 = d.ps1, s3 = (long)d.ps1 * d.ns2;
     double dUt[HARM_NPR];
 #pragma unroll
     for (int pl = 0; pl < HARM_NPR; pl++) {
@@ -469,12 +494,12 @@ __global__ __launch_bounds__(kBlock) void k_invert(Dims d, Params P, const doubl
                                                    const double* __restrict__ gcent, const double* __restrict__ floors,
                                                    int finalstep, unsigned long long* __restrict__ di)
 {
-  const int t = blockIdx.x * kBlock + threadIdx.x;
+  const int t = zone_thread();
   const int nact = d.n1 * d.n2 * d.n3;
   int ns = 0, nf = 0;
   if (t < nact) {
-    const int r = t / d.n1, i = t - r * d.n1;
-    const int k = r / d.n2, j = r - k * d.n2;
+    int i, j, k;
+    zone_ijk(d, t, i, j, k);
     const long o = hd::cidx(d, i, j, k);
     const long pp = hd::pidx(d, i, j);
     const long nc = d.ncell;
@@ -897,7 +922,8 @@ void make_dims(const harm_config* cfg, Dims& d, Params& P)
   d.ng2 = d.n2 > 1 ? cfg->ng : 0;
   d.ng3 = d.n3 > 1 ? cfg->ng : 0;
   d.ns1 = d.n1 + 2 * d.ng1; d.ns2 = d.n2 + 2 * d.ng2; d.ns3 = d.n3 + 2 * d.ng3;
-  d.nplane = (long)d.ns1 * d.ns2;
+  d.ps1 = (d.ns1 + 15) & ~15;
+  d.nplane = (long)d.ps1 * d.ns2;
   d.ncell = d.nplane * d.ns3;
   P.gam = cfg->gam; P.cour = cfg->cour; P.safe = cfg->safe; P.gammamax = cfg->gammamax; P.gammafail = cfg->gammafail;
   P.startx[0] = 0.0; P.dx[0] = 0.0;
@@ -911,7 +937,7 @@ void make_dims(const harm_config* cfg, Dims& d, Params& P)
 void host_tables(const harm_config& c, const Dims& d, const Params& P, std::vector<double>* geom_aos,
                  std::vector<double>& conn_aos, std::vector<double>& gc3, std::vector<double>& rcent)
 {
-  const long np = d.nplane;
+  const long np = (long)d.ns1 * d.ns2;  // host tables cover the logical storage plane
   for (int loc = 0; loc < HARM_NLOC; loc++) {
     geom_aos[loc].assign((size_t)np * HARM_G_NFIELD, 0.0);
 #pragma omp parallel for collapse(2)
@@ -953,28 +979,35 @@ harm_status build_geometry(harm_handle* h)
 {
   const Dims& d = h->d;
   const harm_config& c = h->cfg;
-  const long np = d.nplane;
+  const long np = d.nplane;  // pitched device plane
   host_tables(c, d, h->P, h->hgeom, h->hconn, h->hgc3, h->hrcent);
-  std::vector<double> soa((size_t)std::max((int)HARM_G_NFIELD, (int)HARM_NCONN) * np);
+  // device position of logical storage point (ii, jj), and its host index
+  auto dev = [&](int ii, int jj) { return (size_t)jj * d.ps1 + ii; };
+  auto hst = [&](int ii, int jj) { return (size_t)jj * d.ns1 + ii; };
+  std::vector<double> soa((size_t)std::max((int)HARM_G_NFIELD, (int)HARM_NCONN) * np, 0.0);
   for (int loc = 0; loc < HARM_NLOC; loc++) {
-    for (long pp = 0; pp < np; pp++)
-      for (int f = 0; f < HARM_G_NFIELD; f++) soa[(size_t)f * np + pp] = h->hgeom[loc][pp * HARM_G_NFIELD + f];
+    for (int jj = 0; jj < d.ns2; jj++)
+      for (int ii = 0; ii < d.ns1; ii++)
+        for (int f = 0; f < HARM_G_NFIELD; f++)
+          soa[(size_t)f * np + dev(ii, jj)] = h->hgeom[loc][hst(ii, jj) * HARM_G_NFIELD + f];
     HH_TRY(alloc((void**)&h->geom[loc], sizeof(double) * HARM_G_NFIELD * np));
     HH_TRY(upload(h, h->geom[loc], soa.data(), (size_t)HARM_G_NFIELD * np));
   }
-  for (long pp = 0; pp < np; pp++)
-    for (int f = 0; f < HARM_NCONN; f++) soa[(size_t)f * np + pp] = h->hconn[pp * HARM_NCONN + f];
+  for (int jj = 0; jj < d.ns2; jj++)
+    for (int ii = 0; ii < d.ns1; ii++)
+      for (int f = 0; f < HARM_NCONN; f++) soa[(size_t)f * np + dev(ii, jj)] = h->hconn[hst(ii, jj) * HARM_NCONN + f];
   HH_TRY(alloc((void**)&h->conn, sizeof(double) * HARM_NCONN * np));
   HH_TRY(upload(h, h->conn, soa.data(), (size_t)HARM_NCONN * np));
-  HH_TRY(alloc((void**)&h->gcorn3, sizeof(double) * np));
-  HH_TRY(upload(h, h->gcorn3, h->hgc3.data(), np));
-  std::vector<double> fl(2 * np);
+  std::vector<double> gc(np, 0.0), fl(2 * np, 0.0);
   for (int jj = 0; jj < d.ns2; jj++)
     for (int ii = 0; ii < d.ns1; ii++) {
+      gc[dev(ii, jj)] = h->hgc3[hst(ii, jj)];
       const double r = h->hrcent[ii];
-      fl[(size_t)jj * d.ns1 + ii] = c.rhomin * pow(r, -1.5);
-      fl[np + (size_t)jj * d.ns1 + ii] = c.uumin * pow(r, -2.5);
+      fl[dev(ii, jj)] = c.rhomin * pow(r, -1.5);
+      fl[np + dev(ii, jj)] = c.uumin * pow(r, -2.5);
     }
+  HH_TRY(alloc((void**)&h->gcorn3, sizeof(double) * np));
+  HH_TRY(upload(h, h->gcorn3, gc.data(), np));
   HH_TRY(alloc((void**)&h->floors, sizeof(double) * 2 * np));
   HH_TRY(upload(h, h->floors, fl.data(), fl.size()));
   HH_TRY(alloc((void**)&h->rcent, sizeof(double) * d.ns1));
@@ -1021,7 +1054,8 @@ harm_status check_which(int which)
 // Tile shape of the face kernels.  Direction 1: the largest face count per
 // block whose cells fit the LDS tile.  Directions 2 and 3: td in {4, 8, 16}
 // (tc = 256/td) minimising blocks x (recon passes x 1 + face pass x 12), the
-// rough cost ratio of one (cell, variable) reconstruction to one face solve.
+// rough cost ratio of one (cell, variable) reconstruction to one face solve;
+// the i extent is covered by ceil(ni/tc) tiles.
 FaceTile plan_face_tile(const Dims& d, int dir)
 {
   FaceTile T{};
@@ -1037,23 +1071,25 @@ FaceTile plan_face_tile(const Dims& d, int dir)
     return T;
   }
   T.nd = dir == 2 ? b.nj : b.nk;
-  T.ncol = dir == 2 ? b.ni * b.nk : b.ni * b.nj;
+  T.nplane = dir == 2 ? b.nk : b.nj;
+  T.i0 = -d.ng1;
+  const int iext = b.is + b.ni - T.i0;  // i span from the aligned origin to the last face
   double best = 1e300;
   for (int td : {4, 8, 16}) {
     const int tc = kBlock / td;
     if ((td + 1) * tc > kCellMax) continue;
-    const long ntc = (T.ncol + tc - 1) / tc, ntd = (T.nd + td - 1) / td;
+    const long nti = (iext + tc - 1) / tc, ntd = (T.nd + td - 1) / td;
     const int passes = ((td + 1) * tc * HARM_NPR + kBlock - 1) / kBlock;
-    const double cost = (double)(ntc * ntd) * (passes + 12.0);
+    const double cost = (double)(nti * ntd * T.nplane) * (passes + 12.0);
     if (cost < best) {
       best = cost;
       T.td = td;
       T.tc = tc;
-      T.ntc = (int)ntc;
+      T.nti = (int)nti;
       T.ntd = (int)ntd;
     }
   }
-  T.nblk = (long)T.ntc * T.ntd;
+  T.nblk = (long)T.nti * T.ntd * T.nplane;
   return T;
 }
 
@@ -1360,14 +1396,24 @@ extern "C" harm_status harm_get_coords(harm_handle* h, int32_t i, int32_t j, int
   return HARM_OK;
 }
 
+// device position of logical storage cell c = (kk*ns2 + jj)*ns1 + ii
+static inline size_t dev_cell(const Dims& d, long c)
+{
+  const long row = c / d.ns1;
+  return (size_t)row * d.ps1 + (size_t)(c - row * d.ns1);
+}
+
 extern "C" harm_status harm_set_array(harm_handle* h, int32_t which, const double* host_aos)
 {
   HH_TRY(check_handle(h));
   HH_TRY(check_which(which));
-  const long nc = h->d.ncell;
-  std::vector<double> soa((size_t)HARM_NPR * nc);
-  for (long c = 0; c < nc; c++)
-    for (int pl = 0; pl < HARM_NPR; pl++) soa[(size_t)pl * nc + c] = host_aos[c * HARM_NPR + pl];
+  const Dims& d = h->d;
+  const long nl = (long)d.ns1 * d.ns2 * d.ns3, nc = d.ncell;
+  std::vector<double> soa((size_t)HARM_NPR * nc, 0.0);
+  for (long c = 0; c < nl; c++) {
+    const size_t o = dev_cell(d, c);
+    for (int pl = 0; pl < HARM_NPR; pl++) soa[(size_t)pl * nc + o] = host_aos[c * HARM_NPR + pl];
+  }
   HH_HIP(hipStreamSynchronize(h->stream));
   return upload(h, h->arr[which], soa.data(), soa.size());
 }
@@ -1376,21 +1422,28 @@ extern "C" harm_status harm_get_array(harm_handle* h, int32_t which, double* hos
 {
   HH_TRY(check_handle(h));
   HH_TRY(check_which(which));
-  const long nc = h->d.ncell;
+  const Dims& d = h->d;
+  const long nl = (long)d.ns1 * d.ns2 * d.ns3, nc = d.ncell;
   std::vector<double> soa((size_t)HARM_NPR * nc);
   HH_HIP(hipStreamSynchronize(h->stream));
   HH_HIP(hipMemcpy(soa.data(), h->arr[which], sizeof(double) * soa.size(), hipMemcpyDeviceToHost));
-  for (long c = 0; c < nc; c++)
-    for (int pl = 0; pl < HARM_NPR; pl++) host_aos[c * HARM_NPR + pl] = soa[(size_t)pl * nc + c];
+  for (long c = 0; c < nl; c++) {
+    const size_t o = dev_cell(d, c);
+    for (int pl = 0; pl < HARM_NPR; pl++) host_aos[c * HARM_NPR + pl] = soa[(size_t)pl * nc + o];
+  }
   return HARM_OK;
 }
 
 extern "C" harm_status harm_get_pflag(harm_handle* h, int32_t* host_flags)
 {
   HH_TRY(check_handle(h));
+  const Dims& d = h->d;
+  std::vector<int32_t> f(d.ncell);
   HH_HIP(hipStreamSynchronize(h->stream));
-  return hh::check_hip(hipMemcpy(host_flags, h->pflag, sizeof(int32_t) * h->d.ncell, hipMemcpyDeviceToHost),
-                       "get_pflag");
+  HH_HIP(hipMemcpy(f.data(), h->pflag, sizeof(int32_t) * d.ncell, hipMemcpyDeviceToHost));
+  const long nl = (long)d.ns1 * d.ns2 * d.ns3;
+  for (long c = 0; c < nl; c++) host_flags[c] = f[dev_cell(d, c)];
+  return HARM_OK;
 }
 
 extern "C" harm_status harm_set_time(harm_handle* h, double t, double dt, int64_t nstep)
@@ -1419,7 +1472,7 @@ extern "C" harm_status harm_init_torus(harm_handle* h, double rin, double rmax, 
 {
   HH_TRY(check_handle(h));
   const Dims& d = h->d;
-  std::vector<double> p((size_t)HARM_NPR * d.ncell, 0.0);
+  std::vector<double> p((size_t)HARM_NPR * d.ns1 * d.ns2 * d.ns3, 0.0);
   hi_grid G;
   G.ng1 = d.ng1; G.ng2 = d.ng2; G.ng3 = d.ng3;
   for (int a = 0; a < 4; a++) { G.startx[a] = h->P.startx[a]; G.dx[a] = h->P.dx[a]; }
@@ -1552,7 +1605,7 @@ extern "C" harm_status harm_host_tables(const harm_config* cfg, double* geom_aos
   host_tables(*cfg, d, P, g, c, gc3, rc);
   if (geom_aos)
     for (int loc = 0; loc < HARM_NLOC; loc++)
-      memcpy(geom_aos + (size_t)loc * d.nplane * HARM_G_NFIELD, g[loc].data(), sizeof(double) * g[loc].size());
+      memcpy(geom_aos + (size_t)loc * d.ns1 * d.ns2 * HARM_G_NFIELD, g[loc].data(), sizeof(double) * g[loc].size());
   if (conn_aos) memcpy(conn_aos, c.data(), sizeof(double) * c.size());
   if (corner_gdet) memcpy(corner_gdet, gc3.data(), sizeof(double) * gc3.size());
   if (rcent) memcpy(rcent, rc.data(), sizeof(double) * rc.size());
@@ -1573,7 +1626,7 @@ extern "C" harm_status harm_host_init_torus(const harm_config* cfg, double rin, 
   hi_grid G;
   G.ng1 = d.ng1; G.ng2 = d.ng2; G.ng3 = d.ng3;
   for (int a = 0; a < 4; a++) { G.startx[a] = P.startx[a]; G.dx[a] = P.dx[a]; }
-  memset(p_aos, 0, sizeof(double) * HARM_NPR * d.ncell);
+  memset(p_aos, 0, sizeof(double) * HARM_NPR * d.ns1 * d.ns2 * d.ns3);
   if (hi_init_torus(cfg, &G, rin, rmax, kappa, beta, randfact, seed, p_aos) != 0)
     return hh::set_error(HARM_EINVAL, "harm_host_init_torus: no torus zones on this grid");
   return HARM_OK;
