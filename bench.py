@@ -47,8 +47,10 @@ def algorithmic_bytes(g, kernel):
         return faces * (64 + 64) + ni * nj * geom
     zones = n1 * n2 * n3
     if kernel == "update":
-        # pi, pb, 3 face fluxes (8 each), ui write, uf r/w, ucum r/w
-        return zones * (64 + 64 + 3 * 64 + 64 + 128 + 128) + n1 * n2 * (geom + 40 * 8)
+        # inside a step every substep moves 448 B per zone: pb 64 + 3 face
+        # fluxes 192 + (pi read, ui write | ui read, uf read) 128 + one of
+        # uf write / ucum write 64; plus geometry and connection per column
+        return zones * 448 + n1 * n2 * (geom + 40 * 8)
     if kernel == "invert":
         # U read, guess read, solution write, flag write
         return zones * (64 + 64 + 64 + 4) + n1 * n2 * geom

@@ -371,10 +371,25 @@ __global__ __launch_bounds__(kBlock) void k_ct(Dims d, int ni, int nj, int nk, c
 // ---------------------------------------------------------------------------
 // update: Ui = U(pi); dU = -div F + source(pb); Uf, ucum; source dt limit
 // ---------------------------------------------------------------------------
-// DM: bit m-1 set when dimension m has more than one zone; a compile-time
+// DM: bit m-1 set when dimension m has more than one zone (a compile-time
 // mask keeps the flux-divergence loads free of branches so they issue
-// together.
-template <int DM>
+// together); DM < 0 tests the dimensions at run time.
+// MODE selects which of the RK arrays are read and written:
+//   kUpdAll    the reference's loop as is: Ui = U(pi) stored in ui; uf and
+//              ucum read (unless timeorder == 0) and written;
+//   kUpdFirst  first substep of a step: Ui = U(pi) stored in ui, uf written,
+//              ucum left alone (its coefficients are zero, so the reference
+//              keeps it at 0 there);
+//   kUpdMiddle later substep before the last: Ui read back from ui (pi is the
+//              same state as on the first substep, so the value is identical),
+//              uf read and written, ucum left alone;
+//   kUpdLast   last substep: Ui from ui, uf read, ucum = 0 + (combination)
+//              written without reading the zero it holds in the reference; uf
+//              is not written (nothing reads it before the next step rebuilds
+//              it).
+enum { kUpdAll = 0, kUpdFirst = 1, kUpdMiddle = 2, kUpdLast = 3 };
+
+template <int DM, int MODE>
 __global__ __launch_bounds__(kBlock) void k_update(Dims d, Params P, const double* __restrict__ pi,
                                                    const double* __restrict__ pb, const double* __restrict__ F1,
                                                    const double* __restrict__ F2, const double* __restrict__ F3,
@@ -406,9 +421,11 @@ __global__ __launch_bounds__(kBlock) void k_update(Dims d, Params P, const doubl
 #pragma unroll
     for (int pl = 0; pl < HARM_NPR; pl++) {
       double dU1 = 0, dU2 = 0, dU3 = 0;
-      if (DM & 1) dU1 = (-(F1[pl * nc + o + 1] - F1[pl * nc + o]) * idx1);
-      if (DM & 2) dU2 = (-(F2[pl * nc + o + s2] - F2[pl * nc + o]) * idx2);
-      if (DM & 4) dU3 = (-(F3[pl * nc + o + s3] - F3[pl * nc + o]) * idx3);
+      const bool e1 = DM < 0 ? d.n1 > 1 : (DM & 1), e2 = DM < 0 ? d.n2 > 1 : (DM & 2) != 0,
+                 e3 = DM < 0 ? d.n3 > 1 : (DM & 4) != 0;
+      if (e1) dU1 = (-(F1[pl * nc + o + 1] - F1[pl * nc + o]) * idx1);
+      if (e2) dU2 = (-(F2[pl * nc + o + s2] - F2[pl * nc + o]) * idx2);
+      if (e3) dU3 = (-(F3[pl * nc + o + s3] - F3[pl * nc + o]) * idx3);
       dUt[pl] = dU1 + dU2 + dU3;
     }
     // keep the phases apart: the scheduler would otherwise hoist every load of
@@ -459,27 +476,39 @@ __global__ __launch_bounds__(kBlock) void k_update(Dims d, Params P, const doubl
         dtsource[jj] = P.cour * (mydx[jj] / ag);
       }
       if (dtsource[0] < dts) dts = dtsource[0];
-      if ((DM & 1) && dtsource[1] < dts) dts = dtsource[1];
-      if ((DM & 2) && dtsource[2] < dts) dts = dtsource[2];
-      if ((DM & 4) && dtsource[3] < dts) dts = dtsource[3];
+      const bool e1 = DM < 0 ? d.n1 > 1 : (DM & 1), e2 = DM < 0 ? d.n2 > 1 : (DM & 2) != 0,
+                 e3 = DM < 0 ? d.n3 > 1 : (DM & 4) != 0;
+      if (e1 && dtsource[1] < dts) dts = dtsource[1];
+      if (e2 && dtsource[2] < dts) dts = dtsource[2];
+      if (e3 && dtsource[3] < dts) dts = dtsource[3];
     }
     __builtin_amdgcn_sched_barrier(0);
-    double pz[HARM_NPR], Ui[HARM_NPR];
+    double Ui[HARM_NPR];
+    if (MODE == kUpdAll || MODE == kUpdFirst) {
+      double pz[HARM_NPR];
 #pragma unroll
-    for (int pl = 0; pl < HARM_NPR; pl++) pz[pl] = pi[pl * nc + o];
-    {
+      for (int pl = 0; pl < HARM_NPR; pl++) pz[pl] = pi[pl * nc + o];
       State q;
       hd::get_state(pz, G, P.gam, q);
       hd::primtoflux(pz, q, 0, G, Ui);
+#pragma unroll
+      for (int pl = 0; pl < HARM_NPR; pl++) ui[pl * nc + o] = Ui[pl];
+    } else {
+#pragma unroll
+      for (int pl = 0; pl < HARM_NPR; pl++) Ui[pl] = ui[pl * nc + o];
     }
 #pragma unroll
     for (int pl = 0; pl < HARM_NPR; pl++) {
-      ui[pl * nc + o] = Ui[pl];
-      const double ufo = (timeorder == 0) ? 0.0 : uf[pl * nc + o];
-      const double uco = (timeorder == 0) ? 0.0 : ucum[pl * nc + o];
+      const bool rd = MODE == kUpdAll ? timeorder != 0 : MODE != kUpdFirst;
+      const double ufo = rd ? uf[pl * nc + o] : 0.0;
       const double ufn = (cuf0 * Ui[pl] + cuf1 * ufo + cuf2 * dt * dUt[pl]);
-      uf[pl * nc + o] = ufn;
-      ucum[pl * nc + o] = uco + (cun0 * Ui[pl] + cun1 * dt * dUt[pl] + cun2 * ufn);
+      if (MODE != kUpdLast) uf[pl * nc + o] = ufn;
+      if (MODE == kUpdAll) {
+        const double uco = (timeorder == 0) ? 0.0 : ucum[pl * nc + o];
+        ucum[pl * nc + o] = uco + (cun0 * Ui[pl] + cun1 * dt * dUt[pl] + cun2 * ufn);
+      } else if (MODE == kUpdLast) {
+        ucum[pl * nc + o] = 0.0 + (cun0 * Ui[pl] + cun1 * dt * dUt[pl] + cun2 * ufn);
+      }
     }
   }
   block_min_to(dts, ds + DS_ACCDT);
@@ -1139,27 +1168,28 @@ harm_status enq_fluxcalc(harm_handle* h, int which_p, int do_ct)
 }
 
 harm_status enq_update(harm_handle* h, int which_pi, int which_pb, const double* CUf, const double* Cunew,
-                       int timeorder)
+                       int timeorder, int mode)
 {
   const Dims& d = h->d;
   prof_begin(h);
   const int dm = (d.n1 > 1) | ((d.n2 > 1) << 1) | ((d.n3 > 1) << 2);
-#define LAUNCH_UPDATE(DM)                                                                                           \
-  hipLaunchKernelGGL(k_update<DM>, dim3(nblocks((long)d.n1 * d.n2 * d.n3)), dim3(kBlock), 0, h->stream, d, h->P,    \
-                     h->arr[which_pi], h->arr[which_pb], h->arr[HARM_ARR_F1], h->arr[HARM_ARR_F2],                  \
+#define LAUNCH_UPDATE(DM, MODE)                                                                                     \
+  hipLaunchKernelGGL((k_update<DM, MODE>), dim3(nblocks((long)d.n1 * d.n2 * d.n3)), dim3(kBlock), 0, h->stream, d, \
+                     h->P, h->arr[which_pi], h->arr[which_pb], h->arr[HARM_ARR_F1], h->arr[HARM_ARR_F2],           \
                      h->arr[HARM_ARR_F3], h->arr[HARM_ARR_UI], h->arr[HARM_ARR_UF], h->arr[HARM_ARR_UCUM],          \
                      h->geom[HARM_LOC_CENT], h->conn, CUf[0], CUf[1], CUf[2], Cunew[0], Cunew[1], Cunew[2],         \
                      timeorder, h->ds)
-  switch (dm) {
-    case 1: LAUNCH_UPDATE(1); break;
-    case 2: LAUNCH_UPDATE(2); break;
-    case 3: LAUNCH_UPDATE(3); break;
-    case 4: LAUNCH_UPDATE(4); break;
-    case 5: LAUNCH_UPDATE(5); break;
-    case 6: LAUNCH_UPDATE(6); break;
-    case 7: LAUNCH_UPDATE(7); break;
-    default: LAUNCH_UPDATE(0); break;
+#define LAUNCH_UPDATE_MODES(DM)                         \
+  switch (mode) {                                       \
+    case kUpdFirst: LAUNCH_UPDATE(DM, kUpdFirst); break;   \
+    case kUpdMiddle: LAUNCH_UPDATE(DM, kUpdMiddle); break; \
+    case kUpdLast: LAUNCH_UPDATE(DM, kUpdLast); break;     \
+    default: LAUNCH_UPDATE(DM, kUpdAll); break;            \
   }
+  if (dm == 7) { LAUNCH_UPDATE_MODES(7); }
+  else if (dm == 3) { LAUNCH_UPDATE_MODES(3); }
+  else { LAUNCH_UPDATE_MODES(-1); }
+#undef LAUNCH_UPDATE_MODES
 #undef LAUNCH_UPDATE
   prof_end(h, HARM_K_UPDATE);
   return hh::check_launch("k_update");
@@ -1235,8 +1265,11 @@ int rk_tables(int timeorder, double CUf[4][4], double Cunew[4][4])
 }
 
 // one RK substep of advance() + post_advance, all enqueued
+// in_step: the substep is part of a whole step enqueued in order (harm_step),
+// so the reduced-traffic update modes apply; a lone substep (harm_advance)
+// runs the reference loop as is.
 harm_status enq_substep(harm_handle* h, int to, int nto, const double CUf[4][4], const double Cunew[4][4],
-                        const double* wflux, bool diag)
+                        const double* wflux, bool diag, bool in_step)
 {
   const int pii[3] = {HARM_ARR_P, HARM_ARR_P, HARM_ARR_P};
   int pbb[3] = {HARM_ARR_P, HARM_ARR_PK0, HARM_ARR_PK1};
@@ -1247,7 +1280,14 @@ harm_status enq_substep(harm_handle* h, int to, int nto, const double CUf[4][4],
   if (diag && h->d.n1 > 1)
     hipLaunchKernelGGL(k_fluxacc, dim3(1), dim3(kBlock), 0, h->stream, h->d, h->P, h->arr[HARM_ARR_F1], h->ds,
                        wflux[to]);
-  HH_TRY(enq_update(h, pii[to], pbb[to], CUf[to], Cunew[to], to));
+  // The reduced-traffic modes rely on the tables keeping ucum at zero until
+  // the last substep (true of the RK2 and RK3 tables); otherwise run the
+  // reference loop as is.
+  bool zero_ucum = in_step;
+  for (int s = 0; s + 1 < nto; s++)
+    zero_ucum = zero_ucum && Cunew[s][0] == 0.0 && Cunew[s][1] == 0.0 && Cunew[s][2] == 0.0;
+  const int mode = !zero_ucum ? kUpdAll : to == 0 ? kUpdFirst : finalstep ? kUpdLast : kUpdMiddle;
+  HH_TRY(enq_update(h, pii[to], pbb[to], CUf[to], Cunew[to], to, mode));
   HH_TRY(enq_invert(h, pbb[to], pff[to], finalstep, false));
   hipLaunchKernelGGL(k_scal_substep_dt, dim3(1), dim3(1), 0, h->stream, h->ds);
   HH_TRY(enq_bound(h, pff[to]));
@@ -1511,7 +1551,7 @@ extern "C" harm_status harm_update(harm_handle* h, int32_t which_pi, int32_t whi
   v[DS_DT] = dt;
   v[DS_ACCDT] = kBIG;
   HH_HIP(hipMemcpy(h->ds, v, sizeof(v), hipMemcpyHostToDevice));
-  HH_TRY(enq_update(h, which_pi, which_pb, CUf, Cunew, timeorder));
+  HH_TRY(enq_update(h, which_pi, which_pb, CUf, Cunew, timeorder, kUpdAll));
   if (accdt) {
     HH_HIP(hipStreamSynchronize(h->stream));
     HH_HIP(hipMemcpy(v, h->ds, sizeof(v), hipMemcpyDeviceToHost));
@@ -1560,7 +1600,7 @@ extern "C" harm_status harm_advance(harm_handle* h, int32_t timeorder, double* n
   if (timeorder < 0 || timeorder >= nto) return hh::set_error(HARM_EINVAL, "harm_advance: bad timeorder");
   rk_flux_weights(h->cfg.timeorder, w);
   if (timeorder == 0) hipLaunchKernelGGL(k_scal_begin_step, dim3(1), dim3(1), 0, h->stream, h->ds, h->di);
-  HH_TRY(enq_substep(h, timeorder, nto, CUf, Cunew, w, false));
+  HH_TRY(enq_substep(h, timeorder, nto, CUf, Cunew, w, false, false));
   if (ndt) {
     double v[DS_N];
     HH_HIP(hipStreamSynchronize(h->stream));
@@ -1578,7 +1618,7 @@ extern "C" harm_status harm_step(harm_handle* h, int32_t nsteps)
   rk_flux_weights(h->cfg.timeorder, w);
   for (int n = 0; n < nsteps; n++) {
     hipLaunchKernelGGL(k_scal_begin_step, dim3(1), dim3(1), 0, h->stream, h->ds, h->di);
-    for (int to = 0; to < nto; to++) HH_TRY(enq_substep(h, to, nto, CUf, Cunew, w, true));
+    for (int to = 0; to < nto; to++) HH_TRY(enq_substep(h, to, nto, CUf, Cunew, w, true, true));
     hipLaunchKernelGGL(k_scal_end_step, dim3(1), dim3(1), 0, h->stream, h->ds, h->P.safe);
   }
   return hh::check_launch("harm_step");
