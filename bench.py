@@ -24,7 +24,28 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 (vector and matrix)
 GRID = (256, 128, 64)
+COUNTERS = os.path.join(ROOT, "profiles", "counters_step_torus256x128x64_weno5_hll_rk3.md")
+
+
+def measured_counters(path=COUNTERS):
+    """Per-launch HBM traffic and FP64 FLOPs of each kernel group, from the
+    committed rocprofv3 --pmc summary (FETCH_SIZE / WRITE_SIZE scaled by the
+    copy-kernel calibration; SQ_INSTS_VALU_*_F64).  None when absent."""
+    try:
+        lines = open(path).read().splitlines()
+    except OSError:
+        return None
+    out = {}
+    for i, ln in enumerate(lines):
+        if ln.startswith("group traffic_bytes"):
+            for row in lines[i + 1:]:
+                f = row.split()
+                if len(f) != 5:
+                    break
+                out[f[0]] = {"traffic": float(f[1]), "read": float(f[2]), "write": float(f[3]), "flops": float(f[4])}
+    return out or None
 
 
 def algorithmic_bytes(g, kernel):
@@ -56,6 +77,12 @@ def algorithmic_bytes(g, kernel):
         return zones * (64 + 64 + 64 + 4) + n1 * n2 * geom
     if kernel == "emf_ct":
         return zones * (6 * 8 + 3 * 8 + 3 * 8 + 6 * 8)
+    if kernel == "bound":
+        # every ghost zone written once from a zone read once (8 doubles each)
+        ghosts = (n1 + 2 * 4 * e0) * (n2 + 2 * 4 * e1) * (n3 + 2 * 4 * e2) - zones
+        return ghosts * 128
+    if kernel == "fixup":
+        return zones * 4  # the failure flags; failed zones are few
     return None
 
 
@@ -147,6 +174,20 @@ def main():
                    "GBs_alg": round(algorithmic_bytes(g, k) / (v[0] / 1e3 / v[1]) / 1e9, 1)
                    if v[1] and algorithmic_bytes(g, k) else None} for k, v in prof.items() if v[1]}
 
+    meas = measured_counters() if (n1, n2, n3) == GRID else None
+    md = meas.get(dom) if meas else None
+    fp64 = None
+    if md and md["flops"] > 0:
+        tf = md["flops"] / avg_s / 1e12
+        fp64 = {"bound": "fp64", "kernel": dom, "achieved": tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": tf / FP64_PEAK_TFLOPS, "flops_per_launch": md["flops"],
+                "source": "rocprofv3 --pmc SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64, " + os.path.relpath(COUNTERS, ROOT)}
+    # whole step against HBM: algorithmic bytes of every kernel group per step
+    step_bytes = sum((algorithmic_bytes(g, k) or 0) * prof[k][1] / args.steps for k in prof if prof[k][1])
+    step_roof = {"bytes_per_step_alg": step_bytes, "achieved": step_bytes / (el / args.steps) / 1e9,
+                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": step_bytes / (el / args.steps) / 1e9 / HBM_PEAK_GBS}
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(cfg, args.cpu_steps)
@@ -169,8 +210,13 @@ def main():
                    "form": "plain (multi-kernel step, Newton from previous primitives)",
                    "sim_time_end": t_sim, "dt": dt_now},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS if achieved else None,
+                     "traffic": md["traffic"] if md else None,
+                     "traffic_source": ("rocprofv3 --pmc FETCH_SIZE x2.0 + WRITE_SIZE x1.0 (1 GiB copy calibration), "
+                                        + os.path.relpath(COUNTERS, ROOT)) if md else None,
                      "bytes_per_launch_alg": nbytes, "avg_launch_ms": avg_s * 1e3},
+        "roofline_fp64": fp64,
+        "step_roofline_hbm": step_roof,
         "kernels": kernels,
         "cpu_baseline": cpu,
     }

@@ -595,8 +595,8 @@ __global__ __launch_bounds__(kBlock) void k_invert(Dims d, Params P, const doubl
 // ---------------------------------------------------------------------------
 // failure averaging (fixup_utoprim with the simple_average cascade)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_post_fixup(Dims d, Params P, const double* __restrict__ pa,
-                                                       double* __restrict__ pf, const int32_t* __restrict__ pflag,
+__global__ __launch_bounds__(kBlock) void k_post_fixup(Dims d, Params P, const double* pa, double* pf,
+                                                       const int32_t* __restrict__ pflag,
                                                        const double* __restrict__ gcent)
 {
   const long t = (long)blockIdx.x * kBlock + threadIdx.x;
@@ -911,7 +911,6 @@ struct harm_handle {
   Params P;
   hipStream_t stream = nullptr;
   double* arr[HARM_NARR] = {nullptr};
-  double* pa = nullptr;
   int32_t* pflag = nullptr;
   double* geom[HARM_NLOC] = {nullptr};
   double* gcorn3 = nullptr;
@@ -1239,9 +1238,10 @@ harm_status enq_post_fixup(harm_handle* h, int which)
 {
   const Dims& d = h->d;
   prof_begin(h);
-  HH_HIP(hipMemcpyAsync(h->pa, h->arr[which], sizeof(double) * HARM_NPR * d.ncell, hipMemcpyDeviceToDevice, h->stream));
+  // No snapshot is needed: a failed zone averages only good neighbours (or
+  // ghost zones, or itself), and only failed zones are written.
   hipLaunchKernelGGL(k_post_fixup, dim3(nblocks((long)d.n1 * d.n2 * d.n3)), dim3(kBlock), 0, h->stream, d, h->P,
-                     h->pa, h->arr[which], h->pflag, h->geom[HARM_LOC_CENT]);
+                     h->arr[which], h->arr[which], h->pflag, h->geom[HARM_LOC_CENT]);
   prof_end(h, HARM_K_FIXUP);
   return hh::check_launch("k_post_fixup");
 }
@@ -1342,7 +1342,6 @@ extern "C" harm_status harm_create(const harm_config* cfg, harm_handle** out)
     TRYC(alloc((void**)&h->arr[a], sizeof(double) * HARM_NPR * d.ncell));
     TRYC(hh::check_hip(hipMemset(h->arr[a], 0, sizeof(double) * HARM_NPR * d.ncell), "hipMemset"));
   }
-  TRYC(alloc((void**)&h->pa, sizeof(double) * HARM_NPR * d.ncell));
   TRYC(alloc((void**)&h->pflag, sizeof(int32_t) * d.ncell));
   TRYC(hh::check_hip(hipMemset(h->pflag, 0, sizeof(int32_t) * d.ncell), "hipMemset"));
   h->emf_ni = d.n1 + (d.n1 > 1);
@@ -1374,7 +1373,7 @@ extern "C" harm_status harm_destroy(harm_handle* h)
     (void)hipEventDestroy(r.second.second);
   }
   for (hipEvent_t e : h->evpool) (void)hipEventDestroy(e);
-  void* bufs[] = {h->pa, h->pflag, h->gcorn3, h->conn, h->floors, h->rcent, h->emf, h->ds, h->di};
+  void* bufs[] = {h->pflag, h->gcorn3, h->conn, h->floors, h->rcent, h->emf, h->ds, h->di};
   for (void* b : bufs) (void)hipFree(b);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;

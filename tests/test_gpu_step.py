@@ -158,6 +158,49 @@ def test_invert_matches_oracle(case, finalstep):
 
 
 @pytest.mark.parametrize("case", list(CASES))
+def test_post_fixup_matches_oracle(case):
+    """Failure averaging after an inversion with planted failures."""
+    cfg, og, g = make(case)
+    p = perturbed_torus(og)
+    pk = perturbed_torus(og, seed=9, amp=0.05)
+    g.set("p", p)
+    g.set("pk0", pk)
+    F = [og.zeros() for _ in range(3)]
+    og.fluxcalc(pk, *F, do_ct=True)
+    g.fluxcalc("pk0", do_ct=True)
+    n, CUf, Cunew = og.rk_tables()
+    ui, uf, ucum = og.zeros(), og.zeros(), og.zeros()
+    og.update(p, pk, *F, ui, uf, ucum, CUf[0], Cunew[0], 0.02, 0)
+    g.update("p", "pk0", CUf[0], Cunew[0], 0.02, 0)
+    Ug = g.get("uf")
+    a = act(og, uf)
+    rng = np.random.default_rng(4)
+    # isolated failures and a few adjacent pairs, so every branch of the
+    # averaging cascade is reached
+    idx = [tuple(rng.integers(0, s, 12) for s in a.shape[:3])]
+    k0, j0, i0 = (x[:4] for x in idx[0])
+    idx.append((k0, j0, np.minimum(i0 + 1, a.shape[2] - 1)))
+    idx.append((k0, np.minimum(j0 + 1, a.shape[1] - 1), i0))
+    for ix in idx:
+        a[ix + (1,)] = np.abs(a[ix + (1,)]) * 3.0
+    act(og, Ug)[...] = a
+    g.set("uf", Ug)
+    pf = og.zeros()
+    pflag = np.zeros(og.shape[:3], dtype=np.int32)
+    og.invert(pk, uf, pf, pflag, 0)
+    g.invert("pk0", "pk1", 0)
+    nfail = int((act(og, pflag) > 0).sum())
+    og.bound(pf)
+    g.bound("pk1")
+    og.post_fixup(pf, pflag)
+    g.post_fixup("pk1")
+    err = relmax(act(og, g.get("pk1")), act(og, pf))
+    print(case, "failed zones", nfail, "post_fixup relmax", err)
+    assert nfail > 0
+    assert err <= 1e-12
+
+
+@pytest.mark.parametrize("case", list(CASES))
 def test_full_steps_match_oracle(case):
     cfg, og, g = make(case)
     p = perturbed_torus(og)
