@@ -872,15 +872,22 @@ __global__ void k_scal_end_step(double* ds, double safe)
   ds[DS_DT] = fullndt;
 }
 
-// boundary flux sums through the inner and outer radial faces (deterministic)
-__global__ __launch_bounds__(kBlock) void k_fluxacc(Dims d, Params P, const double* __restrict__ F1,
-                                                    double* __restrict__ ds, double weight)
+// Boundary flux sums through the inner and outer radial faces
+// (diag_flux_pureflux), deterministic: kFaccBlocks blocks each reduce a fixed
+// slice of the (j, k) faces in a fixed tree order into `part`, then one block
+// adds the partial sums in block order.
+constexpr int kFaccBlocks = 64;
+
+__global__ __launch_bounds__(kBlock) void k_fluxacc_part(Dims d, const double* __restrict__ F1,
+                                                         double* __restrict__ part)
 {
   __shared__ double red[4][kBlock];
   double s[4] = {0, 0, 0, 0};
+  const int nf = d.n2 * d.n3, chunk = (nf + kFaccBlocks - 1) / kFaccBlocks;
+  const int t0 = blockIdx.x * chunk, t1 = min(nf, t0 + chunk);
   const long nc = d.ncell;
-  for (long t = threadIdx.x; t < (long)d.n2 * d.n3; t += kBlock) {
-    const int j = (int)(t % d.n2), k = (int)(t / d.n2);
+  for (int t = t0 + threadIdx.x; t < t1; t += kBlock) {
+    const int k = t / d.n2, j = t - k * d.n2;
     const long oi = hd::cidx(d, 0, j, k), oo = hd::cidx(d, d.n1, j, k);
     s[0] += F1[HARM_RHO * nc + oi];
     s[1] += F1[HARM_UU * nc + oi];
@@ -894,8 +901,16 @@ __global__ __launch_bounds__(kBlock) void k_fluxacc(Dims d, Params P, const doub
       for (int c = 0; c < 4; c++) red[c][threadIdx.x] += red[c][threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0)
-    for (int c = 0; c < 4; c++) ds[DS_FLUXACC + c] += red[c][0] * P.dx[2] * P.dx[3] * ds[DS_DT] * weight;
+  if (threadIdx.x < 4) part[blockIdx.x * 4 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ void k_fluxacc_final(Params P, const double* __restrict__ part, double* __restrict__ ds, double weight)
+{
+  const int c = threadIdx.x;
+  if (c >= 4) return;
+  double v = 0.0;
+  for (int b = 0; b < kFaccBlocks; b++) v += part[b * 4 + c];
+  ds[DS_FLUXACC + c] += v * P.dx[2] * P.dx[3] * ds[DS_DT] * weight;
 }
 
 inline unsigned nblocks(long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
@@ -918,6 +933,7 @@ struct harm_handle {
   double* floors = nullptr;
   double* rcent = nullptr;
   double* emf = nullptr;
+  double* facc = nullptr;  // boundary-flux partial sums
   double* ds = nullptr;
   unsigned long long* di = nullptr;
   int emf_ni = 0, emf_nj = 0, emf_nk = 0;
@@ -1277,9 +1293,11 @@ harm_status enq_substep(harm_handle* h, int to, int nto, const double CUf[4][4],
   if (nto == 2) { pbb[1] = HARM_ARR_PK0; pff[1] = HARM_ARR_P; }
   const int finalstep = (to == nto - 1);
   HH_TRY(enq_fluxcalc(h, pbb[to], 1));
-  if (diag && h->d.n1 > 1)
-    hipLaunchKernelGGL(k_fluxacc, dim3(1), dim3(kBlock), 0, h->stream, h->d, h->P, h->arr[HARM_ARR_F1], h->ds,
-                       wflux[to]);
+  if (diag && h->d.n1 > 1) {
+    hipLaunchKernelGGL(k_fluxacc_part, dim3(kFaccBlocks), dim3(kBlock), 0, h->stream, h->d, h->arr[HARM_ARR_F1],
+                       h->facc);
+    hipLaunchKernelGGL(k_fluxacc_final, dim3(1), dim3(64), 0, h->stream, h->P, h->facc, h->ds, wflux[to]);
+  }
   // The reduced-traffic modes rely on the tables keeping ucum at zero until
   // the last substep (true of the RK2 and RK3 tables); otherwise run the
   // reference loop as is.
@@ -1349,6 +1367,7 @@ extern "C" harm_status harm_create(const harm_config* cfg, harm_handle** out)
   h->emf_nk = d.n3 + (d.n3 > 1);
   TRYC(alloc((void**)&h->emf, sizeof(double) * 3 * (size_t)h->emf_ni * h->emf_nj * h->emf_nk));
   TRYC(alloc((void**)&h->ds, sizeof(double) * DS_N));
+  TRYC(alloc((void**)&h->facc, sizeof(double) * 4 * kFaccBlocks));
   TRYC(hh::check_hip(hipMemset(h->ds, 0, sizeof(double) * DS_N), "hipMemset"));
   TRYC(alloc((void**)&h->di, sizeof(unsigned long long) * DI_N));
   TRYC(hh::check_hip(hipMemset(h->di, 0, sizeof(unsigned long long) * DI_N), "hipMemset"));
@@ -1373,7 +1392,7 @@ extern "C" harm_status harm_destroy(harm_handle* h)
     (void)hipEventDestroy(r.second.second);
   }
   for (hipEvent_t e : h->evpool) (void)hipEventDestroy(e);
-  void* bufs[] = {h->pflag, h->gcorn3, h->conn, h->floors, h->rcent, h->emf, h->ds, h->di};
+  void* bufs[] = {h->pflag, h->gcorn3, h->conn, h->floors, h->rcent, h->emf, h->facc, h->ds, h->di};
   for (void* b : bufs) (void)hipFree(b);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
