@@ -100,6 +100,16 @@ _OPTIONAL_PROTOS = [
     ("harm_get_stream", _I, [_P, ctypes.POINTER(_P)]),
     ("harm_profile", _I, [_P, _I]),
     ("harm_profile_read", _I, [_P, _P, _P]),
+    ("harm_mlp_create", _I, [ctypes.c_uint64, ctypes.POINTER(_P)]),
+    ("harm_mlp_destroy", _I, [_P]),
+    ("harm_mlp_get_params", _I, [_P, _P, _P]),
+    ("harm_mlp_set_params", _I, [_P, _P, _P]),
+    ("harm_mlp_pairs", _I, [_P, _P, _P, _L, _D, _D, _P, _P, _P, _P]),
+    ("harm_mlp_fit_norm", _I, [_P, _P, _P, _P, _L]),
+    ("harm_mlp_train", _I, [_P, _P, _P, _L, _P, _L, _L, _I, ctypes.c_float, ctypes.POINTER(ctypes.c_float), _P]),
+    ("harm_mlp_grad", _I, [_P, _P, _P, _L, _P, _P, ctypes.POINTER(ctypes.c_float)]),
+    ("harm_mlp_forward", _I, [_P, _P, _P, _L, _P]),
+    ("harm_invert_batch_mlp", _I, [_P, _P, _P, _P, _P, _P, _L, _D, _D, _P, _P]),
 ]
 
 _lib = None

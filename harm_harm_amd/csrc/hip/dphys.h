@@ -428,19 +428,21 @@ __device__ __forceinline__ int newton(int& pflag, double& xio, const Inv& s, int
 
 // Utoprim_new_body on alpha-scaled U (Ut) and guess (pt).  If wp_seed is
 // finite it replaces the W' guess (MLP-seeded form); Newton stays the judge.
-__device__ __forceinline__ int new_body(int& pflag, const double* Ut, const Geom& g, double gam, double gammafail,
-                                        double* prim, int& nstroke, double wp_seed)
+// The frame-independent invariants of the conserved state that the 1-D W'
+// solve works with (the set-up part of Utoprim_new_body, utoprim_jon.c:716):
+// D, B^2, the spatial momentum Q~^2, Q.B and Q.n (rest mass removed), plus
+// Q~^mu and B^mu for the final primitive reconstruction.
+__device__ __forceinline__ void inv_setup(const double* Ut, const Geom& g, double gam, double gammafail, Inv& s,
+                                          double* Qtcon, double* Bcon)
 {
-  Inv s;
-  pflag = 0;
   s.gam = gam;
   s.gsq_too_big = gammafail * gammafail;
   s.utsq_too_big = (gammafail - 1.0) * (gammafail - 1.0);
   s.D = Ut[HARM_RHO];
-  prim[HARM_B1] = Ut[HARM_B1];
-  prim[HARM_B2] = Ut[HARM_B2];
-  prim[HARM_B3] = Ut[HARM_B3];
-  const double Bcon[4] = {0., Ut[HARM_B1], Ut[HARM_B2], Ut[HARM_B3]};
+  Bcon[0] = 0.;
+  Bcon[1] = Ut[HARM_B1];
+  Bcon[2] = Ut[HARM_B2];
+  Bcon[3] = Ut[HARM_B3];
   double Bcov[4];
   lower(Bcon, g, Bcov);
   s.Bsq = 0.;
@@ -451,7 +453,7 @@ __device__ __forceinline__ int new_body(int& pflag, const double* Ut, const Geom
   const double Qcovp[4] = {Ut[HARM_UU], Ut[HARM_U1], Ut[HARM_U2], Ut[HARM_U3]};
   double Qconp[4];
   raise(Qcovp, g, Qconp);
-  double Qtcon[4] = {0.0, 0.0, 0.0, 0.0};
+  Qtcon[0] = 0.0; Qtcon[1] = 0.0; Qtcon[2] = 0.0; Qtcon[3] = 0.0;
 #pragma unroll
   for (int j = 1; j < 4; j++)
 #pragma unroll
@@ -472,6 +474,18 @@ __device__ __forceinline__ int new_body(int& pflag, const double* Ut, const Geom
     const double Dfactor = (-g.gpert_tt + alphasq * (g.bsqoa2)) / (alphasq + alpha);
     s.Qdotnp = Qconp[0] * ncov[0] + (s.D) * (Dfactor);
   }
+}
+
+__device__ __forceinline__ int new_body(int& pflag, const double* Ut, const Geom& g, double gam, double gammafail,
+                                        double* prim, int& nstroke, double wp_seed)
+{
+  Inv s;
+  pflag = 0;
+  prim[HARM_B1] = Ut[HARM_B1];
+  prim[HARM_B2] = Ut[HARM_B2];
+  prim[HARM_B3] = Ut[HARM_B3];
+  double Qtcon[4], Bcon[4];
+  inv_setup(Ut, g, gam, gammafail, s, Qtcon, Bcon);
 
   // set_guess_Wp + verify_Wlast
   double Wp_last, W_last;

@@ -252,6 +252,55 @@ void harm_geom_mks_record(const harm_config* cfg, const double X[4], double* rec
 /* n Kerr-Schild records into host SoA rec[f*n + z] (host threads). */
 void harm_geom_ks_batch(int64_t n, const double* a, const double* r, const double* th, double* rec);
 
+/* ------------------------------------------------------------------------ */
+/* Learned initial guess for the inversion (no counterpart in the reference:  */
+/* HARM starts every Newton solve from the previous primitives).             */
+/* ------------------------------------------------------------------------ */
+/* An 8 -> 64 -> 64 -> 2 perceptron with tanh hidden layers, evaluated with
+ * BF16 operands on the matrix cores (FP32 accumulation), trained on the GPU
+ * by this library's own forward and backward kernels with Adam.  Inputs are
+ * eight scale-free features of the solve's invariants (D, B^2, Q~^2, (Q.B)^2,
+ * Q.n); outputs are log(W'/D) and log(u~^2) of the solution.  The FP64 Newton
+ * solve stays the judge: the network only proposes its starting W'.
+ * Parameters: W1[8][64] b1[64] W2[64][64] b2[64] W3[64][2] b3[2] (FP32,
+ * row-major [in][out]); standardisation: feature mean[8], feature scale[8]
+ * (1/std), target mean[2], target std[2]. */
+enum { HARM_MLP_NPARAM = 4866, HARM_MLP_NNORM = 20, HARM_MLP_NIN = 8, HARM_MLP_NOUT = 2,
+       HARM_MLP_BATCH = 128 };
+typedef struct harm_mlp harm_mlp;
+/* Glorot-uniform weights from a deterministic generator, zero biases. */
+harm_status harm_mlp_create(uint64_t seed, harm_mlp** out);
+harm_status harm_mlp_destroy(harm_mlp* m);
+/* Host copies in and out (either pointer may be NULL). */
+harm_status harm_mlp_get_params(harm_mlp* m, float* params, float* norm);
+harm_status harm_mlp_set_params(harm_mlp* m, const float* params, const float* norm);
+/* Training pairs from solved inversions (device pointers, SoA as in
+ * harm_invert_batch): X[z][8] raw features, T[z][2] targets, ok[z] = 1 when
+ * the pair is usable. */
+harm_status harm_mlp_pairs(const double* U, const double* geom, const double* p, int64_t n, double gam,
+                           double gammafail, float* X, float* T, int32_t* ok, void* stream);
+/* Mean and spread of the usable pairs' features and targets. */
+harm_status harm_mlp_fit_norm(harm_mlp* m, const float* X, const float* T, const int32_t* ok, int64_t n);
+/* nsteps Adam steps on minibatches of HARM_MLP_BATCH pairs taken in the order
+ * order[(first + s*128 + b) % norder] (device int32 indices into the nx rows
+ * of X/T, each in [0, nx)); *loss (host, may be NULL) gets the mean loss over
+ * the steps (synchronises). */
+harm_status harm_mlp_train(harm_mlp* m, const float* X, const float* T, int64_t nx, const int32_t* order,
+                           int64_t norder, int64_t first, int32_t nsteps, float lr, float* loss, void* stream);
+/* Gradients of the mean-squared loss of one minibatch (rows order[0..127] of
+ * the nx rows of X/T) with respect to all HARM_MLP_NPARAM parameters into
+ * device grad[], no update; *loss (host) gets the batch loss. */
+harm_status harm_mlp_grad(harm_mlp* m, const float* X, const float* T, int64_t nx, const int32_t* order, float* grad,
+                          float* loss);
+/* Y[z][2] = network(X[z][8]) in target units (device pointers). */
+harm_status harm_mlp_forward(harm_mlp* m, const float* X, float* Y, int64_t n, void* stream);
+/* harm_invert_batch started from the network's W'; a zone whose seeded solve
+ * fails is solved again from p_inout's guess, and *nfallback (device
+ * counter, may be NULL) is incremented for it. */
+harm_status harm_invert_batch_mlp(harm_mlp* m, const double* U, const double* geom, double* p_inout,
+                                  int32_t* pflag, int32_t* niter, int64_t n, double gam, double gammafail,
+                                  unsigned long long* nfallback, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

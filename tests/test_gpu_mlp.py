@@ -1,0 +1,132 @@
+"""GPU tests of the learned inversion guess: the forward and backward kernels
+against the CPU reference (oracle/mlp_ref.py, BF16 rounding points
+emulated), deterministic training, and the seeded Newton solve.
+
+Tolerances: the GPU and CPU sides round the same operands to BF16 but may
+accumulate products in a different order (FP32 on the matrix cores), so a
+hidden activation can land one BF16 ulp apart; 1e-3 of each quantity's
+largest magnitude covers that.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import harm_harm_amd as H
+from harm_harm_amd.mlp import MLP, NPARAM
+from harm_harm_amd.sampling import sample_inversion_inputs
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import mlp_ref  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+D = H.DeviceArray
+
+
+def random_norm(rng):
+    return np.concatenate([rng.normal(0, 0.5, 8), rng.uniform(0.5, 2, 8), rng.normal(0, 1, 2),
+                           rng.uniform(0.5, 2, 2)]).astype(np.float32)
+
+
+def test_forward_matches_cpu():
+    rng = np.random.default_rng(0)
+    m = MLP(seed=3)
+    w, _ = m.params()
+    nm = random_norm(rng)
+    m.set_params(None, nm)
+    n = 1000  # not a multiple of the block size
+    X = rng.normal(0, 1.5, (n, 8)).astype(np.float32)
+    Y = m.forward(D.from_numpy(X), n).numpy()
+    Xn, _ = mlp_ref.standardise(X, None, nm)
+    Yr, _ = mlp_ref.forward(w, Xn)
+    Yr = Yr * nm[18:20] + nm[16:18]
+    err = np.abs(Y - Yr).max() / np.abs(Yr).max()
+    print("forward relmax", err)
+    assert err <= 1e-3
+
+
+def test_gradients_match_cpu():
+    rng = np.random.default_rng(1)
+    m = MLP(seed=5)
+    w = rng.normal(0, 0.3, NPARAM).astype(np.float32)
+    nm = random_norm(rng)
+    m.set_params(w, nm)
+    X = rng.normal(0, 1.5, (300, 8)).astype(np.float32)
+    T = rng.normal(0, 1, (300, 2)).astype(np.float32)
+    order = rng.choice(300, 128, replace=False).astype(np.int32)
+    g, loss = m.grad(D.from_numpy(X), D.from_numpy(T), 300, D.from_numpy(order))
+    Xn, Tn = mlp_ref.standardise(X[order], T[order], nm)
+    lr, gr = mlp_ref.loss_and_grads(w, Xn, Tn)
+    assert abs(loss - lr) <= 1e-4 * lr
+    for k, (o, sh) in mlp_ref.OFF.items():
+        a, b = g[o:o + int(np.prod(sh))], gr[o:o + int(np.prod(sh))]
+        err = np.abs(a - b).max() / np.abs(b).max()
+        cos = np.dot(a, b) / np.linalg.norm(a) / np.linalg.norm(b)
+        print(k, "relmax", err, "cos", cos)
+        assert err <= 1e-3 and cos >= 1 - 1e-6
+
+
+def solved_pairs(n, seed):
+    p, guess, geom = sample_inversion_inputs(n, seed=seed)
+    dp, dg, dU = D.from_numpy(p), D.from_numpy(geom), D(p.shape, np.float64)
+    H.check(H.lib().harm_prim_to_cons_batch(dp.ptr, dg.ptr, dU.ptr, n, 4.0 / 3.0, None), "p2u")
+    dq = D.from_numpy(guess)
+    fl, it = D.zeros(n, np.int32), D.zeros(n, np.int32)
+    H.check(H.lib().harm_invert_batch(dU.ptr, dg.ptr, dq.ptr, fl.ptr, it.ptr, n, 4.0 / 3.0, 2500.0, None), "inv")
+    H.synchronize()
+    return p, guess, dg, dU, dq, fl.numpy(), it.numpy()
+
+
+@pytest.fixture(scope="module")
+def trained():
+    n = 1 << 16
+    _, _, dg, dU, dq, fl, _ = solved_pairs(n, seed=11)
+    assert (fl != 0).sum() == 0
+    X, T, ok = MLP.pairs(dU, dg, dq, n)
+    okh = ok.numpy()
+    assert okh.sum() == n
+    order = np.nonzero(okh)[0].astype(np.int32)
+    np.random.default_rng(0).shuffle(order)
+    dorder = D.from_numpy(order)
+    models = []
+    for rep in range(2):
+        m = MLP(seed=1)
+        m.fit_norm(X, T, ok, n)
+        losses = m.train(X, T, n, dorder, 3000, lr=3e-3, lr_end=3e-5, chunk=500)
+        models.append((m, losses))
+    return models
+
+
+def test_training_reduces_loss_and_is_reproducible(trained):
+    (m1, l1), (m2, l2) = trained
+    print("losses", l1)
+    assert l1[-1] < 0.1 * l1[0]
+    w1, n1 = m1.params()
+    w2, n2 = m2.params()
+    assert np.array_equal(w1, w2) and np.array_equal(n1, n2) and l1 == l2
+
+
+def test_seeded_newton_converges_like_plain(trained):
+    """Every zone converges from the network's W' (no extra failures), to the
+    plain solve's accuracy against the true primitives; the mean iteration
+    counts of both are reported (the gain is small: the inverter's stopping
+    test for ultra-relativistic zones needs |dW'/W'| <= machine epsilon and
+    takes 11-13 iterations from any start)."""
+    m, _ = trained[0]
+    n = 1 << 14
+    p, guess, dg, dU, dq, fl_plain, it_plain = solved_pairs(n, seed=7)
+    plain = dq.numpy()
+    ds = D.from_numpy(guess)
+    fl, it, fb = D.zeros(n, np.int32), D.zeros(n, np.int32), D.zeros(1, np.uint64)
+    m.invert_batch(dU, dg, ds, fl, it, n, dfallback=fb)
+    H.synchronize()
+    seeded, fls, its = ds.numpy(), fl.numpy(), it.numpy()
+    print("mean iterations plain", it_plain.mean(), "seeded", its.mean(), "fallbacks", int(fb.numpy()[0]))
+    assert (fls != 0).sum() <= (fl_plain != 0).sum()
+    assert (fls != 0).sum() == 0
+    for v in range(5):
+        e_plain = np.abs(plain[v] - p[v]).max() / np.abs(p[v]).max()
+        e_seed = np.abs(seeded[v] - p[v]).max() / np.abs(p[v]).max()
+        print("var", v, "error vs truth: plain", e_plain, "seeded", e_seed)
+        assert e_seed <= 10 * e_plain + 1e-12
