@@ -93,14 +93,15 @@ def trained():
     for rep in range(2):
         m = MLP(seed=1)
         m.fit_norm(X, T, ok, n)
+        _, loss0 = m.grad(X, T, n, dorder)  # untrained loss on the first minibatch
         losses = m.train(X, T, n, dorder, 3000, lr=3e-3, lr_end=3e-5, chunk=500)
-        models.append((m, losses))
+        models.append((m, [loss0] + losses))
     return models
 
 
 def test_training_reduces_loss_and_is_reproducible(trained):
     (m1, l1), (m2, l2) = trained
-    print("losses", l1)
+    print("untrained loss and mean loss per 500 steps", l1)
     assert l1[-1] < 0.1 * l1[0]
     w1, n1 = m1.params()
     w2, n2 = m2.params()
