@@ -114,14 +114,115 @@ def cpu_baseline(cfg, nsteps):
                       f"OpenMP on {threads} host threads"}
 
 
+def inversion_microbench(args):
+    """Batched inversion on seeded synthetic zones (Lorentz factor 1-50,
+    magnetisation 1e-4-1e3, u/rho 1e-6-1e2 log-uniform, random Kerr-Schild
+    points): the plain Newton solve from a +-20% perturbed guess against the
+    same solve seeded by the learned guess (trained first, on the GPU, on pairs
+    from plain solves of an independent sample)."""
+    import numpy as np
+    import harm_harm_amd as H
+    from harm_harm_amd.mlp import MLP
+    from harm_harm_amd.sampling import sample_inversion_inputs
+    D = H.DeviceArray
+    n = args.n
+    log = lambda *a: print(*a, file=sys.stderr, flush=True)
+    t0 = time.time()
+    p, guess, geom = sample_inversion_inputs(n, seed=7)
+    log(f"sampled {n} zones in {time.time() - t0:.1f} s")
+    dp, dg, dU = D.from_numpy(p), D.from_numpy(geom), D(p.shape, np.float64)
+    H.check(H.lib().harm_prim_to_cons_batch(dp.ptr, dg.ptr, dU.ptr, n, 4.0 / 3.0, None), "p2u")
+    dp.free()
+    dgs = D.from_numpy(guess)
+    dq = D(p.shape, np.float64)
+    fl, it = D.zeros(n, np.int32), D.zeros(n, np.int32)
+    e0, e1 = H.Event(), H.Event()
+
+    def timed(fn, reps=3):
+        best = None
+        for _ in range(reps):
+            dq.copy_from_device(dgs)
+            H.synchronize()
+            e0.record()
+            fn()
+            e1.record()
+            H.synchronize()
+            ms = e0.elapsed_ms(e1)
+            best = ms if best is None else min(best, ms)
+        return best
+
+    ms_plain = timed(lambda: H.check(H.lib().harm_invert_batch(dU.ptr, dg.ptr, dq.ptr, fl.ptr, it.ptr, n, 4.0 / 3.0,
+                                                               2500.0, None), "invert"))
+    q_plain, f_plain, i_plain = dq.numpy(), fl.numpy(), it.numpy()
+    # training pairs from an independent sample solved by the plain inversion
+    ntr = args.train_n
+    pt, gt, geot = sample_inversion_inputs(ntr, seed=11)
+    dpt, dgt, dUt = D.from_numpy(pt), D.from_numpy(geot), D(pt.shape, np.float64)
+    H.check(H.lib().harm_prim_to_cons_batch(dpt.ptr, dgt.ptr, dUt.ptr, ntr, 4.0 / 3.0, None), "p2u")
+    dqt, flt, itt = D.from_numpy(gt), D.zeros(ntr, np.int32), D.zeros(ntr, np.int32)
+    H.check(H.lib().harm_invert_batch(dUt.ptr, dgt.ptr, dqt.ptr, flt.ptr, itt.ptr, ntr, 4.0 / 3.0, 2500.0, None), "inv")
+    X, T, ok = MLP.pairs(dUt, dgt, dqt, ntr)
+    m = MLP(seed=1)
+    m.fit_norm(X, T, ok, ntr)
+    order = np.nonzero(ok.numpy())[0].astype(np.int32)
+    np.random.default_rng(0).shuffle(order)
+    dorder = D.from_numpy(order)
+    H.synchronize()
+    t0 = time.time()
+    losses = m.train(X, T, ntr, dorder, args.train_steps, lr=3e-3, lr_end=3e-5, chunk=1000)
+    H.synchronize()
+    t_train = time.time() - t0
+    fb = D.zeros(1, np.uint64)
+    ms_mlp = timed(lambda: m.invert_batch(dU, dg, dq, fl, it, n, dfallback=fb))
+    q_mlp, f_mlp, i_mlp = dq.numpy(), fl.numpy(), it.numpy()
+    nfb = int(fb.numpy()[0]) // 3
+    good = (f_plain == 0) & (f_mlp == 0)
+    agree = float(max(np.abs(q_mlp[v] - q_plain[v])[good].max() / np.abs(q_plain[v])[good].max() for v in range(5)))
+    err_plain = float(max(np.abs(q_plain[v] - p[v]).max() / np.abs(p[v]).max() for v in range(5)))
+    err_mlp = float(max(np.abs(q_mlp[v] - p[v]).max() / np.abs(p[v]).max() for v in range(5)))
+    cpu = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import _oracle
+        ns = min(n, 1 << 23)
+        threads = min(16, os.cpu_count() or 1)
+        t0 = time.time()
+        _oracle.invert_batch(_oracle.p2u_batch(p[:, :ns], geom[:, :ns]), geom[:, :ns], guess[:, :ns], nthreads=threads)
+        el = time.time() - t0
+        cpu = {"value": ns / el, "unit": "inversions/s", "cores": threads, "kind": "port",
+               "sample": f"the first {ns} zones of the same sample, the oracle's C restatement of the same solve "
+                         f"(incl. its p->U), OpenMP on {threads} host threads, {el:.2f} s"}
+    out = {
+        "metric": "inversions/s, batched conserved-to-primitive inversion (FP64 Newton), plain vs MLP-seeded",
+        "value": n / (ms_plain / 1e3), "unit": "inversions/s", "n_gpus": 1, "steps": 3, "warmup": 0,
+        "ms_per_step": ms_plain, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seed 7, log-uniform Lorentz factor / magnetisation / u over rho, random KS points)",
+        "config": {"workload": f"batched inversion microbench, {n} zones", "guess": "truth perturbed by +-20%"},
+        "plain": {"ms": ms_plain, "inversions_per_s": n / (ms_plain / 1e3), "mean_newton_iterations": float(i_plain.mean()),
+                  "failures": int((f_plain != 0).sum()), "max_rel_error_vs_truth": err_plain},
+        "mlp_seeded": {"ms": ms_mlp, "inversions_per_s": n / (ms_mlp / 1e3), "mean_newton_iterations": float(i_mlp.mean()),
+                       "failures": int((f_mlp != 0).sum()), "fallbacks": nfb, "max_rel_error_vs_truth": err_mlp,
+                       "max_rel_difference_vs_plain": agree, "train_pairs": ntr, "train_steps": args.train_steps,
+                       "train_seconds": t_train, "final_loss": losses[-1]},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--grid", default="x".join(map(str, GRID)))
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", choices=["step", "inversion"], default="step",
+                    help="step: the 256x128x64 torus RK3 step (default); inversion: the batched-inversion "
+                         "microbench, plain vs MLP-seeded")
+    ap.add_argument("--n", type=int, default=1 << 24, help="inversion microbench: zones")
+    ap.add_argument("--train-n", type=int, default=1 << 18, help="inversion microbench: training pairs")
+    ap.add_argument("--train-steps", type=int, default=20000, help="inversion microbench: Adam steps")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="before the run, one device-to-device copy of this many MiB (the reference dispatch "
                          "for scaling HBM byte counters)")
@@ -130,6 +231,9 @@ def main():
         print(f"bench.py: this build runs the whole domain on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is a design note only)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "inversion":
+        return inversion_microbench(args)
 
     import harm_harm_amd as H
     from harm_harm_amd.grid import Grid
