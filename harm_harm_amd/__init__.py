@@ -100,6 +100,13 @@ _OPTIONAL_PROTOS = [
     ("harm_get_stream", _I, [_P, ctypes.POINTER(_P)]),
     ("harm_profile", _I, [_P, _I]),
     ("harm_profile_read", _I, [_P, _P, _P]),
+    ("harm_tune", _I, [_P, _I, _P]),
+    ("harm_batch_create", _I, [_P, _I, ctypes.POINTER(_P)]),
+    ("harm_batch_destroy", _I, [_P]),
+    ("harm_batch_grid", _I, [_P, _I, ctypes.POINTER(_P)]),
+    ("harm_batch_tune", _I, [_P, _I]),
+    ("harm_batch_step", _I, [_P, _I, _I]),
+    ("harm_batch_sync", _I, [_P]),
     ("harm_mlp_create", _I, [ctypes.c_uint64, ctypes.POINTER(_P)]),
     ("harm_mlp_destroy", _I, [_P]),
     ("harm_mlp_get_params", _I, [_P, _P, _P]),

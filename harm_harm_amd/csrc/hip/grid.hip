@@ -131,8 +131,7 @@ __device__ __forceinline__ int xcd_tile(int b, int n)
 // Active-zone order for the per-zone kernels: i fastest (coalesced), then k,
 // then j.  Consecutive rows are consecutive phi planes of the same (i, j)
 // row, so the 2-D geometry and connection rows stay in L2 across phi; blocks
-// are remapped so each XCD works on a contiguous range of rows.
-__device__ __forceinline__ int zone_thread(void) { return xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x; }
+// are remapped (xcd_tile) so each XCD works on a contiguous range of rows.
 __device__ __forceinline__ void zone_ijk(const Dims& d, int t, int& i, int& j, int& k)
 {
   const int r = t / d.n1;
@@ -192,12 +191,12 @@ __device__ __forceinline__ double face_riemann(double* p_l, double* p_r, const G
 
 // Three waves per SIMD: the LDS tile allows three blocks per CU.
 template <int DIR, int RECON>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_flux(Dims d, FaceTile T, const double* __restrict__ pb,
+__device__ __forceinline__ void flux_body(Dims d, FaceTile T, const double* __restrict__ pb,
                                                  double* __restrict__ F, const double* __restrict__ gface, Params P,
-                                                 int fluxtype, int do_ct, double* __restrict__ ds)
+                                                 int fluxtype, int do_ct, double* __restrict__ ds, const int blk)
 {
   __shared__ double s_l[HARM_NPR * kCellMax], s_r[HARM_NPR * kCellMax];
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int tile = blk;
   const int st = (DIR == 1) ? 1 : (DIR == 2) ? d.ps1 : d.ps1 * d.ns2;
   const Box& b = T.b;
 
@@ -303,13 +302,21 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) 
   block_min_to(dtij, ds + DS_NDT1 + DIR - 1);
 }
 
+template <int DIR, int RECON>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_flux(Dims d, FaceTile T, const double* __restrict__ pb,
+                                                 double* __restrict__ F, const double* __restrict__ gface, Params P,
+                                                 int fluxtype, int do_ct, double* __restrict__ ds)
+{
+  flux_body<DIR, RECON>(d, T, pb, F, gface, P, fluxtype, do_ct, ds, xcd_tile(blockIdx.x, gridDim.x));
+}
+
 // Toth EMFs at corners, each multiplied by the corner gdet (CORNGDETVERSION)
-__global__ __launch_bounds__(kBlock) void k_emf(Dims d, int ni, int nj, int nk, const double* __restrict__ F1,
+__device__ __forceinline__ void emf_body(Dims d, int ni, int nj, int nk, const double* __restrict__ F1,
                                                 const double* __restrict__ F2, const double* __restrict__ F3,
                                                 const double* __restrict__ gf1, const double* __restrict__ gf2,
-                                                const double* __restrict__ gc3, double* __restrict__ emf)
+                                                const double* __restrict__ gc3, double* __restrict__ emf, const int blk)
 {
-  const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+  const long t = (long)blk * kBlock + threadIdx.x;
   const long n = (long)ni * nj * nk;
   if (t >= n) return;
   const int i = (int)(t % ni), j = (int)((t / ni) % nj), k = (int)(t / ((long)ni * nj));
@@ -340,13 +347,21 @@ __global__ __launch_bounds__(kBlock) void k_emf(Dims d, int ni, int nj, int nk, 
   emf[2 * n + t] = e * gc3[pp];
 }
 
+__global__ __launch_bounds__(kBlock) void k_emf(Dims d, int ni, int nj, int nk, const double* __restrict__ F1,
+                                                const double* __restrict__ F2, const double* __restrict__ F3,
+                                                const double* __restrict__ gf1, const double* __restrict__ gf2,
+                                                const double* __restrict__ gc3, double* __restrict__ emf)
+{
+  emf_body(d, ni, nj, nk, F1, F2, F3, gf1, gf2, gc3, emf, blockIdx.x);
+}
+
 template <int DIR>
-__global__ __launch_bounds__(kBlock) void k_ct(Dims d, int ni, int nj, int nk, const double* __restrict__ emf,
-                                               double* __restrict__ F)
+__device__ __forceinline__ void ct_body(Dims d, int ni, int nj, int nk, const double* __restrict__ emf,
+                                               double* __restrict__ F, const int blk)
 {
   const int e0 = d.n1 > 1, e1 = d.n2 > 1, e2 = d.n3 > 1;
   const int fi = d.n1 + (DIR == 1 ? e0 : 0), fj = d.n2 + (DIR == 2 ? e1 : 0), fk = d.n3 + (DIR == 3 ? e2 : 0);
-  const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+  const long t = (long)blk * kBlock + threadIdx.x;
   if (t >= (long)fi * fj * fk) return;
   const int i = (int)(t % fi), j = (int)((t / fi) % fj), k = (int)(t / ((long)fi * fj));
   const long n = (long)ni * nj * nk;
@@ -366,6 +381,13 @@ __global__ __launch_bounds__(kBlock) void k_ct(Dims d, int ni, int nj, int nk, c
     F[HARM_B2 * nc + o] = -0.5 * (E(0, i, j, k) + E(0, i, j + e1, k));
     F[HARM_B3 * nc + o] = 0.;
   }
+}
+
+template <int DIR>
+__global__ __launch_bounds__(kBlock) void k_ct(Dims d, int ni, int nj, int nk, const double* __restrict__ emf,
+                                               double* __restrict__ F)
+{
+  ct_body<DIR>(d, ni, nj, nk, emf, F, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -390,19 +412,19 @@ __global__ __launch_bounds__(kBlock) void k_ct(Dims d, int ni, int nj, int nk, c
 enum { kUpdAll = 0, kUpdFirst = 1, kUpdMiddle = 2, kUpdLast = 3 };
 
 template <int DM, int MODE>
-__global__ __launch_bounds__(kBlock) void k_update(Dims d, Params P, const double* __restrict__ pi,
+__device__ __forceinline__ void update_body(Dims d, Params P, const double* __restrict__ pi,
                                                    const double* __restrict__ pb, const double* __restrict__ F1,
                                                    const double* __restrict__ F2, const double* __restrict__ F3,
                                                    double* __restrict__ ui, double* __restrict__ uf,
                                                    double* __restrict__ ucum, const double* __restrict__ gcent,
                                                    const double* __restrict__ conn, double cuf0, double cuf1,
                                                    double cuf2, double cun0, double cun1, double cun2, int timeorder,
-                                                   double* __restrict__ ds)
+                                                   double* __restrict__ ds, const int blk)
 {
   // Work is ordered so that few values are live at once: flux divergence,
   // then everything that needs the state of pb (source, source dt), then the
   // state of pi (Ui) and the RK combination.
-  const int t = zone_thread();
+  const int t = blk * kBlock + threadIdx.x;
   const int nact = d.n1 * d.n2 * d.n3;
   double dts = kBIG;
   if (t < nact) {
@@ -514,16 +536,29 @@ __global__ __launch_bounds__(kBlock) void k_update(Dims d, Params P, const doubl
   block_min_to(dts, ds + DS_ACCDT);
 }
 
+template <int DM, int MODE>
+__global__ __launch_bounds__(kBlock) void k_update(Dims d, Params P, const double* __restrict__ pi,
+                                                   const double* __restrict__ pb, const double* __restrict__ F1,
+                                                   const double* __restrict__ F2, const double* __restrict__ F3,
+                                                   double* __restrict__ ui, double* __restrict__ uf,
+                                                   double* __restrict__ ucum, const double* __restrict__ gcent,
+                                                   const double* __restrict__ conn, double cuf0, double cuf1,
+                                                   double cuf2, double cun0, double cun1, double cun2, int timeorder,
+                                                   double* __restrict__ ds)
+{
+  update_body<DM, MODE>(d, P, pi, pb, F1, F2, F3, ui, uf, ucum, gcent, conn, cuf0, cuf1, cuf2, cun0, cun1, cun2, timeorder, ds, xcd_tile(blockIdx.x, gridDim.x));
+}
+
 // ---------------------------------------------------------------------------
 // inversion + fixup1zone (floors via ZAMO-frame mass/energy addition)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_invert(Dims d, Params P, const double* __restrict__ U,
+__device__ __forceinline__ void invert_body(Dims d, Params P, const double* __restrict__ U,
                                                    const double* __restrict__ pg, double* __restrict__ pf,
                                                    int32_t* __restrict__ pflag,
                                                    const double* __restrict__ gcent, const double* __restrict__ floors,
-                                                   int finalstep, unsigned long long* __restrict__ di)
+                                                   int finalstep, unsigned long long* __restrict__ di, const int blk)
 {
-  const int t = zone_thread();
+  const int t = blk * kBlock + threadIdx.x;
   const int nact = d.n1 * d.n2 * d.n3;
   int ns = 0, nf = 0;
   if (t < nact) {
@@ -592,14 +627,23 @@ __global__ __launch_bounds__(kBlock) void k_invert(Dims d, Params P, const doubl
   }
 }
 
+__global__ __launch_bounds__(kBlock) void k_invert(Dims d, Params P, const double* __restrict__ U,
+                                                   const double* __restrict__ pg, double* __restrict__ pf,
+                                                   int32_t* __restrict__ pflag,
+                                                   const double* __restrict__ gcent, const double* __restrict__ floors,
+                                                   int finalstep, unsigned long long* __restrict__ di)
+{
+  invert_body(d, P, U, pg, pf, pflag, gcent, floors, finalstep, di, xcd_tile(blockIdx.x, gridDim.x));
+}
+
 // ---------------------------------------------------------------------------
 // failure averaging (fixup_utoprim with the simple_average cascade)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_post_fixup(Dims d, Params P, const double* pa, double* pf,
+__device__ __forceinline__ void post_fixup_body(Dims d, Params P, const double* pa, double* pf,
                                                        const int32_t* __restrict__ pflag,
-                                                       const double* __restrict__ gcent)
+                                                       const double* __restrict__ gcent, const int blk)
 {
-  const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+  const long t = (long)blk * kBlock + threadIdx.x;
   const long nact = (long)d.n1 * d.n2 * d.n3;
   if (t >= nact) return;
   const int i = (int)(t % d.n1), j = (int)((t / d.n1) % d.n2), k = (int)(t / ((long)d.n1 * d.n2));
@@ -636,6 +680,13 @@ __global__ __launch_bounds__(kBlock) void k_post_fixup(Dims d, Params P, const d
   hd::limit_gamma(P.gammamax, pz, G);
 #pragma unroll
   for (int pl = 0; pl < HARM_NPR; pl++) pf[pl * nc + o] = pz[pl];
+}
+
+__global__ __launch_bounds__(kBlock) void k_post_fixup(Dims d, Params P, const double* pa, double* pf,
+                                                       const int32_t* __restrict__ pflag,
+                                                       const double* __restrict__ gcent)
+{
+  post_fixup_body(d, P, pa, pf, pflag, gcent, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -741,10 +792,10 @@ __device__ void extrap_line(const Dims& d, const Params& P, double* p, const dou
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_bound_x1(Dims d, Params P, double* __restrict__ p,
-                                                     const double* __restrict__ gcent)
+__device__ __forceinline__ void bound_x1_body(Dims d, Params P, double* __restrict__ p,
+                                                     const double* __restrict__ gcent, const int blk)
 {
-  const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+  const long t = (long)blk * kBlock + threadIdx.x;
   if (t >= 2L * d.n2 * d.n3) return;
   const int up = (int)(t & 1);
   const long l = t >> 1;
@@ -752,9 +803,15 @@ __global__ __launch_bounds__(kBlock) void k_bound_x1(Dims d, Params P, double* _
   extrap_line(d, P, p, gcent, up, j, k);
 }
 
-__global__ __launch_bounds__(kBlock) void k_x2_reflect(Dims d, double* __restrict__ p, int side)
+__global__ __launch_bounds__(kBlock) void k_bound_x1(Dims d, Params P, double* __restrict__ p,
+                                                     const double* __restrict__ gcent)
 {
-  const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+  bound_x1_body(d, P, p, gcent, blockIdx.x);
+}
+
+__device__ __forceinline__ void x2_reflect_body(Dims d, double* __restrict__ p, int side, const int blk)
+{
+  const long t = (long)blk * kBlock + threadIdx.x;
   const long ncol = (long)d.ns1 * d.n3;
   if (t >= ncol * d.ng2) return;
   const int g = (int)(t / ncol);
@@ -774,11 +831,16 @@ __global__ __launch_bounds__(kBlock) void k_x2_reflect(Dims d, double* __restric
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_x2_poledeath(Dims d, Params P, double* __restrict__ p,
-                                                         const double* __restrict__ gcent,
-                                                         const double* __restrict__ rcent, int side)
+__global__ __launch_bounds__(kBlock) void k_x2_reflect(Dims d, double* __restrict__ p, int side)
 {
-  const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+  x2_reflect_body(d, p, side, blockIdx.x);
+}
+
+__device__ __forceinline__ void x2_poledeath_body(Dims d, Params P, double* __restrict__ p,
+                                                         const double* __restrict__ gcent,
+                                                         const double* __restrict__ rcent, int side, const int blk)
+{
+  const long t = (long)blk * kBlock + threadIdx.x;
   const long ncol = (long)d.ns1 * d.n3;
   if (t >= ncol) return;
   const int i = (int)(t % d.ns1) - d.ng1, k = (int)(t / d.ns1);
@@ -826,9 +888,16 @@ __global__ __launch_bounds__(kBlock) void k_x2_poledeath(Dims d, Params P, doubl
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_bound_x3(Dims d, double* __restrict__ p)
+__global__ __launch_bounds__(kBlock) void k_x2_poledeath(Dims d, Params P, double* __restrict__ p,
+                                                         const double* __restrict__ gcent,
+                                                         const double* __restrict__ rcent, int side)
 {
-  const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+  x2_poledeath_body(d, P, p, gcent, rcent, side, blockIdx.x);
+}
+
+__device__ __forceinline__ void bound_x3_body(Dims d, double* __restrict__ p, const int blk)
+{
+  const long t = (long)blk * kBlock + threadIdx.x;
   const long nplane = d.nplane;
   if (t >= nplane * 2 * d.ng3) return;
   const long pp = t % nplane;
@@ -839,6 +908,11 @@ __global__ __launch_bounds__(kBlock) void k_bound_x3(Dims d, double* __restrict_
   const long o = (long)(k + d.ng3) * nplane + pp, so = (long)(ks + d.ng3) * nplane + pp;
 #pragma unroll
   for (int pl = 0; pl < HARM_NPR; pl++) p[pl * nc + o] = p[pl * nc + so];
+}
+
+__global__ __launch_bounds__(kBlock) void k_bound_x3(Dims d, double* __restrict__ p)
+{
+  bound_x3_body(d, p, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -878,13 +952,13 @@ __global__ void k_scal_end_step(double* ds, double safe)
 // adds the partial sums in block order.
 constexpr int kFaccBlocks = 64;
 
-__global__ __launch_bounds__(kBlock) void k_fluxacc_part(Dims d, const double* __restrict__ F1,
-                                                         double* __restrict__ part)
+__device__ __forceinline__ void fluxacc_part_body(Dims d, const double* __restrict__ F1,
+                                                         double* __restrict__ part, const int blk)
 {
   __shared__ double red[4][kBlock];
   double s[4] = {0, 0, 0, 0};
   const int nf = d.n2 * d.n3, chunk = (nf + kFaccBlocks - 1) / kFaccBlocks;
-  const int t0 = blockIdx.x * chunk, t1 = min(nf, t0 + chunk);
+  const int t0 = blk * chunk, t1 = min(nf, t0 + chunk);
   const long nc = d.ncell;
   for (int t = t0 + threadIdx.x; t < t1; t += kBlock) {
     const int k = t / d.n2, j = t - k * d.n2;
@@ -901,7 +975,13 @@ __global__ __launch_bounds__(kBlock) void k_fluxacc_part(Dims d, const double* _
       for (int c = 0; c < 4; c++) red[c][threadIdx.x] += red[c][threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x < 4) part[blockIdx.x * 4 + threadIdx.x] = red[threadIdx.x][0];
+  if (threadIdx.x < 4) part[blk * 4 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(kBlock) void k_fluxacc_part(Dims d, const double* __restrict__ F1,
+                                                         double* __restrict__ part)
+{
+  fluxacc_part_body(d, F1, part, blockIdx.x);
 }
 
 __global__ void k_fluxacc_final(Params P, const double* __restrict__ part, double* __restrict__ ds, double weight)
@@ -934,6 +1014,8 @@ struct harm_handle {
   double* rcent = nullptr;
   double* emf = nullptr;
   double* facc = nullptr;  // boundary-flux partial sums
+  int tile_td[4] = {0, 0, 0, 0};  // sweep depth of the face tiles of dirs 2,3 (0: cost model)
+  FaceTile ft[4];                 // planned face tiles per direction (index 1..3)
   double* ds = nullptr;
   unsigned long long* di = nullptr;
   int emf_ni = 0, emf_nj = 0, emf_nk = 0;
@@ -1100,7 +1182,7 @@ harm_status check_which(int which)
 // (tc = 256/td) minimising blocks x (recon passes x 1 + face pass x 12), the
 // rough cost ratio of one (cell, variable) reconstruction to one face solve;
 // the i extent is covered by ceil(ni/tc) tiles.
-FaceTile plan_face_tile(const Dims& d, int dir)
+FaceTile plan_face_tile(const Dims& d, int dir, int td_force = 0)
 {
   FaceTile T{};
   T.b = face_box(d, dir);
@@ -1122,6 +1204,7 @@ FaceTile plan_face_tile(const Dims& d, int dir)
   for (int td : {4, 8, 16}) {
     const int tc = kBlock / td;
     if ((td + 1) * tc > kCellMax) continue;
+    if (td_force && td != td_force) continue;
     const long nti = (iext + tc - 1) / tc, ntd = (T.nd + td - 1) / td;
     const int passes = ((td + 1) * tc * HARM_NPR + kBlock - 1) / kBlock;
     const double cost = (double)(nti * ntd * T.nplane) * (passes + 12.0);
@@ -1146,7 +1229,7 @@ harm_status enq_fluxcalc(harm_handle* h, int which_p, int do_ct)
   const int recon = h->cfg.recon, ft = h->cfg.flux;
 #define LAUNCH_FLUX(DIR, F, G)                                                                                     \
   do {                                                                                                             \
-    const FaceTile T = plan_face_tile(d, DIR);                                                                     \
+    const FaceTile& T = h->ft[DIR];                                                                                \
     const dim3 grid((unsigned)T.nblk), blk(kBlock);                                                                \
     if (recon == HARM_RECON_WENO5)                                                                                 \
       hipLaunchKernelGGL((k_flux<DIR, HARM_RECON_WENO5>), grid, blk, 0, s, d, T, pb, F, G, h->P, ft, do_ct, h->ds); \
@@ -1326,6 +1409,347 @@ void rk_flux_weights(int timeorder, double* w)
   for (int r = 0; r < n; r++) w[r] = C[n - 1][r];
 }
 
+// ---------------------------------------------------------------------------
+// Batches of independent grids stepped together.
+//
+// Every stage of the step (face fluxes per direction, EMFs, CT, update,
+// inversion, each boundary pass, failure averaging, the dt scalars) is one
+// launch over all grids of the batch: a block finds its grid by binary search
+// in the stage's prefix sums of per-grid block counts and runs the same
+// kernel body as the single-grid launch, on that grid's arrays.  Grids of
+// different sizes, spins, field strengths and seeds share the launches, so a
+// batch of small 2-D problems fills the GPU that one of them alone could not.
+// A whole RK step is captured once in a hipGraph (kernel nodes only) and
+// replayed.
+// ---------------------------------------------------------------------------
+struct GridRef {
+  Dims d;
+  Params P;
+  int fluxtype;
+  double* arr[HARM_NARR];
+  int32_t* pflag;
+  const double* geom[HARM_NLOC];
+  const double* gcorn3;
+  const double* conn;
+  const double* floors;
+  const double* rcent;
+  double* emf;
+  int emf_ni, emf_nj, emf_nk;
+  double* ds;
+  unsigned long long* di;
+  double* facc;
+  FaceTile ft[4];
+};
+
+enum { BS_FLUX1 = 0, BS_FLUX2, BS_FLUX3, BS_EMF, BS_CT1, BS_CT2, BS_CT3, BS_FACC, BS_ZONE, BS_BX1, BS_X2R, BS_X2P,
+       BS_BX3, BS_ONE, BS_N };
+
+__device__ __forceinline__ void batch_decode(const int* __restrict__ pre, int ng, int b, int& gi, int& lb)
+{
+  int lo = 0, hi = ng - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (pre[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  gi = lo;
+  lb = b - pre[lo];
+}
+
+#define BATCH_DECODE(REMAP)                                                        \
+  int gi, lb;                                                                      \
+  batch_decode(pre, ng, (REMAP) ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x, gi, lb); \
+  const GridRef& g = G[gi];
+
+template <int DIR, int RECON>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) void kb_flux(
+    const GridRef* __restrict__ G, const int* __restrict__ pre, int ng, int which, int do_ct)
+{
+  BATCH_DECODE(true)
+  flux_body<DIR, RECON>(g.d, g.ft[DIR], g.arr[which], g.arr[HARM_ARR_F1 + DIR - 1], g.geom[DIR], g.P, g.fluxtype,
+                        do_ct, g.ds, lb);
+}
+
+__global__ __launch_bounds__(kBlock) void kb_emf(const GridRef* __restrict__ G, const int* __restrict__ pre, int ng)
+{
+  BATCH_DECODE(false)
+  emf_body(g.d, g.emf_ni, g.emf_nj, g.emf_nk, g.arr[HARM_ARR_F1], g.arr[HARM_ARR_F2], g.arr[HARM_ARR_F3],
+           g.geom[HARM_LOC_FACE1], g.geom[HARM_LOC_FACE2], g.gcorn3, g.emf, lb);
+}
+
+template <int DIR>
+__global__ __launch_bounds__(kBlock) void kb_ct(const GridRef* __restrict__ G, const int* __restrict__ pre, int ng)
+{
+  BATCH_DECODE(false)
+  ct_body<DIR>(g.d, g.emf_ni, g.emf_nj, g.emf_nk, g.emf, g.arr[HARM_ARR_F1 + DIR - 1], lb);
+}
+
+__global__ __launch_bounds__(kBlock) void kb_fluxacc_part(const GridRef* __restrict__ G, const int* __restrict__ pre,
+                                                          int ng)
+{
+  BATCH_DECODE(false)
+  fluxacc_part_body(g.d, g.arr[HARM_ARR_F1], g.facc, lb);
+}
+
+__global__ void kb_fluxacc_final(const GridRef* __restrict__ G, double weight)
+{
+  const GridRef& g = G[blockIdx.x];
+  const int c = threadIdx.x;
+  if (c >= 4 || g.d.n1 < 2) return;
+  double v = 0.0;
+  for (int b = 0; b < kFaccBlocks; b++) v += g.facc[b * 4 + c];
+  g.ds[DS_FLUXACC + c] += v * g.P.dx[2] * g.P.dx[3] * g.ds[DS_DT] * weight;
+}
+
+template <int DM, int MODE>
+__global__ __launch_bounds__(kBlock) void kb_update(const GridRef* __restrict__ G, const int* __restrict__ pre, int ng,
+                                                    int which_pi, int which_pb, double cuf0, double cuf1, double cuf2,
+                                                    double cun0, double cun1, double cun2, int timeorder)
+{
+  BATCH_DECODE(true)
+  update_body<DM, MODE>(g.d, g.P, g.arr[which_pi], g.arr[which_pb], g.arr[HARM_ARR_F1], g.arr[HARM_ARR_F2],
+                        g.arr[HARM_ARR_F3], g.arr[HARM_ARR_UI], g.arr[HARM_ARR_UF], g.arr[HARM_ARR_UCUM],
+                        g.geom[HARM_LOC_CENT], g.conn, cuf0, cuf1, cuf2, cun0, cun1, cun2, timeorder, g.ds, lb);
+}
+
+__global__ __launch_bounds__(kBlock) void kb_invert(const GridRef* __restrict__ G, const int* __restrict__ pre, int ng,
+                                                    int which_pb, int which_pf, int finalstep)
+{
+  BATCH_DECODE(true)
+  invert_body(g.d, g.P, g.arr[finalstep ? HARM_ARR_UCUM : HARM_ARR_UF], g.arr[which_pb], g.arr[which_pf], g.pflag,
+              g.geom[HARM_LOC_CENT], g.floors, finalstep, g.di, lb);
+}
+
+__global__ __launch_bounds__(kBlock) void kb_post_fixup(const GridRef* __restrict__ G, const int* __restrict__ pre,
+                                                        int ng, int which)
+{
+  BATCH_DECODE(false)
+  post_fixup_body(g.d, g.P, g.arr[which], g.arr[which], g.pflag, g.geom[HARM_LOC_CENT], lb);
+}
+
+__global__ __launch_bounds__(kBlock) void kb_bound_x1(const GridRef* __restrict__ G, const int* __restrict__ pre,
+                                                      int ng, int which)
+{
+  BATCH_DECODE(false)
+  bound_x1_body(g.d, g.P, g.arr[which], g.geom[HARM_LOC_CENT], lb);
+}
+
+__global__ __launch_bounds__(kBlock) void kb_x2_reflect(const GridRef* __restrict__ G, const int* __restrict__ pre,
+                                                        int ng, int which, int side)
+{
+  BATCH_DECODE(false)
+  x2_reflect_body(g.d, g.arr[which], side, lb);
+}
+
+__global__ __launch_bounds__(kBlock) void kb_x2_poledeath(const GridRef* __restrict__ G, const int* __restrict__ pre,
+                                                          int ng, int which, int side)
+{
+  BATCH_DECODE(false)
+  x2_poledeath_body(g.d, g.P, g.arr[which], g.geom[HARM_LOC_CENT], g.rcent, side, lb);
+}
+
+__global__ __launch_bounds__(kBlock) void kb_bound_x3(const GridRef* __restrict__ G, const int* __restrict__ pre,
+                                                      int ng, int which)
+{
+  BATCH_DECODE(false)
+  bound_x3_body(g.d, g.arr[which], lb);
+}
+
+// one thread per grid
+__global__ void kb_scal(const GridRef* __restrict__ G, int ng, int what)
+{
+  const int gi = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gi >= ng) return;
+  double* ds = G[gi].ds;
+  if (what == 0) {  // begin flux
+    ds[DS_NDT1] = kBIG; ds[DS_NDT2] = kBIG; ds[DS_NDT3] = kBIG; ds[DS_ACCDT] = kBIG;
+  } else if (what == 1) {  // begin step
+    ds[DS_LASTNDT] = kBIG;
+    G[gi].di[DI_NSTROKE] = 0;
+    G[gi].di[DI_NFAIL] = 0;
+  } else if (what == 2) {  // substep dt
+    const double wavedt = 1. / (1. / ds[DS_NDT1] + 1. / ds[DS_NDT2] + 1. / ds[DS_NDT3]);
+    const double nd = 1.0 * fmin(wavedt, ds[DS_ACCDT]);
+    if (nd < ds[DS_LASTNDT]) ds[DS_LASTNDT] = nd;
+  } else {  // end step
+    const double safe = G[gi].P.safe;
+    double fullndt = ds[DS_LASTNDT];
+    const double dt = ds[DS_DT];
+    if (fullndt > safe * dt) fullndt = safe * dt;
+    ds[DS_T] = ds[DS_T] + dt;
+    ds[DS_DT] = fullndt;
+  }
+}
+
+GridRef make_ref(const harm_handle* h)
+{
+  GridRef r{};
+  r.d = h->d;
+  r.P = h->P;
+  r.fluxtype = h->cfg.flux;
+  for (int a = 0; a < HARM_NARR; a++) r.arr[a] = h->arr[a];
+  r.pflag = h->pflag;
+  for (int l = 0; l < HARM_NLOC; l++) r.geom[l] = h->geom[l];
+  r.gcorn3 = h->gcorn3;
+  r.conn = h->conn;
+  r.floors = h->floors;
+  r.rcent = h->rcent;
+  r.emf = h->emf;
+  r.emf_ni = h->emf_ni;
+  r.emf_nj = h->emf_nj;
+  r.emf_nk = h->emf_nk;
+  r.ds = h->ds;
+  r.di = h->di;
+  r.facc = h->facc;
+  for (int dir = 0; dir < 4; dir++) r.ft[dir] = h->ft[dir];
+  return r;
+}
+
+}  // namespace
+
+struct harm_batch {
+  std::vector<harm_handle*> grids;
+  hipStream_t stream = nullptr;
+  GridRef* dref = nullptr;
+  int* dpre = nullptr;  // BS_N prefix arrays of ng+1 ints
+  int ntot[BS_N] = {0};
+  int recon = 0, timeorder = 0, dm = 0;
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+};
+
+namespace {
+
+harm_status batch_build(harm_batch* b)
+{
+  const int ng = (int)b->grids.size();
+  std::vector<GridRef> refs(ng);
+  std::vector<int> pre((size_t)BS_N * (ng + 1), 0);
+  for (int gi = 0; gi < ng; gi++) {
+    const harm_handle* h = b->grids[gi];
+    const Dims& d = h->d;
+    refs[gi] = make_ref(h);
+    long cnt[BS_N] = {0};
+    const int e0 = d.n1 > 1, e1 = d.n2 > 1, e2 = d.n3 > 1;
+    cnt[BS_FLUX1] = e0 ? h->ft[1].nblk : 0;
+    cnt[BS_FLUX2] = e1 ? h->ft[2].nblk : 0;
+    cnt[BS_FLUX3] = e2 ? h->ft[3].nblk : 0;
+    cnt[BS_EMF] = nblocks((long)h->emf_ni * h->emf_nj * h->emf_nk);
+    cnt[BS_CT1] = e0 ? nblocks((long)(d.n1 + 1) * d.n2 * d.n3) : 0;
+    cnt[BS_CT2] = e1 ? nblocks((long)d.n1 * (d.n2 + 1) * d.n3) : 0;
+    cnt[BS_CT3] = e2 ? nblocks((long)d.n1 * d.n2 * (d.n3 + 1)) : 0;
+    cnt[BS_FACC] = e0 ? kFaccBlocks : 0;
+    cnt[BS_ZONE] = nblocks((long)d.n1 * d.n2 * d.n3);
+    cnt[BS_BX1] = e0 ? nblocks(2L * d.n2 * d.n3) : 0;
+    cnt[BS_X2R] = e1 ? nblocks((long)d.ns1 * d.n3 * d.ng2) : 0;
+    cnt[BS_X2P] = e1 ? nblocks((long)d.ns1 * d.n3) : 0;
+    cnt[BS_BX3] = e2 ? nblocks(d.nplane * 2L * d.ng3) : 0;
+    cnt[BS_ONE] = 1;
+    for (int st = 0; st < BS_N; st++) {
+      const long nx = (long)pre[(size_t)st * (ng + 1) + gi] + cnt[st];
+      if (nx > 0x7fffffffL) return hh::set_error(HARM_EINVAL, "harm_batch: too many blocks in one stage");
+      pre[(size_t)st * (ng + 1) + gi + 1] = (int)nx;
+    }
+  }
+  for (int st = 0; st < BS_N; st++) b->ntot[st] = pre[(size_t)st * (ng + 1) + ng];
+  if (!b->dref) HH_HIP(hipMalloc((void**)&b->dref, sizeof(GridRef) * ng));
+  if (!b->dpre) HH_HIP(hipMalloc((void**)&b->dpre, sizeof(int) * pre.size()));
+  HH_HIP(hipMemcpy(b->dref, refs.data(), sizeof(GridRef) * ng, hipMemcpyHostToDevice));
+  HH_HIP(hipMemcpy(b->dpre, pre.data(), sizeof(int) * pre.size(), hipMemcpyHostToDevice));
+  return HARM_OK;
+}
+
+harm_status enq_batch_substep(harm_batch* b, int to, int nto, const double CUf[4][4], const double Cunew[4][4],
+                              const double* wflux)
+{
+  const int ng = (int)b->grids.size();
+  hipStream_t s = b->stream;
+  const GridRef* G = b->dref;
+  auto P = [&](int st) { return b->dpre + (size_t)st * (ng + 1); };
+  const dim3 blk(kBlock);
+  const int pii[3] = {HARM_ARR_P, HARM_ARR_P, HARM_ARR_P};
+  int pbb[3] = {HARM_ARR_P, HARM_ARR_PK0, HARM_ARR_PK1};
+  int pff[3] = {HARM_ARR_PK0, HARM_ARR_PK1, HARM_ARR_P};
+  if (nto == 2) { pbb[1] = HARM_ARR_PK0; pff[1] = HARM_ARR_P; }
+  const int finalstep = (to == nto - 1);
+  const unsigned g1 = (unsigned)((ng + 63) / 64);
+  hipLaunchKernelGGL(kb_scal, dim3(g1), dim3(64), 0, s, G, ng, 0);
+  // fluxes
+#define BFLUX(DIR)                                                                                                  \
+  if (b->ntot[BS_FLUX1 + DIR - 1] > 0) {                                                                            \
+    const dim3 gr((unsigned)b->ntot[BS_FLUX1 + DIR - 1]);                                                           \
+    if (b->recon == HARM_RECON_WENO5)                                                                               \
+      hipLaunchKernelGGL((kb_flux<DIR, HARM_RECON_WENO5>), gr, blk, 0, s, G, P(BS_FLUX1 + DIR - 1), ng, pbb[to], 1); \
+    else if (b->recon == HARM_RECON_PARA)                                                                           \
+      hipLaunchKernelGGL((kb_flux<DIR, HARM_RECON_PARA>), gr, blk, 0, s, G, P(BS_FLUX1 + DIR - 1), ng, pbb[to], 1);  \
+    else                                                                                                            \
+      hipLaunchKernelGGL((kb_flux<DIR, HARM_RECON_DONOR>), gr, blk, 0, s, G, P(BS_FLUX1 + DIR - 1), ng, pbb[to], 1); \
+  }
+  BFLUX(1)
+  BFLUX(2)
+  BFLUX(3)
+#undef BFLUX
+  hipLaunchKernelGGL(kb_emf, dim3((unsigned)b->ntot[BS_EMF]), blk, 0, s, G, P(BS_EMF), ng);
+  if (b->ntot[BS_CT1]) hipLaunchKernelGGL(kb_ct<1>, dim3((unsigned)b->ntot[BS_CT1]), blk, 0, s, G, P(BS_CT1), ng);
+  if (b->ntot[BS_CT2]) hipLaunchKernelGGL(kb_ct<2>, dim3((unsigned)b->ntot[BS_CT2]), blk, 0, s, G, P(BS_CT2), ng);
+  if (b->ntot[BS_CT3]) hipLaunchKernelGGL(kb_ct<3>, dim3((unsigned)b->ntot[BS_CT3]), blk, 0, s, G, P(BS_CT3), ng);
+  if (b->ntot[BS_FACC]) {
+    hipLaunchKernelGGL(kb_fluxacc_part, dim3((unsigned)b->ntot[BS_FACC]), blk, 0, s, G, P(BS_FACC), ng);
+    hipLaunchKernelGGL(kb_fluxacc_final, dim3((unsigned)ng), dim3(64), 0, s, G, wflux[to]);
+  }
+  // update (the reduced-traffic modes: the RK2/RK3 tables keep ucum at zero
+  // until the last substep)
+  bool zero_ucum = true;
+  for (int st = 0; st + 1 < nto; st++)
+    zero_ucum = zero_ucum && Cunew[st][0] == 0.0 && Cunew[st][1] == 0.0 && Cunew[st][2] == 0.0;
+  const int mode = !zero_ucum ? kUpdAll : to == 0 ? kUpdFirst : finalstep ? kUpdLast : kUpdMiddle;
+  const dim3 gz((unsigned)b->ntot[BS_ZONE]);
+#define BUPD(DM, MODE)                                                                                            \
+  hipLaunchKernelGGL((kb_update<DM, MODE>), gz, blk, 0, s, G, P(BS_ZONE), ng, pii[to], pbb[to], CUf[to][0],      \
+                     CUf[to][1], CUf[to][2], Cunew[to][0], Cunew[to][1], Cunew[to][2], to)
+#define BUPD_MODES(DM)                                   \
+  switch (mode) {                                        \
+    case kUpdFirst: BUPD(DM, kUpdFirst); break;          \
+    case kUpdMiddle: BUPD(DM, kUpdMiddle); break;        \
+    case kUpdLast: BUPD(DM, kUpdLast); break;            \
+    default: BUPD(DM, kUpdAll); break;                   \
+  }
+  if (b->dm == 7) { BUPD_MODES(7); }
+  else if (b->dm == 3) { BUPD_MODES(3); }
+  else { BUPD_MODES(-1); }
+#undef BUPD_MODES
+#undef BUPD
+  hipLaunchKernelGGL(kb_invert, gz, blk, 0, s, G, P(BS_ZONE), ng, pbb[to], pff[to], finalstep);
+  hipLaunchKernelGGL(kb_scal, dim3(g1), dim3(64), 0, s, G, ng, 2);
+  // bounds, failure averaging, bounds
+  for (int pass = 0; pass < 2; pass++) {
+    if (pass == 1) hipLaunchKernelGGL(kb_post_fixup, gz, blk, 0, s, G, P(BS_ZONE), ng, pff[to]);
+    if (b->ntot[BS_BX1]) hipLaunchKernelGGL(kb_bound_x1, dim3((unsigned)b->ntot[BS_BX1]), blk, 0, s, G, P(BS_BX1), ng,
+                                            pff[to]);
+    if (b->ntot[BS_X2R])
+      for (int side = 0; side < 2; side++) {
+        hipLaunchKernelGGL(kb_x2_reflect, dim3((unsigned)b->ntot[BS_X2R]), blk, 0, s, G, P(BS_X2R), ng, pff[to], side);
+        hipLaunchKernelGGL(kb_x2_poledeath, dim3((unsigned)b->ntot[BS_X2P]), blk, 0, s, G, P(BS_X2P), ng, pff[to],
+                           side);
+      }
+    if (b->ntot[BS_BX3]) hipLaunchKernelGGL(kb_bound_x3, dim3((unsigned)b->ntot[BS_BX3]), blk, 0, s, G, P(BS_BX3), ng,
+                                            pff[to]);
+  }
+  return hh::check_launch("batch substep");
+}
+
+harm_status enq_batch_step(harm_batch* b)
+{
+  const int ng = (int)b->grids.size();
+  double CUf[4][4], Cunew[4][4], w[4];
+  const int nto = rk_tables(b->timeorder, CUf, Cunew);
+  rk_flux_weights(b->timeorder, w);
+  hipLaunchKernelGGL(kb_scal, dim3((unsigned)((ng + 63) / 64)), dim3(64), 0, b->stream, b->dref, ng, 1);
+  for (int to = 0; to < nto; to++) HH_TRY(enq_batch_substep(b, to, nto, CUf, Cunew, w));
+  hipLaunchKernelGGL(kb_scal, dim3((unsigned)((ng + 63) / 64)), dim3(64), 0, b->stream, b->dref, ng, 3);
+  return hh::check_launch("batch step");
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -1372,6 +1796,7 @@ extern "C" harm_status harm_create(const harm_config* cfg, harm_handle** out)
   TRYC(alloc((void**)&h->di, sizeof(unsigned long long) * DI_N));
   TRYC(hh::check_hip(hipMemset(h->di, 0, sizeof(unsigned long long) * DI_N), "hipMemset"));
   TRYC(build_geometry(h));
+  for (int dir = 1; dir <= 3; dir++) h->ft[dir] = plan_face_tile(d, dir);
 #undef TRYC
   *out = h;
   return HARM_OK;
@@ -1717,4 +2142,164 @@ extern "C" harm_status harm_profile_read(harm_handle* h, double* ms_total, int64
     h->prof_n[k] = 0;
   }
   return HARM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Tile autotuning and batches
+// ---------------------------------------------------------------------------
+extern "C" harm_status harm_tune(harm_handle* h, int32_t reps, int32_t td_out[4])
+{
+  HH_TRY(check_handle(h));
+  if (reps < 1) reps = 1;
+  const Dims& d = h->d;
+  hipEvent_t e0, e1;
+  HH_HIP(hipEventCreate(&e0));
+  HH_HIP(hipEventCreate(&e1));
+  const int recon = h->cfg.recon, ft = h->cfg.flux;
+  hipStream_t s = h->stream;
+  const double* pb = h->arr[HARM_ARR_P];
+  harm_status st = HARM_OK;
+  for (int dir = 2; dir <= 3 && st == HARM_OK; dir++) {
+    if ((dir == 2 && d.n2 < 2) || (dir == 3 && d.n3 < 2)) continue;
+    double best = 1e300;
+    int best_td = 0;
+    for (int td : {4, 8, 16}) {
+      if ((td + 1) * (kBlock / td) > kCellMax) continue;
+      const FaceTile T = plan_face_tile(d, dir, td);
+      double* F = h->arr[HARM_ARR_F1 + dir - 1];
+      const double* G = h->geom[dir];
+      auto launch = [&]() {
+        const dim3 grid((unsigned)T.nblk), blk(kBlock);
+#define TUNE_LAUNCH(DIR)                                                                                           \
+  if (recon == HARM_RECON_WENO5)                                                                                   \
+    hipLaunchKernelGGL((k_flux<DIR, HARM_RECON_WENO5>), grid, blk, 0, s, d, T, pb, F, G, h->P, ft, 1, h->ds);       \
+  else if (recon == HARM_RECON_PARA)                                                                               \
+    hipLaunchKernelGGL((k_flux<DIR, HARM_RECON_PARA>), grid, blk, 0, s, d, T, pb, F, G, h->P, ft, 1, h->ds);        \
+  else                                                                                                             \
+    hipLaunchKernelGGL((k_flux<DIR, HARM_RECON_DONOR>), grid, blk, 0, s, d, T, pb, F, G, h->P, ft, 1, h->ds);
+        if (dir == 2) { TUNE_LAUNCH(2) } else { TUNE_LAUNCH(3) }
+#undef TUNE_LAUNCH
+      };
+      launch();  // warm
+      (void)hipEventRecord(e0, s);
+      for (int r = 0; r < reps; r++) launch();
+      (void)hipEventRecord(e1, s);
+      if ((st = hh::check_launch("harm_tune")) != HARM_OK) break;
+      if ((st = hh::check_hip(hipEventSynchronize(e1), "harm_tune")) != HARM_OK) break;
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      if (ms < best) {
+        best = ms;
+        best_td = td;
+      }
+    }
+    if (st == HARM_OK && best_td) {
+      h->tile_td[dir] = best_td;
+      h->ft[dir] = plan_face_tile(d, dir, best_td);
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (td_out)
+    for (int dir = 0; dir < 4; dir++) td_out[dir] = dir >= 2 ? h->ft[dir].td : 0;
+  return st;
+}
+
+extern "C" harm_status harm_batch_destroy(harm_batch* b)
+{
+  if (!b) return HARM_OK;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  for (harm_handle* h : b->grids) {
+    if (!h) continue;
+    h->stream = nullptr;  // the batch's stream, destroyed below
+    harm_destroy(h);
+  }
+  if (b->dref) (void)hipFree(b->dref);
+  if (b->dpre) (void)hipFree(b->dpre);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return HARM_OK;
+}
+
+extern "C" harm_status harm_batch_create(const harm_config* cfgs, int32_t n, harm_batch** out)
+{
+  if (!cfgs || !out || n < 1) return hh::set_error(HARM_EINVAL, "harm_batch_create: bad arguments");
+  const harm_config& c0 = cfgs[0];
+  const int dm0 = (c0.n1 > 1) | ((c0.n2 > 1) << 1) | ((c0.n3 > 1) << 2);
+  for (int i = 0; i < n; i++) {
+    const harm_config& c = cfgs[i];
+    const int dm = (c.n1 > 1) | ((c.n2 > 1) << 1) | ((c.n3 > 1) << 2);
+    if (c.recon != c0.recon || c.flux != c0.flux || c.timeorder != c0.timeorder || dm != dm0)
+      return hh::set_error(HARM_EINVAL,
+                           "harm_batch_create: grid %d differs in reconstruction, flux, time order or dimensionality", i);
+  }
+  harm_batch* b = new harm_batch();
+  b->recon = c0.recon;
+  b->timeorder = c0.timeorder;
+  b->dm = dm0;
+  harm_status st = hh::check_hip(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
+  for (int i = 0; i < n && st == HARM_OK; i++) {
+    harm_handle* h = nullptr;
+    st = harm_create(&cfgs[i], &h);
+    if (st != HARM_OK) break;
+    (void)hipStreamDestroy(h->stream);
+    h->stream = b->stream;
+    b->grids.push_back(h);
+  }
+  if (st == HARM_OK) st = batch_build(b);
+  if (st != HARM_OK) {
+    harm_batch_destroy(b);
+    return st;
+  }
+  *out = b;
+  return HARM_OK;
+}
+
+extern "C" harm_status harm_batch_grid(harm_batch* b, int32_t i, harm_handle** h)
+{
+  if (!b || !h || i < 0 || i >= (int32_t)b->grids.size()) return hh::set_error(HARM_EINVAL, "harm_batch_grid: bad index");
+  *h = b->grids[i];
+  return HARM_OK;
+}
+
+extern "C" harm_status harm_batch_tune(harm_batch* b, int32_t reps)
+{
+  if (!b) return hh::set_error(HARM_EINVAL, "harm_batch_tune: null batch");
+  for (harm_handle* h : b->grids) HH_TRY(harm_tune(h, reps, nullptr));
+  HH_HIP(hipStreamSynchronize(b->stream));
+  if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
+  if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+  return batch_build(b);
+}
+
+extern "C" harm_status harm_batch_step(harm_batch* b, int32_t nsteps, int32_t use_graph)
+{
+  if (!b || nsteps < 0) return hh::set_error(HARM_EINVAL, "harm_batch_step: bad arguments");
+  if (!use_graph) {
+    for (int n = 0; n < nsteps; n++) HH_TRY(enq_batch_step(b));
+    return hh::check_launch("harm_batch_step");
+  }
+  if (!b->exec) {
+    HH_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    const harm_status st = enq_batch_step(b);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (st != HARM_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return st;
+    }
+    HH_HIP(e);
+    b->graph = g;
+    HH_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  }
+  for (int n = 0; n < nsteps; n++) HH_HIP(hipGraphLaunch(b->exec, b->stream));
+  return HARM_OK;
+}
+
+extern "C" harm_status harm_batch_sync(harm_batch* b)
+{
+  if (!b) return hh::set_error(HARM_EINVAL, "harm_batch_sync: null batch");
+  return hh::check_hip(hipStreamSynchronize(b->stream), "harm_batch_sync");
 }

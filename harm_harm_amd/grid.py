@@ -12,10 +12,16 @@ from . import lib, check, HarmConfig, NewtonStats, ARR, G_NFIELD, NCONN, default
 
 
 class Grid:
-    def __init__(self, cfg: HarmConfig):
+    def __init__(self, cfg: HarmConfig, handle=None):
+        """A grid on one handle; `handle` wraps a handle owned elsewhere (a
+        batch member), which close() then leaves alone."""
         self.cfg = cfg
-        h = ctypes.c_void_p()
-        check(lib().harm_create(ctypes.byref(cfg), ctypes.byref(h)), "harm_create")
+        self.owned = handle is None
+        if handle is None:
+            h = ctypes.c_void_p()
+            check(lib().harm_create(ctypes.byref(cfg), ctypes.byref(h)), "harm_create")
+        else:
+            h = handle
         self.h = h
         dims = (ctypes.c_int32 * 3)()
         check(lib().harm_storage_dims(self.h, dims), "harm_storage_dims")
@@ -145,9 +151,15 @@ class Grid:
         check(lib().harm_profile_read(self.h, ms.ctypes.data, n.ctypes.data), "harm_profile_read")
         return {k: (ms[i], int(n[i])) for i, k in enumerate(self.KERNELS)}
 
+    def tune(self, reps=5):
+        td = (ctypes.c_int32 * 4)()
+        check(lib().harm_tune(self.h, reps, td), "harm_tune")
+        return list(td)
+
     def close(self):
         if getattr(self, "h", None):
-            lib().harm_destroy(self.h)
+            if self.owned:
+                lib().harm_destroy(self.h)
             self.h = None
 
     def __del__(self):

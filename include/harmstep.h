@@ -253,6 +253,36 @@ void harm_geom_mks_record(const harm_config* cfg, const double X[4], double* rec
 void harm_geom_ks_batch(int64_t n, const double* a, const double* r, const double* th, double* rec);
 
 /* ------------------------------------------------------------------------ */
+/* Tile tuning and batches of independent grids (no counterpart in the       */
+/* reference, which steps one grid per process)                              */
+/* ------------------------------------------------------------------------ */
+/* Times the face-flux kernels of directions 2 and 3 with each admissible tile
+ * depth (4, 8, 16 faces along the sweep) on the current primitives and keeps
+ * the fastest for this handle; reps launches per candidate.  td_out[dir]
+ * (may be NULL) gets the chosen depths.  Overwrites F2/F3 and the wave dt
+ * slots, which every flux calculation rewrites anyway. */
+harm_status harm_tune(harm_handle* h, int32_t reps, int32_t td_out[4]);
+
+/* n independent grids that share one stream and are stepped together: every
+ * stage of the step is one launch over all grids.  All grids must share the
+ * reconstruction, flux, time order and dimensionality; sizes, spin, floors,
+ * Courant factor and initial data may differ.  Each grid keeps its own time
+ * step. */
+typedef struct harm_batch harm_batch;
+harm_status harm_batch_create(const harm_config* cfgs, int32_t n, harm_batch** out);
+harm_status harm_batch_destroy(harm_batch* b);
+/* The handle of grid i (owned by the batch): every single-grid entry point
+ * (set/get arrays, init, time, the hot-path pieces) works on it. */
+harm_status harm_batch_grid(harm_batch* b, int32_t i, harm_handle** h);
+/* harm_tune on every grid, then rebuild the batch's launch tables. */
+harm_status harm_batch_tune(harm_batch* b, int32_t reps);
+/* nsteps full RK steps of every grid (harm_step's sequence, batched).  With
+ * use_graph != 0 one step is captured in a hipGraph on first use and
+ * replayed. */
+harm_status harm_batch_step(harm_batch* b, int32_t nsteps, int32_t use_graph);
+harm_status harm_batch_sync(harm_batch* b);
+
+/* ------------------------------------------------------------------------ */
 /* Learned initial guess for the inversion (no counterpart in the reference:  */
 /* HARM starts every Newton solve from the previous primitives).             */
 /* ------------------------------------------------------------------------ */
