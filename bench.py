@@ -209,6 +209,102 @@ def inversion_microbench(args):
     print(json.dumps(out))
 
 
+def batch_bench(args):
+    """64 independent 2-D Fishbone-Moncrief tori of uneven sizes (64^2 ... 256^2),
+    spins 0-0.99, beta 30-300, seeds 0-63, PARA / LAX-F / RK2, stepped together
+    (one launch per stage over all grids, the RK step replayed from a hipGraph,
+    tile shapes autotuned per grid) against the same grids stepped one after
+    another with the single-grid step."""
+    import numpy as np
+    import harm_harm_amd as H
+    from harm_harm_amd.batch import Batch
+    from harm_harm_amd.grid import Grid
+    log = lambda *a: print(*a, file=sys.stderr, flush=True)
+    rng = np.random.default_rng(2026)
+    sizes = [64, 96, 128, 192, 256]
+    members = []
+    for seed in range(args.ngrids):
+        n = int(rng.choice(sizes))
+        members.append((n, float(rng.uniform(0.0, 0.99)), float(np.exp(rng.uniform(np.log(30), np.log(300)))), seed))
+    cfgs = []
+    for n, a, beta, seed in members:
+        c = H.default_config(n, n, 1, recon="para", flux="laxf", timeorder=2)
+        c.a = a
+        cfgs.append(c)
+    zones = sum(n * n for n, _, _, _ in members)
+    B = Batch(cfgs)
+    init = []
+    for g, (n, a, beta, seed) in zip(B.grids, members):
+        g.init_torus(beta=beta, seed=seed)
+        ndt = g.fluxcalc("p")
+        dt = 0.5 / sum(1.0 / x for x in ndt)
+        g.set_time(0.0, dt)
+        init.append((g.get("p"), dt))
+    t0 = time.time()
+    B.tune(reps=3)
+    t_tune = time.time() - t0
+    # batched, graph-replayed
+    B.step(args.warmup, graph=True)
+    B.sync()
+    for g, (p0, dt) in zip(B.grids, init):
+        g.set("p", p0)
+        g.set_time(0.0, dt)
+    B.sync()
+    t0 = time.perf_counter()
+    B.step(args.steps, graph=True)
+    B.sync()
+    el_b = time.perf_counter() - t0
+    log(f"batched: {args.steps} steps of {len(members)} grids ({zones} zones) in {el_b:.3f} s")
+    # the same grids one after another with the single-grid step
+    singles = [Grid(c) for c in cfgs]
+    for s_, (p0, dt) in zip(singles, init):
+        s_.set("p", p0)
+        s_.set_time(0.0, dt)
+        s_.step(args.warmup)
+        s_.sync()
+        s_.set("p", p0)
+        s_.set_time(0.0, dt)
+    H.synchronize()
+    t0 = time.perf_counter()
+    for s_ in singles:
+        s_.step(args.steps)
+        s_.sync()
+    el_s = time.perf_counter() - t0
+    log(f"one by one: {el_s:.3f} s")
+    same = all(np.array_equal(g.get("p"), s_.get("p")) for g, s_ in zip(B.grids, singles))
+    # one member against its own CPU solve
+    check = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import _oracle
+        k = min(range(len(members)), key=lambda i: members[i][0])
+        og = _oracle.OGrid(cfgs[k], nthreads=min(16, os.cpu_count() or 1))
+        st = og.new_state(init[k][0].copy())
+        dt = init[k][1]
+        t0 = time.time()
+        for _ in range(args.steps):
+            dt = og.step(st, dt)
+        el_c = time.time() - t0
+        a = B.grids[k].get("p")[:, og.ng2:og.ng2 + og.cfg.n2, og.ng1:og.ng1 + og.cfg.n1]
+        b = st["p"][:, og.ng2:og.ng2 + og.cfg.n2, og.ng1:og.ng1 + og.cfg.n1]
+        rel = max(float(np.abs(a[..., v] - b[..., v]).max() / max(np.abs(b[..., v]).max(), 1e-300)) for v in range(8))
+        n = members[k][0]
+        check = {"grid": k, "size": f"{n}x{n}", "steps": args.steps, "max_rel_difference": rel,
+                 "cpu_zone_updates_per_s": n * n * args.steps / el_c, "cpu_threads": min(16, os.cpu_count() or 1)}
+    out = {
+        "metric": "zone-updates/s, 64 uneven 2-D tori stepped together (batched launches + hipGraph) vs one by one",
+        "value": zones * args.steps / el_b, "unit": "zone-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": el_b / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded torus initial data)",
+        "config": {"workload": f"{len(members)} 2-D FM tori, sizes {sizes}, PARA/LAX-F/RK2, {zones} zones",
+                   "tune_seconds": t_tune},
+        "one_by_one": {"value": zones * args.steps / el_s, "ms_per_step": el_s / args.steps * 1e3,
+                       "speedup_of_batch": el_s / el_b, "identical_results": same},
+        "oracle_check": check,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -217,9 +313,10 @@ def main():
     ap.add_argument("--grid", default="x".join(map(str, GRID)))
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["step", "inversion"], default="step",
+    ap.add_argument("--workload", choices=["step", "inversion", "batch"], default="step",
                     help="step: the 256x128x64 torus RK3 step (default); inversion: the batched-inversion "
-                         "microbench, plain vs MLP-seeded")
+                         "microbench, plain vs MLP-seeded; batch: 64 uneven 2-D tori stepped together")
+    ap.add_argument("--ngrids", type=int, default=64, help="batch: number of grids")
     ap.add_argument("--n", type=int, default=1 << 24, help="inversion microbench: zones")
     ap.add_argument("--train-n", type=int, default=1 << 18, help="inversion microbench: training pairs")
     ap.add_argument("--train-steps", type=int, default=20000, help="inversion microbench: Adam steps")
@@ -234,6 +331,8 @@ def main():
 
     if args.workload == "inversion":
         return inversion_microbench(args)
+    if args.workload == "batch":
+        return batch_bench(args)
 
     import harm_harm_amd as H
     from harm_harm_amd.grid import Grid
