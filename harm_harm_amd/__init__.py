@@ -12,7 +12,8 @@ __all__ = ["lib", "HarmConfig", "NewtonStats", "HarmError", "check", "default_co
            "device_count", "synchronize", "Event"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libharmstep.so")
+# HARM_HARM_AMD_LIB points at another build of the library (same ABI)
+LIB_PATH = os.environ.get("HARM_HARM_AMD_LIB") or os.path.join(_HERE, "libharmstep.so")
 
 NPR = 8
 RHO, UU, U1, U2, U3, B1, B2, B3 = range(8)

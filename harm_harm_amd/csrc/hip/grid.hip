@@ -352,7 +352,7 @@ __global__ __launch_bounds__(kBlock) void k_emf(Dims d, int ni, int nj, int nk, 
                                                 const double* __restrict__ gf1, const double* __restrict__ gf2,
                                                 const double* __restrict__ gc3, double* __restrict__ emf)
 {
-  emf_body(d, ni, nj, nk, F1, F2, F3, gf1, gf2, gc3, emf, blockIdx.x);
+  emf_body(d, ni, nj, nk, F1, F2, F3, gf1, gf2, gc3, emf, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 template <int DIR>
@@ -387,7 +387,7 @@ template <int DIR>
 __global__ __launch_bounds__(kBlock) void k_ct(Dims d, int ni, int nj, int nk, const double* __restrict__ emf,
                                                double* __restrict__ F)
 {
-  ct_body<DIR>(d, ni, nj, nk, emf, F, blockIdx.x);
+  ct_body<DIR>(d, ni, nj, nk, emf, F, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 // ---------------------------------------------------------------------------
@@ -576,6 +576,14 @@ __device__ __forceinline__ void invert_body(Dims d, Params P, const double* __re
       pz[pl] = pg[pl * nc + o];  // guess: the substep's starting primitives
     }
     int fl = hd::utoprimgen(Uz, G, P.gam, P.gammafail, pz, ns, __builtin_nan(""));
+    // The geometry is needed again only for the floors below.  Reloading it
+    // through a pointer the compiler cannot see through frees its 54
+    // registers during the Newton iterations (they come back from L1/L2).
+    {
+      const double* g2 = gcent;
+      asm volatile("" : "+s"(g2));
+      hd::load_geom(g2, pp, d.nplane, G);
+    }
     if (finalstep) {
       double prfloor[2] = {floors[pp], floors[d.nplane + pp]};
       for (int pl = 0; pl < 2; pl++)
@@ -627,7 +635,9 @@ __device__ __forceinline__ void invert_body(Dims d, Params P, const double* __re
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_invert(Dims d, Params P, const double* __restrict__ U,
+// Four waves per SIMD (128 VGPRs, a few bytes of spill outside the Newton
+// loop): measured faster than three on the torus step.
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_invert(Dims d, Params P, const double* __restrict__ U,
                                                    const double* __restrict__ pg, double* __restrict__ pf,
                                                    int32_t* __restrict__ pflag,
                                                    const double* __restrict__ gcent, const double* __restrict__ floors,
@@ -692,7 +702,12 @@ __global__ __launch_bounds__(kBlock) void k_post_fixup(Dims d, Params P, const d
 // ---------------------------------------------------------------------------
 // boundary conditions
 // ---------------------------------------------------------------------------
-__device__ void extrap_line(const Dims& d, const Params& P, double* p, const double* gcent, int up, int j, int k)
+// Radial extrapolation into ghost zone number `gsel` (0 .. ng1-1, counted
+// outward from the boundary) of the (j, k) line; the slopes come from the
+// three boundary zones, which no ghost write touches, so the ghost zones of a
+// line can be filled by separate threads.
+__device__ void extrap_line(const Dims& d, const Params& P, double* p, const double* gcent, int up, int j, int k,
+                            int gsel)
 {
   const long nc = d.ncell;
   const int ri = up ? d.n1 - 1 : 0;
@@ -750,9 +765,8 @@ __device__ void extrap_line(const Dims& d, const Params& P, double* p, const dou
     dql[HARM_UU + jj] = dql[HARM_UU + jj] * (1.0 - ydqfrac);
     dqg[HARM_UU + jj] = dqg[HARM_UU + jj] * (1.0 - ydqfrac);
   }
-  const int is = up ? d.n1 : -d.ng1;
-  const int ie = up ? d.n1 + d.ng1 - 1 : -1;
-  for (int i = is; i <= ie; i++) {
+  {
+    const int i = up ? d.n1 + gsel : -1 - gsel;
     double pi[HARM_NPR];
 #pragma unroll
     for (int pl = 0; pl < HARM_NPR; pl++) pi[pl] = pr[pl];
@@ -796,11 +810,13 @@ __device__ __forceinline__ void bound_x1_body(Dims d, Params P, double* __restri
                                                      const double* __restrict__ gcent, const int blk)
 {
   const long t = (long)blk * kBlock + threadIdx.x;
-  if (t >= 2L * d.n2 * d.n3) return;
+  if (t >= 2L * d.ng1 * d.n2 * d.n3) return;
   const int up = (int)(t & 1);
   const long l = t >> 1;
-  const int j = (int)(l % d.n2), k = (int)(l / d.n2);
-  extrap_line(d, P, p, gcent, up, j, k);
+  const int gsel = (int)(l % d.ng1);
+  const long line = l / d.ng1;
+  const int j = (int)(line % d.n2), k = (int)(line / d.n2);
+  extrap_line(d, P, p, gcent, up, j, k, gsel);
 }
 
 __global__ __launch_bounds__(kBlock) void k_bound_x1(Dims d, Params P, double* __restrict__ p,
@@ -1319,7 +1335,7 @@ harm_status enq_bound(harm_handle* h, int which)
   double* p = h->arr[which];
   prof_begin(h);
   if (d.n1 > 1)
-    hipLaunchKernelGGL(k_bound_x1, dim3(nblocks(2L * d.n2 * d.n3)), dim3(kBlock), 0, s, d, h->P, p,
+    hipLaunchKernelGGL(k_bound_x1, dim3(nblocks(2L * d.ng1 * d.n2 * d.n3)), dim3(kBlock), 0, s, d, h->P, p,
                        h->geom[HARM_LOC_CENT]);
   if (d.n2 > 1) {
     for (int side = 0; side < 2; side++) {
@@ -1472,7 +1488,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) 
 
 __global__ __launch_bounds__(kBlock) void kb_emf(const GridRef* __restrict__ G, const int* __restrict__ pre, int ng)
 {
-  BATCH_DECODE(false)
+  BATCH_DECODE(true)
   emf_body(g.d, g.emf_ni, g.emf_nj, g.emf_nk, g.arr[HARM_ARR_F1], g.arr[HARM_ARR_F2], g.arr[HARM_ARR_F3],
            g.geom[HARM_LOC_FACE1], g.geom[HARM_LOC_FACE2], g.gcorn3, g.emf, lb);
 }
@@ -1480,7 +1496,7 @@ __global__ __launch_bounds__(kBlock) void kb_emf(const GridRef* __restrict__ G, 
 template <int DIR>
 __global__ __launch_bounds__(kBlock) void kb_ct(const GridRef* __restrict__ G, const int* __restrict__ pre, int ng)
 {
-  BATCH_DECODE(false)
+  BATCH_DECODE(true)
   ct_body<DIR>(g.d, g.emf_ni, g.emf_nj, g.emf_nk, g.emf, g.arr[HARM_ARR_F1 + DIR - 1], lb);
 }
 
@@ -1512,7 +1528,7 @@ __global__ __launch_bounds__(kBlock) void kb_update(const GridRef* __restrict__ 
                         g.geom[HARM_LOC_CENT], g.conn, cuf0, cuf1, cuf2, cun0, cun1, cun2, timeorder, g.ds, lb);
 }
 
-__global__ __launch_bounds__(kBlock) void kb_invert(const GridRef* __restrict__ G, const int* __restrict__ pre, int ng,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void kb_invert(const GridRef* __restrict__ G, const int* __restrict__ pre, int ng,
                                                     int which_pb, int which_pf, int finalstep)
 {
   BATCH_DECODE(true)
@@ -1640,7 +1656,7 @@ harm_status batch_build(harm_batch* b)
     cnt[BS_CT3] = e2 ? nblocks((long)d.n1 * d.n2 * (d.n3 + 1)) : 0;
     cnt[BS_FACC] = e0 ? kFaccBlocks : 0;
     cnt[BS_ZONE] = nblocks((long)d.n1 * d.n2 * d.n3);
-    cnt[BS_BX1] = e0 ? nblocks(2L * d.n2 * d.n3) : 0;
+    cnt[BS_BX1] = e0 ? nblocks(2L * d.ng1 * d.n2 * d.n3) : 0;
     cnt[BS_X2R] = e1 ? nblocks((long)d.ns1 * d.n3 * d.ng2) : 0;
     cnt[BS_X2P] = e1 ? nblocks((long)d.ns1 * d.n3) : 0;
     cnt[BS_BX3] = e2 ? nblocks(d.nplane * 2L * d.ng3) : 0;
