@@ -12,10 +12,12 @@
  * state, fluxes, wave speeds and the inversion against textbook GRMHD and
  * round trips; reconstruction, the face Riemann solver, RK tables, ghost
  * zones, flux-CT and the update/source balance against known answers and
- * invariants.  Parity unpinned: the WENO5BND extras that are not restated
- * (SMONO, stencil reduction, shock reduction), the radial outflow
- * extrapolation formulas, the pole-death velocity interpolation and the
- * failure-averaging cascade, which only the GPU comparison exercises.
+ * invariants; the radial outflow extrapolation against power laws (exact
+ * on the log-radius grid), constant gdet B^1 and the minmod continuation of
+ * B_3; the failure-averaging cascade against linear fields.  Parity
+ * unpinned: the WENO5BND extras that are not restated (SMONO, stencil
+ * reduction, shock reduction) and the pole-death velocity interpolation,
+ * which only the GPU comparison exercises.
  */
 #ifndef HARM_ORACLE_H
 #define HARM_ORACLE_H

@@ -283,3 +283,113 @@ def test_connection_trace_is_gradient_of_log_gdet():
             worst = max(worst, e1, e2)
     print("worst relative mismatch", worst)
     assert worst < 1e-4
+
+
+# ---------------------------------------------------------------------------
+# radial extrapolation and failure averaging
+# ---------------------------------------------------------------------------
+def power_law_state(og, A=2.0, B=0.3, c0=0.02, c1=1e-4, bseed=None):
+    """Active zones: rho = A r^-1.5, u = B r^-2.5, u~^1 linear in i (outflow),
+    other velocities zero, B^1 with gdet B^1 constant."""
+    cfg = og.cfg
+    p = og.zeros()
+    rc = og.rc  # r at zone centres per storage i
+    gdet = og.geom[0][..., 21]
+    for i in range(cfg.n1):
+        ii = i + og.ng1
+        r = rc[ii]
+        a = p[:, og.ng2:og.ng2 + cfg.n2, ii]
+        a[..., 0] = A * r ** -1.5
+        a[..., 1] = B * r ** -2.5
+        a[..., 2] = c0 + c1 * i
+        a[..., 5] = 1e-3 / gdet[og.ng2:og.ng2 + cfg.n2, ii]
+    return p
+
+
+def test_radial_extrapolation_reproduces_power_laws():
+    """The radial ghost zones continue log rho and log u linearly in the zone
+    index (exact for power laws on the log-radius grid), u~ linearly, keep
+    gdet B^1 constant (the reference's bounds.tools.c:1920-1933 "gdet B^x1 ==
+    constant", its slope term vanishing here), and set B^3 so that the
+    covariant B_3 continues linearly."""
+    cfg, og = grid((128, 16, 1), recon="para", flux="laxf", timeorder=2)
+    p = power_law_state(og)
+    og.bound(p)
+    rc = og.rc
+    J = slice(og.ng2, og.ng2 + cfg.n2)
+    gdet = og.geom[0][..., 21]
+    gcov = og.geom[0][..., 0:10]
+    worst = 0.0
+    for i in list(range(-og.ng1, 0)) + list(range(cfg.n1, cfg.n1 + og.ng1)):
+        ii = i + og.ng1
+        z = p[0, J, ii]
+        r = rc[ii]
+        for v, want in ((0, 2.0 * r ** -1.5), (1, 0.3 * r ** -2.5)):
+            worst = max(worst, np.abs(z[:, v] / want - 1).max())
+        assert np.allclose(z[:, 2], 0.02 + 1e-4 * i, rtol=1e-12, atol=1e-16)
+        assert np.allclose(gdet[J, ii] * z[:, 5], 1e-3, rtol=1e-13, atol=0)
+        # B_3 = g_31 B^1 + g_32 B^2 + g_33 B^3 of the ghost continues the edge
+        # value with the minmod-limited slope of the three edge zones
+        def Bd3_at(iii):
+            zz = p[0, J, iii + og.ng1]
+            g_ = gcov[J, iii + og.ng1]
+            return g_[:, 6] * zz[:, 5] + g_[:, 8] * zz[:, 6] + g_[:, 9] * zz[:, 7]
+        Bd3 = gcov[J, ii, 6] * z[:, 5] + gcov[J, ii, 8] * z[:, 6] + gcov[J, ii, 9] * z[:, 7]
+        up = i >= 0
+        edge = cfg.n1 - 1 if up else 0
+        step_in = -1 if up else 1
+        b0, b1, b2 = Bd3_at(edge), Bd3_at(edge + step_in), Bd3_at(edge + 2 * step_in)
+        mm = lambda a, b: np.where(a * b <= 0, 0.0, np.where(np.abs(a) < np.abs(b), a, b))
+        s_in = mm(mm(b2 - b1, b1 - b0), b2 - b0)  # per zone, inward
+        want = b0 + (-s_in if up else s_in) * (i - edge)
+        # the rows next to the poles are rewritten afterwards by the pole
+        # treatment (POLEDEATH=1 row per pole), so they are left out here
+        inner = slice(1, cfg.n2 - 1)
+        assert np.allclose(Bd3[inner], want[inner], rtol=1e-12, atol=1e-14 * np.abs(b0).max())
+    print("worst relative error of the power laws in the ghost zones", worst)
+    assert worst < 1e-12
+
+
+def test_radial_extrapolation_limits_inward_growth():
+    """Steep profiles: going inward, log rho may grow by at most log(1.1) per
+    zone (the ghost density is the edge value times 1.1^|di|)."""
+    cfg, og = grid((24, 8, 1), recon="para", flux="laxf", timeorder=2)  # coarse: 1.5 dx > log 1.1
+    p = power_law_state(og)
+    og.bound(p)
+    J = slice(og.ng2, og.ng2 + cfg.n2)
+    edge = p[0, J, og.ng1, 0]
+    for i in range(-og.ng1, 0):
+        assert np.allclose(p[0, J, i + og.ng1, 0], edge * 1.1 ** (-i), rtol=1e-13)
+
+
+def test_failure_averaging_reproduces_linear_fields():
+    """A failed zone takes the average of good neighbours: for a field linear
+    in (i, j) every symmetric choice of neighbours returns the zone's own
+    value, so the result pins which neighbours are used; a zone whose eight
+    neighbours all failed keeps its value."""
+    cfg, og = grid((32, 24, 1), recon="para", flux="laxf", timeorder=2)
+    p = og.zeros()
+    I, Jg = np.meshgrid(np.arange(og.ns1) - og.ng1, np.arange(og.ns2) - og.ng2)
+    lin = {0: 1.0 + 0.01 * I + 0.02 * Jg, 1: 0.1 + 0.001 * I - 0.002 * Jg, 2: 1e-3 * (1 + 0.01 * I),
+           3: 1e-3 * (1 - 0.02 * Jg), 4: 1e-4 * (1 + 0.03 * I + 0.01 * Jg)}
+    for v, f in lin.items():
+        p[0, :, :, v] = f
+    pflag = np.zeros(og.shape[:3], dtype=np.int32)
+    marks = [(5, 5), (10, 7), (11, 7), (20, 12), (20, 13)]  # isolated, radial pair, polar pair
+    block = [(ii, jj) for ii in range(14, 17) for jj in range(16, 19)]  # 3x3 failed block
+    for (i, j) in marks + block:
+        pflag[0, j + og.ng2, i + og.ng1] = 1
+    before = p.copy()
+    for (i, j) in marks + block:  # scramble the failed zones
+        p[0, j + og.ng2, i + og.ng1, :5] = -7.0
+    centre = p[0, 17 + og.ng2, 15 + og.ng1, :5].copy()
+    og.post_fixup(p, pflag)
+    for (i, j) in marks:
+        got = p[0, j + og.ng2, i + og.ng1, :5]
+        want = before[0, j + og.ng2, i + og.ng1, :5]
+        assert np.allclose(got, want, rtol=1e-13, atol=1e-16), ((i, j), got, want)
+    # the centre keeps its own values; the Lorentz-factor cap then scales its
+    # (scrambled, superluminal) velocity down along the same direction
+    got = p[0, 17 + og.ng2, 15 + og.ng1, :5]
+    assert np.array_equal(got[:2], centre[:2])
+    assert got[2] == got[3] == got[4] and -7.0 < got[2] < 0.0
