@@ -35,6 +35,24 @@ __host__ __device__ constexpr int sym(int a, int b)
                   : ((b == 0) ? a : (b == 1) ? 3 + a : (b == 2) ? 5 + a : 9);
 }
 
+// A copy of a global-memory pointer that the compiler cannot see through (an
+// empty asm).  Loads through it are not merged with earlier loads of the same
+// addresses, so a value read before a register-hungry stretch of code is read
+// again afterwards (an L1/L2 hit) instead of being held in registers across
+// it.  The copy stays in the global address space, so the loads remain
+// global_load (a generic pointer would give flat_load).  The pointer must be
+// the same in every lane of the wave (a base address); it is taken from the
+// first lane into scalar registers.
+__device__ __forceinline__ const double* fresh(const double* p)
+{
+  const unsigned long long v = (unsigned long long)p;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+  auto g = (const __attribute__((address_space(1))) double*)(((unsigned long long)hi << 32) | lo);
+  asm volatile("" : "+s"(g));
+  return (const double*)g;
+}
+
 // Load a record stored SoA with field stride `fs` starting at `base`.
 __device__ __forceinline__ void load_geom(const double* __restrict__ tab, long base, long fs, Geom& g)
 {

@@ -1,0 +1,154 @@
+// gkern.h -- definitions shared by the grid step's translation units: the
+// device scalar slots, run parameters, face-tile plans, the XCD-aware block
+// order, block reductions and the per-grid descriptor of batched launches.
+// The face-flux kernels live in their own translation unit (flux.hip, built
+// with FMA contraction); everything else of the step is in grid.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "dgrid.h"
+
+namespace hk {
+
+using hd::Dims;
+
+constexpr int kBlock = 256;  // threads per block of every grid kernel
+constexpr double kBIG = 1.e+100;
+
+// device scalar slots
+enum { DS_DT = 0, DS_T = 1, DS_NDT1 = 2, DS_NDT2 = 3, DS_NDT3 = 4, DS_ACCDT = 5, DS_LASTNDT = 6, DS_FLUXACC = 8,
+       DS_N = 16 };
+enum { DI_NSTROKE = 0, DI_NFAIL = 1, DI_N = 4 };
+
+struct Params {
+  double gam, cour, safe, gammamax, gammafail;
+  double dx[4], startx[4];
+};
+
+__device__ __forceinline__ void atomic_min_pos(double* addr, double v)
+{
+  // valid for v >= 0 (IEEE order of non-negative doubles equals integer order)
+  atomicMin((unsigned long long*)addr, (unsigned long long)__double_as_longlong(v));
+}
+
+// block-wide minimum followed by one atomic per block
+__device__ __forceinline__ void block_min_to(double v, double* addr)
+{
+  __shared__ double red[kBlock / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_down(v, off, 64));
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = red[0];
+    for (int w = 1; w < kBlock / 64; w++) m = fmin(m, red[w]);
+    atomic_min_pos(addr, m);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// fluxes
+// ---------------------------------------------------------------------------
+
+struct Box {
+  int is, js, ks, ni, nj, nk;
+  __device__ __forceinline__ bool at(long t, int& i, int& j, int& k) const
+  {
+    const long n = (long)ni * nj * nk;
+    if (t >= n) return false;
+    i = is + (int)(t % ni);
+    const long r = t / ni;
+    j = js + (int)(r % nj);
+    k = ks + (int)(r / nj);
+    return true;
+  }
+  long count() const { return (long)ni * nj * nk; }
+};
+
+inline Box face_box(const Dims& d, int dir)
+{
+  const int e0 = d.n1 > 1, e1 = d.n2 > 1, e2 = d.n3 > 1;
+  int is = 0, ie = d.n1 - 1, js = 0, je = d.n2 - 1, ks = 0, ke = d.n3 - 1;
+  if (dir == 1) { ie = d.n1; js -= e1; je += e1; ks -= e2; ke += e2; }
+  if (dir == 2) { je = d.n2; is -= e0; ie += e0; ks -= e2; ke += e2; }
+  if (dir == 3) { ke = d.n3; is -= e0; ie += e0; js -= e1; je += e1; }
+  Box b{is, js, ks, ie - is + 1, je - js + 1, ke - ks + 1};
+  return b;
+}
+
+// Face tiles.  A block first reconstructs every cell its faces touch, one
+// (cell, variable) pair per lane, into LDS (each cell once, both edges), then
+// solves one face per lane.  Direction 1 runs along i, the contiguous
+// dimension: a block takes `fpb` consecutive faces of the face box flattened
+// as (i, k, j) (rows of ni faces own ni+1 cells, so a block needs
+// fpb + rows + 1 cells).  Directions 2 and 3 take tc consecutive i by td
+// faces along the sweep in one transverse plane (k for direction 2, j for
+// direction 3), i.e. (td+1)*tc cells.  Tiles are ordered so that neighbours
+// in the launch reuse the same (i, j) rows of the 2-D face geometry: phi
+// fastest for directions 1 and 2, the sweep fastest for direction 3.
+constexpr int kCellMax = 320;
+
+struct FaceTile {
+  Box b;
+  int ncol;       // dir 1: rows (nj*nk)
+  int nd;         // faces along the sweep
+  int fpb;        // dir 1: faces per block
+  int tc, td;     // dirs 2,3: tile shape
+  int nti, ntd;   // dirs 2,3: tiles along i / along the sweep
+  int nplane;     // dirs 2,3: transverse planes (nk for dir 2, nj for dir 3)
+  int i0;         // dirs 2,3: i of the first tile's first column (-ng1: tiles start on 128-byte lines)
+  long nblk;
+};
+
+// Blocks are dealt to the 8 XCDs round-robin; remap so that each XCD works on
+// a contiguous range of tiles (neighbouring tiles share stencil halos in L2).
+__device__ __forceinline__ int xcd_tile(int b, int n)
+{
+  const int x = b & 7, loc = b >> 3, q = n >> 3, r = n & 7;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + loc;
+}
+
+struct GridRef {
+  Dims d;
+  Params P;
+  int fluxtype;
+  double* arr[HARM_NARR];
+  int32_t* pflag;
+  const double* geom[HARM_NLOC];
+  const double* gcorn3;
+  const double* conn;
+  const double* floors;
+  const double* rcent;
+  double* emf;
+  int emf_ni, emf_nj, emf_nk;
+  double* ds;
+  unsigned long long* di;
+  double* facc;
+  FaceTile ft[4];
+};
+
+// A pointer read from memory (a GridRef) is generic to the compiler, so loads
+// and stores through it would be flat instructions.  This says that it points
+// to global memory (global_load/global_store): the pointer, the same in every
+// lane (a grid's base address), is taken into scalar registers and passed
+// through an empty asm typed as a global pointer, which the compiler cannot
+// fold back into the generic one.
+template <class T>
+__device__ __forceinline__ T* gmem(T* p)
+{
+  const unsigned long long v = (unsigned long long)p;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+  auto g = (__attribute__((address_space(1))) T*)(((unsigned long long)hi << 32) | lo);
+  asm("" : "+s"(g));
+  return (T*)g;
+}
+
+// Launchers of the face-flux kernels (flux.hip): one direction of one grid,
+// or one direction of every grid of a batch.
+void launch_flux(int dir, int recon, const FaceTile& T, hipStream_t s, const Dims& d, const double* pb, double* F,
+                 const double* gface, const Params& P, int fluxtype, int do_ct, double* ds);
+void launch_flux_batch(int dir, int recon, unsigned nblk, hipStream_t s, const GridRef* G, const int* pre, int ng,
+                       int which, int do_ct);
+
+}  // namespace hk

@@ -4,7 +4,8 @@
 // box (i fastest, so every kernel's lanes walk i and HBM accesses coalesce);
 // geometry is stored once per (i,j) plane and reused across phi.  Each hot loop
 // of advance() is one kernel:
-//   k_flux<dir>   reconstruct + Riemann flux per face, fused   (flux.c:852)
+//   k_flux<dir>   reconstruct + Riemann flux per face, fused   (flux.c:852;
+//                 in flux.hip, built with FMA contraction)
 //   k_emf/k_ct    Toth corner EMFs and face B fluxes           (fluxct.c:186)
 //   k_update      U(pi), flux divergence, connection source,
 //                 RK combine and the source dt limit           (advance.c:420)
@@ -21,6 +22,7 @@
 #include <vector>
 #include <algorithm>
 #include "dgrid.h"
+#include "gkern.h"
 #include "hhost.h"
 #include "../host/hgeom.h"
 #include "../host/hinit.h"
@@ -28,105 +30,9 @@
 using hd::Dims;
 using hd::Geom;
 using hd::State;
+using namespace hk;
 
 namespace {
-
-constexpr int kBlock = 256;
-constexpr double kBIG = 1.e+100;
-
-// device scalar slots
-enum { DS_DT = 0, DS_T = 1, DS_NDT1 = 2, DS_NDT2 = 3, DS_NDT3 = 4, DS_ACCDT = 5, DS_LASTNDT = 6, DS_FLUXACC = 8,
-       DS_N = 16 };
-enum { DI_NSTROKE = 0, DI_NFAIL = 1, DI_N = 4 };
-
-struct Params {
-  double gam, cour, safe, gammamax, gammafail;
-  double dx[4], startx[4];
-};
-
-__device__ __forceinline__ void atomic_min_pos(double* addr, double v)
-{
-  // valid for v >= 0 (IEEE order of non-negative doubles equals integer order)
-  atomicMin((unsigned long long*)addr, (unsigned long long)__double_as_longlong(v));
-}
-
-// block-wide minimum followed by one atomic per block
-__device__ __forceinline__ void block_min_to(double v, double* addr)
-{
-  __shared__ double red[kBlock / 64];
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_down(v, off, 64));
-  if (lane == 0) red[wid] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double m = red[0];
-    for (int w = 1; w < kBlock / 64; w++) m = fmin(m, red[w]);
-    atomic_min_pos(addr, m);
-  }
-}
-
-// ---------------------------------------------------------------------------
-// fluxes
-// ---------------------------------------------------------------------------
-
-struct Box {
-  int is, js, ks, ni, nj, nk;
-  __device__ __forceinline__ bool at(long t, int& i, int& j, int& k) const
-  {
-    const long n = (long)ni * nj * nk;
-    if (t >= n) return false;
-    i = is + (int)(t % ni);
-    const long r = t / ni;
-    j = js + (int)(r % nj);
-    k = ks + (int)(r / nj);
-    return true;
-  }
-  long count() const { return (long)ni * nj * nk; }
-};
-
-Box face_box(const Dims& d, int dir)
-{
-  const int e0 = d.n1 > 1, e1 = d.n2 > 1, e2 = d.n3 > 1;
-  int is = 0, ie = d.n1 - 1, js = 0, je = d.n2 - 1, ks = 0, ke = d.n3 - 1;
-  if (dir == 1) { ie = d.n1; js -= e1; je += e1; ks -= e2; ke += e2; }
-  if (dir == 2) { je = d.n2; is -= e0; ie += e0; ks -= e2; ke += e2; }
-  if (dir == 3) { ke = d.n3; is -= e0; ie += e0; js -= e1; je += e1; }
-  Box b{is, js, ks, ie - is + 1, je - js + 1, ke - ks + 1};
-  return b;
-}
-
-// Face tiles.  A block first reconstructs every cell its faces touch, one
-// (cell, variable) pair per lane, into LDS (each cell once, both edges), then
-// solves one face per lane.  Direction 1 runs along i, the contiguous
-// dimension: a block takes `fpb` consecutive faces of the face box flattened
-// as (i, k, j) (rows of ni faces own ni+1 cells, so a block needs
-// fpb + rows + 1 cells).  Directions 2 and 3 take tc consecutive i by td
-// faces along the sweep in one transverse plane (k for direction 2, j for
-// direction 3), i.e. (td+1)*tc cells.  Tiles are ordered so that neighbours
-// in the launch reuse the same (i, j) rows of the 2-D face geometry: phi
-// fastest for directions 1 and 2, the sweep fastest for direction 3.
-constexpr int kCellMax = 320;
-
-struct FaceTile {
-  Box b;
-  int ncol;       // dir 1: rows (nj*nk)
-  int nd;         // faces along the sweep
-  int fpb;        // dir 1: faces per block
-  int tc, td;     // dirs 2,3: tile shape
-  int nti, ntd;   // dirs 2,3: tiles along i / along the sweep
-  int nplane;     // dirs 2,3: transverse planes (nk for dir 2, nj for dir 3)
-  int i0;         // dirs 2,3: i of the first tile's first column (-ng1: tiles start on 128-byte lines)
-  long nblk;
-};
-
-// Blocks are dealt to the 8 XCDs round-robin; remap so that each XCD works on
-// a contiguous range of tiles (neighbouring tiles share stencil halos in L2).
-__device__ __forceinline__ int xcd_tile(int b, int n)
-{
-  const int x = b & 7, loc = b >> 3, q = n >> 3, r = n & 7;
-  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + loc;
-}
 
 // Active-zone order for the per-zone kernels: i fastest (coalesced), then k,
 // then j.  Consecutive rows are consecutive phi planes of the same (i, j)
@@ -138,176 +44,6 @@ __device__ __forceinline__ void zone_ijk(const Dims& d, int t, int& i, int& j, i
   i = t - r * d.n1;
   j = r / d.n3;
   k = r - j * d.n3;
-}
-
-// Riemann flux of one face from its two reconstructed states
-// (getplpr/check_plpr flux.c:1952-2110, flux_compute fluxcompute.c:19-402,
-// cminmax_calc wavespeeds.c:397).  Returns ctop.
-template <int DIR>
-__device__ __forceinline__ double face_riemann(double* p_l, double* p_r, const Geom& G, const Params& P, int fluxtype,
-                                               int do_ct, double* Fo)
-{
-  const int pbd = HARM_B1 + DIR - 1;
-  p_l[pbd] = p_r[pbd] = 0.5 * (p_l[pbd] + p_r[pbd]);
-  hd::limit_gamma(P.gammamax, p_l, G);
-  hd::limit_gamma(P.gammamax, p_r, G);
-  double F_l[HARM_NPR], U_l[HARM_NPR], F_r[HARM_NPR], U_r[HARM_NPR];
-  double cmaxl, cminl, cmaxr, cminr;
-  {
-    State q;
-    hd::get_state(p_l, G, P.gam, q);
-    hd::primtoflux(p_l, q, DIR, G, F_l);
-    hd::primtoflux(p_l, q, 0, G, U_l);
-    hd::vchar(p_l, q, DIR, G, P.gam, cmaxl, cminl);
-  }
-  {
-    State q;
-    hd::get_state(p_r, G, P.gam, q);
-    hd::primtoflux(p_r, q, DIR, G, F_r);
-    hd::primtoflux(p_r, q, 0, G, U_r);
-    hd::vchar(p_r, q, DIR, G, P.gam, cmaxr, cminr);
-  }
-  double lmin = fmin(cminl, cminr), lmax = fmax(cmaxl, cmaxr);
-  lmin = fabs(fmax(0., -lmin));
-  lmax = fabs(fmax(0., lmax));
-  const double cmin = lmin, cmax = lmax, ct = fmax(lmax, lmin);
-  if (fluxtype == HARM_FLUX_HLL && cmax + cmin != 0.0) {
-    const double ic = 1.0 / (cmax + cmin);
-#pragma unroll
-    for (int pl = 0; pl < HARM_NPR; pl++)
-      Fo[pl] = ((cmax * F_l[pl] + cmin * F_r[pl]) - (cmax * cmin) * (U_r[pl] - U_l[pl])) * ic;
-  } else {
-#pragma unroll
-    for (int pl = 0; pl < HARM_NPR; pl++) Fo[pl] = (0.5 * ((F_l[pl] + F_r[pl]) - ct * (U_r[pl] - U_l[pl])));
-  }
-  if (do_ct) {
-    const double ign = hd::igdetnosing(G.gdet);
-    Fo[HARM_B1] *= ign;
-    Fo[HARM_B2] *= ign;
-    Fo[HARM_B3] *= ign;
-  }
-  return ct;
-}
-
-// Three waves per SIMD: the LDS tile allows three blocks per CU.
-template <int DIR, int RECON>
-__device__ __forceinline__ void flux_body(Dims d, FaceTile T, const double* __restrict__ pb,
-                                                 double* __restrict__ F, const double* __restrict__ gface, Params P,
-                                                 int fluxtype, int do_ct, double* __restrict__ ds, const int blk)
-{
-  __shared__ double s_l[HARM_NPR * kCellMax], s_r[HARM_NPR * kCellMax];
-  const int tile = blk;
-  const int st = (DIR == 1) ? 1 : (DIR == 2) ? d.ps1 : d.ps1 * d.ns2;
-  const Box& b = T.b;
-
-  // ---- phase 1: reconstruct the tile's cells ----
-  // (32-bit index arithmetic: a face box stays far below 2^31 faces)
-  int f0 = 0, g_lo = 0, ncell, bc = 0, bd = 0, pl = 0;
-  const int rowc = T.nd + 1;  // dir 1: cells per row
-  if (DIR == 1) {
-    const int nf = T.nd * T.ncol;
-    f0 = tile * T.fpb;
-    const int fl = min(f0 + T.fpb, nf) - 1;
-    g_lo = (f0 / T.nd) * rowc + f0 % T.nd;
-    ncell = (fl / T.nd) * rowc + fl % T.nd + 1 - g_lo + 1;
-  } else {
-    // bc: i tile, bd: sweep tile, pl: transverse plane
-    if (DIR == 2) {
-      pl = tile % T.nplane;
-      const int r = tile / T.nplane;
-      bd = r % T.ntd;
-      bc = r / T.ntd;
-    } else {
-      bd = tile % T.ntd;
-      const int r = tile / T.ntd;
-      pl = r % T.nplane;
-      bc = r / T.nplane;
-    }
-    ncell = (T.td + 1) * T.tc;
-  }
-  for (int it = threadIdx.x; it < ncell * HARM_NPR; it += kBlock) {
-    const int v = it / ncell, slot = it - v * ncell;
-    int i, j, k;
-    if (DIR == 1) {
-      const int g = g_lo + slot;
-      const int row = g / rowc;
-      i = b.is - 1 + (g - row * rowc);
-      const int rj = row / b.nk;
-      k = b.ks + (row - rj * b.nk);
-      j = b.js + rj;
-    } else {
-      const int dd = slot / T.tc, cc = slot - dd * T.tc;
-      i = T.i0 + bc * T.tc + cc;
-      const int pd = bd * T.td - 1 + dd;  // sweep index of the cell relative to the first face
-      if (i < b.is || i >= b.is + b.ni || pd > T.nd - 1) continue;
-      if (DIR == 2) { k = b.ks + pl; j = b.js + pd; }
-      else { j = b.js + pl; k = b.ks + pd; }
-    }
-    const double* a = pb + (long)v * d.ncell + hd::cidx(d, i, j, k);
-    double y[5];
-#pragma unroll
-    for (int m = 0; m < 5; m++) y[m] = a[(m - 2) * st];
-    double l, r;
-    hd::recon_cell<RECON>(y + 2, l, r);
-    s_l[v * kCellMax + slot] = l;
-    s_r[v * kCellMax + slot] = r;
-  }
-  __syncthreads();
-
-  // ---- phase 2: one face per lane ----
-  double dtij = kBIG;
-  int i, j, k, sl = 0, sr = 0;
-  bool ok;
-  if (DIR == 1) {
-    const int f = f0 + (int)threadIdx.x;
-    ok = (int)threadIdx.x < T.fpb && f < T.nd * T.ncol;
-    if (ok) {
-      const int row = f / T.nd;
-      const int fi = f - row * T.nd;
-      sr = row * rowc + fi + 1 - g_lo;
-      sl = sr - 1;
-      const int rj = row / b.nk;
-      i = b.is + fi;
-      k = b.ks + (row - rj * b.nk);
-      j = b.js + rj;
-    }
-  } else {
-    const int dd = (int)threadIdx.x / T.tc, cc = (int)threadIdx.x - dd * T.tc;
-    const int fd = bd * T.td + dd;
-    i = T.i0 + bc * T.tc + cc;
-    ok = dd < T.td && i >= b.is && i < b.is + b.ni && fd < T.nd;
-    if (ok) {
-      sl = dd * T.tc + cc;
-      sr = sl + T.tc;
-      if (DIR == 2) { k = b.ks + pl; j = b.js + fd; }
-      else { j = b.js + pl; k = b.ks + fd; }
-    }
-  }
-  if (ok) {
-    double p_l[HARM_NPR], p_r[HARM_NPR];
-#pragma unroll
-    for (int v = 0; v < HARM_NPR; v++) {
-      p_l[v] = s_r[v * kCellMax + sl];
-      p_r[v] = s_l[v * kCellMax + sr];
-    }
-    Geom G;
-    hd::load_geom(gface, hd::pidx(d, i, j), d.nplane, G);
-    double Fo[HARM_NPR];
-    const double ct = face_riemann<DIR>(p_l, p_r, G, P, fluxtype, do_ct, Fo);
-    const long c = hd::cidx(d, i, j, k);
-#pragma unroll
-    for (int pl = 0; pl < HARM_NPR; pl++) F[(long)pl * d.ncell + c] = Fo[pl];
-    dtij = P.cour * P.dx[DIR] / ct;
-  }
-  block_min_to(dtij, ds + DS_NDT1 + DIR - 1);
-}
-
-template <int DIR, int RECON>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_flux(Dims d, FaceTile T, const double* __restrict__ pb,
-                                                 double* __restrict__ F, const double* __restrict__ gface, Params P,
-                                                 int fluxtype, int do_ct, double* __restrict__ ds)
-{
-  flux_body<DIR, RECON>(d, T, pb, F, gface, P, fluxtype, do_ct, ds, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 // Toth EMFs at corners, each multiplied by the corner gdet (CORNGDETVERSION)
@@ -391,6 +127,83 @@ __global__ __launch_bounds__(kBlock) void k_ct(Dims d, int ni, int nj, int nk, c
 }
 
 // ---------------------------------------------------------------------------
+// inversion + fixup1zone (floors via ZAMO-frame mass/energy addition)
+// ---------------------------------------------------------------------------
+// One zone: conserved Uz (with gdet) in registers, the guess read from pg at
+// offset o, the result written to pf and pflag.  Returns the flag; the Newton
+// stroke count is added to ns.
+__device__ __forceinline__ int invert_zone(const Dims& d, const Params& P, double* Uz,
+                                           const double* __restrict__ pg, double* __restrict__ pf,
+                                           int32_t* __restrict__ pflag, const double* __restrict__ gcent,
+                                           const double* __restrict__ floors, int finalstep, long o, long pp, int& ns)
+{
+  const long nc = d.ncell;
+  Geom G;
+  hd::load_geom(gcent, pp, d.nplane, G);
+  double pz[HARM_NPR];
+#pragma unroll
+  for (int pl = 0; pl < HARM_NPR; pl++) pz[pl] = pg[pl * nc + o];  // guess: the substep's starting primitives
+  int fl = hd::utoprimgen(Uz, G, P.gam, P.gammafail, pz, ns, __builtin_nan(""));
+  // The geometry is needed again only for the floors below.  Reloading it
+  // through a pointer the compiler cannot see through frees its 54
+  // registers during the Newton iterations (they come back from L1/L2).
+  hd::load_geom(hd::fresh(gcent), pp, d.nplane, G);
+  if (finalstep) {
+    double prfloor[2] = {floors[pp], floors[d.nplane + pp]};
+    for (int pl = 0; pl < 2; pl++)
+      if (prfloor[pl] < D_SMALL) prfloor[pl] = D_SMALL;
+    double prnew[HARM_NPR];
+    bool didchange = false;
+#pragma unroll
+    for (int pl = 0; pl < HARM_NPR; pl++) {
+      if (pl <= HARM_UU && prfloor[pl] > pz[pl]) { didchange = true; prnew[pl] = prfloor[pl]; }
+      else prnew[pl] = pz[pl];
+    }
+    if (didchange) {
+      Geom G1 = G;
+      G1.gdet = 1.0;
+      State q, dq;
+      double Ua[HARM_NPR], dpr[HARM_NPR], dUa[HARM_NPR];
+      hd::get_state(pz, G, P.gam, q);
+      hd::primtoflux(pz, q, 0, G1, Ua);
+#pragma unroll
+      for (int pl = 0; pl < HARM_NPR; pl++) dpr[pl] = 0.0;
+      for (int pl = HARM_RHO; pl <= HARM_UU; pl++) dpr[pl] = prnew[pl] - pz[pl];
+      hd::get_state(dpr, G, P.gam, dq);
+      hd::primtoflux(dpr, dq, 0, G1, dUa);
+#pragma unroll
+      for (int pl = 0; pl < HARM_NPR; pl++) Ua[pl] += dUa[pl];
+      fl = hd::jon(Ua, G, P.gam, P.gammafail, pz, ns, __builtin_nan(""));
+      if (fl > 0) {
+#pragma unroll
+        for (int pl = 0; pl < HARM_NPR; pl++) pz[pl] = prnew[pl];
+      }
+    }
+    hd::limit_gamma(P.gammamax, pz, G);
+  }
+#pragma unroll
+  for (int pl = 0; pl < HARM_NPR; pl++) pf[pl * nc + o] = pz[pl];
+  pflag[o] = fl;
+  return fl;
+}
+
+// Block sums of the Newton strokes and failures into di (integer sums are
+// order independent).  Every thread of the block calls it.
+__device__ __forceinline__ void newton_counts_to(int ns, int nf, unsigned long long* __restrict__ di)
+{
+  __shared__ unsigned long long red[2];
+  if (threadIdx.x == 0) { red[0] = 0; red[1] = 0; }
+  __syncthreads();
+  if (ns) atomicAdd(&red[0], (unsigned long long)ns);
+  if (nf) atomicAdd(&red[1], 1ull);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (red[0]) atomicAdd(&di[DI_NSTROKE], red[0]);
+    if (red[1]) atomicAdd(&di[DI_NFAIL], red[1]);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // update: Ui = U(pi); dU = -div F + source(pb); Uf, ucum; source dt limit
 // ---------------------------------------------------------------------------
 // DM: bit m-1 set when dimension m has more than one zone (a compile-time
@@ -459,10 +272,10 @@ __device__ __forceinline__ void update_body(Dims d, Params P, const double* __re
       for (int pl = 0; pl < HARM_NPR; pl++) pbz[pl] = pb[pl * nc + o];
       State qb;
       hd::get_state(pbz, G, P.gam, qb);
-      const double* cz = conn + pp;
-      const long np = d.nplane;
       double T00;
       double dUg[4];
+      const double* cz = conn + pp;
+      const long np = d.nplane;
       hd::source_rows(pbz, G, qb, [&](int c) { return cz[(long)c * np]; }, dUg, T00);
 #pragma unroll
       for (int l = 0; l < 4; l++) dUt[HARM_UU + l] = dUt[HARM_UU + l] + dUg[l];
@@ -549,9 +362,6 @@ __global__ __launch_bounds__(kBlock) void k_update(Dims d, Params P, const doubl
   update_body<DM, MODE>(d, P, pi, pb, F1, F2, F3, ui, uf, ucum, gcent, conn, cuf0, cuf1, cuf2, cun0, cun1, cun2, timeorder, ds, xcd_tile(blockIdx.x, gridDim.x));
 }
 
-// ---------------------------------------------------------------------------
-// inversion + fixup1zone (floors via ZAMO-frame mass/energy addition)
-// ---------------------------------------------------------------------------
 __device__ __forceinline__ void invert_body(Dims d, Params P, const double* __restrict__ U,
                                                    const double* __restrict__ pg, double* __restrict__ pf,
                                                    int32_t* __restrict__ pflag,
@@ -567,72 +377,12 @@ __device__ __forceinline__ void invert_body(Dims d, Params P, const double* __re
     const long o = hd::cidx(d, i, j, k);
     const long pp = hd::pidx(d, i, j);
     const long nc = d.ncell;
-    Geom G;
-    hd::load_geom(gcent, pp, d.nplane, G);
-    double Uz[HARM_NPR], pz[HARM_NPR];
+    double Uz[HARM_NPR];
 #pragma unroll
-    for (int pl = 0; pl < HARM_NPR; pl++) {
-      Uz[pl] = U[pl * nc + o];
-      pz[pl] = pg[pl * nc + o];  // guess: the substep's starting primitives
-    }
-    int fl = hd::utoprimgen(Uz, G, P.gam, P.gammafail, pz, ns, __builtin_nan(""));
-    // The geometry is needed again only for the floors below.  Reloading it
-    // through a pointer the compiler cannot see through frees its 54
-    // registers during the Newton iterations (they come back from L1/L2).
-    {
-      const double* g2 = gcent;
-      asm volatile("" : "+s"(g2));
-      hd::load_geom(g2, pp, d.nplane, G);
-    }
-    if (finalstep) {
-      double prfloor[2] = {floors[pp], floors[d.nplane + pp]};
-      for (int pl = 0; pl < 2; pl++)
-        if (prfloor[pl] < D_SMALL) prfloor[pl] = D_SMALL;
-      double prnew[HARM_NPR];
-      bool didchange = false;
-#pragma unroll
-      for (int pl = 0; pl < HARM_NPR; pl++) {
-        if (pl <= HARM_UU && prfloor[pl] > pz[pl]) { didchange = true; prnew[pl] = prfloor[pl]; }
-        else prnew[pl] = pz[pl];
-      }
-      if (didchange) {
-        Geom G1 = G;
-        G1.gdet = 1.0;
-        State q, dq;
-        double Ua[HARM_NPR], dpr[HARM_NPR], dUa[HARM_NPR];
-        hd::get_state(pz, G, P.gam, q);
-        hd::primtoflux(pz, q, 0, G1, Ua);
-#pragma unroll
-        for (int pl = 0; pl < HARM_NPR; pl++) dpr[pl] = 0.0;
-        for (int pl = HARM_RHO; pl <= HARM_UU; pl++) dpr[pl] = prnew[pl] - pz[pl];
-        hd::get_state(dpr, G, P.gam, dq);
-        hd::primtoflux(dpr, dq, 0, G1, dUa);
-#pragma unroll
-        for (int pl = 0; pl < HARM_NPR; pl++) Ua[pl] += dUa[pl];
-        fl = hd::jon(Ua, G, P.gam, P.gammafail, pz, ns, __builtin_nan(""));
-        if (fl > 0) {
-#pragma unroll
-          for (int pl = 0; pl < HARM_NPR; pl++) pz[pl] = prnew[pl];
-        }
-      }
-      hd::limit_gamma(P.gammamax, pz, G);
-    }
-#pragma unroll
-    for (int pl = 0; pl < HARM_NPR; pl++) pf[pl * nc + o] = pz[pl];
-    pflag[o] = fl;
-    nf = fl > 0;
+    for (int pl = 0; pl < HARM_NPR; pl++) Uz[pl] = U[pl * nc + o];
+    nf = invert_zone(d, P, Uz, pg, pf, pflag, gcent, floors, finalstep, o, pp, ns) > 0;
   }
-  // integer sums are order independent
-  __shared__ unsigned long long red[2];
-  if (threadIdx.x == 0) { red[0] = 0; red[1] = 0; }
-  __syncthreads();
-  if (ns) atomicAdd(&red[0], (unsigned long long)ns);
-  if (nf) atomicAdd(&red[1], 1ull);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    if (red[0]) atomicAdd(&di[DI_NSTROKE], red[0]);
-    if (red[1]) atomicAdd(&di[DI_NFAIL], red[1]);
-  }
+  newton_counts_to(ns, nf, di);
 }
 
 // Four waves per SIMD (128 VGPRs, a few bytes of spill outside the Newton
@@ -1031,6 +781,8 @@ struct harm_handle {
   double* emf = nullptr;
   double* facc = nullptr;  // boundary-flux partial sums
   int tile_td[4] = {0, 0, 0, 0};  // sweep depth of the face tiles of dirs 2,3 (0: cost model)
+  hipGraph_t sgraph = nullptr;      // one full step, captured on first use when not profiling
+  hipGraphExec_t sexec = nullptr;
   FaceTile ft[4];                 // planned face tiles per direction (index 1..3)
   double* ds = nullptr;
   unsigned long long* di = nullptr;
@@ -1187,6 +939,14 @@ void prof_end(harm_handle* h, int kind)
 }
 
 harm_status check_handle(const harm_handle* h) { return h ? HARM_OK : hh::set_error(HARM_EINVAL, "null handle"); }
+
+void drop_step_graph(harm_handle* h)
+{
+  if (h->sexec) (void)hipGraphExecDestroy(h->sexec);
+  if (h->sgraph) (void)hipGraphDestroy(h->sgraph);
+  h->sexec = nullptr;
+  h->sgraph = nullptr;
+}
 harm_status check_which(int which)
 {
   return (which >= 0 && which < HARM_NARR) ? HARM_OK : hh::set_error(HARM_EINVAL, "bad array index %d", which);
@@ -1243,17 +1003,7 @@ harm_status enq_fluxcalc(harm_handle* h, int which_p, int do_ct)
   const double* pb = h->arr[which_p];
   hipLaunchKernelGGL(k_scal_begin_flux, dim3(1), dim3(1), 0, s, h->ds);
   const int recon = h->cfg.recon, ft = h->cfg.flux;
-#define LAUNCH_FLUX(DIR, F, G)                                                                                     \
-  do {                                                                                                             \
-    const FaceTile& T = h->ft[DIR];                                                                                \
-    const dim3 grid((unsigned)T.nblk), blk(kBlock);                                                                \
-    if (recon == HARM_RECON_WENO5)                                                                                 \
-      hipLaunchKernelGGL((k_flux<DIR, HARM_RECON_WENO5>), grid, blk, 0, s, d, T, pb, F, G, h->P, ft, do_ct, h->ds); \
-    else if (recon == HARM_RECON_PARA)                                                                             \
-      hipLaunchKernelGGL((k_flux<DIR, HARM_RECON_PARA>), grid, blk, 0, s, d, T, pb, F, G, h->P, ft, do_ct, h->ds);  \
-    else                                                                                                           \
-      hipLaunchKernelGGL((k_flux<DIR, HARM_RECON_DONOR>), grid, blk, 0, s, d, T, pb, F, G, h->P, ft, do_ct, h->ds); \
-  } while (0)
+#define LAUNCH_FLUX(DIR, F, G) launch_flux(DIR, recon, h->ft[DIR], s, d, pb, F, G, h->P, ft, do_ct, h->ds)
   if (d.n1 > 1) { prof_begin(h); LAUNCH_FLUX(1, h->arr[HARM_ARR_F1], h->geom[HARM_LOC_FACE1]); prof_end(h, HARM_K_FLUX1); }
   if (d.n2 > 1) { prof_begin(h); LAUNCH_FLUX(2, h->arr[HARM_ARR_F2], h->geom[HARM_LOC_FACE2]); prof_end(h, HARM_K_FLUX2); }
   if (d.n3 > 1) { prof_begin(h); LAUNCH_FLUX(3, h->arr[HARM_ARR_F3], h->geom[HARM_LOC_FACE3]); prof_end(h, HARM_K_FLUX3); }
@@ -1438,25 +1188,6 @@ void rk_flux_weights(int timeorder, double* w)
 // A whole RK step is captured once in a hipGraph (kernel nodes only) and
 // replayed.
 // ---------------------------------------------------------------------------
-struct GridRef {
-  Dims d;
-  Params P;
-  int fluxtype;
-  double* arr[HARM_NARR];
-  int32_t* pflag;
-  const double* geom[HARM_NLOC];
-  const double* gcorn3;
-  const double* conn;
-  const double* floors;
-  const double* rcent;
-  double* emf;
-  int emf_ni, emf_nj, emf_nk;
-  double* ds;
-  unsigned long long* di;
-  double* facc;
-  FaceTile ft[4];
-};
-
 enum { BS_FLUX1 = 0, BS_FLUX2, BS_FLUX3, BS_EMF, BS_CT1, BS_CT2, BS_CT3, BS_FACC, BS_ZONE, BS_BX1, BS_X2R, BS_X2P,
        BS_BX3, BS_ONE, BS_N };
 
@@ -1468,8 +1199,10 @@ __device__ __forceinline__ void batch_decode(const int* __restrict__ pre, int ng
     if (pre[mid] <= b) lo = mid;
     else hi = mid - 1;
   }
-  gi = lo;
-  lb = b - pre[lo];
+  // the same for every thread of the block: keep it in scalar registers, so
+  // the grid's descriptor is read with scalar loads
+  gi = __builtin_amdgcn_readfirstlane(lo);
+  lb = __builtin_amdgcn_readfirstlane(b - pre[lo]);
 }
 
 #define BATCH_DECODE(REMAP)                                                        \
@@ -1477,34 +1210,25 @@ __device__ __forceinline__ void batch_decode(const int* __restrict__ pre, int ng
   batch_decode(pre, ng, (REMAP) ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x, gi, lb); \
   const GridRef& g = G[gi];
 
-template <int DIR, int RECON>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) void kb_flux(
-    const GridRef* __restrict__ G, const int* __restrict__ pre, int ng, int which, int do_ct)
-{
-  BATCH_DECODE(true)
-  flux_body<DIR, RECON>(g.d, g.ft[DIR], g.arr[which], g.arr[HARM_ARR_F1 + DIR - 1], g.geom[DIR], g.P, g.fluxtype,
-                        do_ct, g.ds, lb);
-}
-
 __global__ __launch_bounds__(kBlock) void kb_emf(const GridRef* __restrict__ G, const int* __restrict__ pre, int ng)
 {
   BATCH_DECODE(true)
-  emf_body(g.d, g.emf_ni, g.emf_nj, g.emf_nk, g.arr[HARM_ARR_F1], g.arr[HARM_ARR_F2], g.arr[HARM_ARR_F3],
-           g.geom[HARM_LOC_FACE1], g.geom[HARM_LOC_FACE2], g.gcorn3, g.emf, lb);
+  emf_body(g.d, g.emf_ni, g.emf_nj, g.emf_nk, gmem(g.arr[HARM_ARR_F1]), gmem(g.arr[HARM_ARR_F2]), gmem(g.arr[HARM_ARR_F3]),
+           gmem(g.geom[HARM_LOC_FACE1]), gmem(g.geom[HARM_LOC_FACE2]), gmem(g.gcorn3), gmem(g.emf), lb);
 }
 
 template <int DIR>
 __global__ __launch_bounds__(kBlock) void kb_ct(const GridRef* __restrict__ G, const int* __restrict__ pre, int ng)
 {
   BATCH_DECODE(true)
-  ct_body<DIR>(g.d, g.emf_ni, g.emf_nj, g.emf_nk, g.emf, g.arr[HARM_ARR_F1 + DIR - 1], lb);
+  ct_body<DIR>(g.d, g.emf_ni, g.emf_nj, g.emf_nk, gmem(g.emf), gmem(g.arr[HARM_ARR_F1 + DIR - 1]), lb);
 }
 
 __global__ __launch_bounds__(kBlock) void kb_fluxacc_part(const GridRef* __restrict__ G, const int* __restrict__ pre,
                                                           int ng)
 {
   BATCH_DECODE(false)
-  fluxacc_part_body(g.d, g.arr[HARM_ARR_F1], g.facc, lb);
+  fluxacc_part_body(g.d, gmem(g.arr[HARM_ARR_F1]), gmem(g.facc), lb);
 }
 
 __global__ void kb_fluxacc_final(const GridRef* __restrict__ G, double weight)
@@ -1513,9 +1237,10 @@ __global__ void kb_fluxacc_final(const GridRef* __restrict__ G, double weight)
   const int c = threadIdx.x;
   if (c >= 4 || g.d.n1 < 2) return;
   double v = 0.0;
-  for (int b = 0; b < kFaccBlocks; b++) v += g.facc[b * 4 + c];
-  g.ds[DS_FLUXACC + c] += v * g.P.dx[2] * g.P.dx[3] * g.ds[DS_DT] * weight;
+  for (int b = 0; b < kFaccBlocks; b++) v += gmem(g.facc)[b * 4 + c];
+  gmem(g.ds)[DS_FLUXACC + c] += v * g.P.dx[2] * g.P.dx[3] * gmem(g.ds)[DS_DT] * weight;
 }
+
 
 template <int DM, int MODE>
 __global__ __launch_bounds__(kBlock) void kb_update(const GridRef* __restrict__ G, const int* __restrict__ pre, int ng,
@@ -1523,52 +1248,54 @@ __global__ __launch_bounds__(kBlock) void kb_update(const GridRef* __restrict__ 
                                                     double cun0, double cun1, double cun2, int timeorder)
 {
   BATCH_DECODE(true)
-  update_body<DM, MODE>(g.d, g.P, g.arr[which_pi], g.arr[which_pb], g.arr[HARM_ARR_F1], g.arr[HARM_ARR_F2],
-                        g.arr[HARM_ARR_F3], g.arr[HARM_ARR_UI], g.arr[HARM_ARR_UF], g.arr[HARM_ARR_UCUM],
-                        g.geom[HARM_LOC_CENT], g.conn, cuf0, cuf1, cuf2, cun0, cun1, cun2, timeorder, g.ds, lb);
+  update_body<DM, MODE>(g.d, g.P, gmem(g.arr[which_pi]), gmem(g.arr[which_pb]), gmem(g.arr[HARM_ARR_F1]),
+                        gmem(g.arr[HARM_ARR_F2]), gmem(g.arr[HARM_ARR_F3]), gmem(g.arr[HARM_ARR_UI]),
+                        gmem(g.arr[HARM_ARR_UF]), gmem(g.arr[HARM_ARR_UCUM]), gmem(g.geom[HARM_LOC_CENT]),
+                        gmem(g.conn), cuf0, cuf1, cuf2, cun0, cun1, cun2, timeorder, gmem(g.ds), lb);
 }
 
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void kb_invert(const GridRef* __restrict__ G, const int* __restrict__ pre, int ng,
-                                                    int which_pb, int which_pf, int finalstep)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void kb_invert(
+    const GridRef* __restrict__ G, const int* __restrict__ pre, int ng, int which_pb, int which_pf, int finalstep)
 {
   BATCH_DECODE(true)
-  invert_body(g.d, g.P, g.arr[finalstep ? HARM_ARR_UCUM : HARM_ARR_UF], g.arr[which_pb], g.arr[which_pf], g.pflag,
-              g.geom[HARM_LOC_CENT], g.floors, finalstep, g.di, lb);
+  invert_body(g.d, g.P, gmem(g.arr[finalstep ? HARM_ARR_UCUM : HARM_ARR_UF]), gmem(g.arr[which_pb]),
+              gmem(g.arr[which_pf]), gmem(g.pflag), gmem(g.geom[HARM_LOC_CENT]), gmem(g.floors), finalstep,
+              gmem(g.di), lb);
 }
 
 __global__ __launch_bounds__(kBlock) void kb_post_fixup(const GridRef* __restrict__ G, const int* __restrict__ pre,
                                                         int ng, int which)
 {
   BATCH_DECODE(false)
-  post_fixup_body(g.d, g.P, g.arr[which], g.arr[which], g.pflag, g.geom[HARM_LOC_CENT], lb);
+  post_fixup_body(g.d, g.P, gmem(g.arr[which]), gmem(g.arr[which]), gmem(g.pflag), gmem(g.geom[HARM_LOC_CENT]), lb);
 }
 
 __global__ __launch_bounds__(kBlock) void kb_bound_x1(const GridRef* __restrict__ G, const int* __restrict__ pre,
                                                       int ng, int which)
 {
   BATCH_DECODE(false)
-  bound_x1_body(g.d, g.P, g.arr[which], g.geom[HARM_LOC_CENT], lb);
+  bound_x1_body(g.d, g.P, gmem(g.arr[which]), gmem(g.geom[HARM_LOC_CENT]), lb);
 }
 
 __global__ __launch_bounds__(kBlock) void kb_x2_reflect(const GridRef* __restrict__ G, const int* __restrict__ pre,
                                                         int ng, int which, int side)
 {
   BATCH_DECODE(false)
-  x2_reflect_body(g.d, g.arr[which], side, lb);
+  x2_reflect_body(g.d, gmem(g.arr[which]), side, lb);
 }
 
 __global__ __launch_bounds__(kBlock) void kb_x2_poledeath(const GridRef* __restrict__ G, const int* __restrict__ pre,
                                                           int ng, int which, int side)
 {
   BATCH_DECODE(false)
-  x2_poledeath_body(g.d, g.P, g.arr[which], g.geom[HARM_LOC_CENT], g.rcent, side, lb);
+  x2_poledeath_body(g.d, g.P, gmem(g.arr[which]), gmem(g.geom[HARM_LOC_CENT]), gmem(g.rcent), side, lb);
 }
 
 __global__ __launch_bounds__(kBlock) void kb_bound_x3(const GridRef* __restrict__ G, const int* __restrict__ pre,
                                                       int ng, int which)
 {
   BATCH_DECODE(false)
-  bound_x3_body(g.d, g.arr[which], lb);
+  bound_x3_body(g.d, gmem(g.arr[which]), lb);
 }
 
 // one thread per grid
@@ -1691,16 +1418,10 @@ harm_status enq_batch_substep(harm_batch* b, int to, int nto, const double CUf[4
   const unsigned g1 = (unsigned)((ng + 63) / 64);
   hipLaunchKernelGGL(kb_scal, dim3(g1), dim3(64), 0, s, G, ng, 0);
   // fluxes
-#define BFLUX(DIR)                                                                                                  \
-  if (b->ntot[BS_FLUX1 + DIR - 1] > 0) {                                                                            \
-    const dim3 gr((unsigned)b->ntot[BS_FLUX1 + DIR - 1]);                                                           \
-    if (b->recon == HARM_RECON_WENO5)                                                                               \
-      hipLaunchKernelGGL((kb_flux<DIR, HARM_RECON_WENO5>), gr, blk, 0, s, G, P(BS_FLUX1 + DIR - 1), ng, pbb[to], 1); \
-    else if (b->recon == HARM_RECON_PARA)                                                                           \
-      hipLaunchKernelGGL((kb_flux<DIR, HARM_RECON_PARA>), gr, blk, 0, s, G, P(BS_FLUX1 + DIR - 1), ng, pbb[to], 1);  \
-    else                                                                                                            \
-      hipLaunchKernelGGL((kb_flux<DIR, HARM_RECON_DONOR>), gr, blk, 0, s, G, P(BS_FLUX1 + DIR - 1), ng, pbb[to], 1); \
-  }
+#define BFLUX(DIR)                                                                                     \
+  if (b->ntot[BS_FLUX1 + DIR - 1] > 0)                                                                  \
+    launch_flux_batch(DIR, b->recon, (unsigned)b->ntot[BS_FLUX1 + DIR - 1], s, G, P(BS_FLUX1 + DIR - 1), ng, \
+                      pbb[to], 1);
   BFLUX(1)
   BFLUX(2)
   BFLUX(3)
@@ -1826,6 +1547,7 @@ extern "C" harm_status harm_destroy(harm_handle* h)
   if (!h) return HARM_OK;
   // teardown: errors here are not actionable, the handle is gone either way
   if (h->stream) (void)hipStreamSynchronize(h->stream);
+  drop_step_graph(h);
   for (int a = 0; a < HARM_NARR; a++) (void)hipFree(h->arr[a]);
   for (int l = 0; l < HARM_NLOC; l++) (void)hipFree(h->geom[l]);
   for (auto& r : h->evrec) {
@@ -2069,18 +1791,47 @@ extern "C" harm_status harm_advance(harm_handle* h, int32_t timeorder, double* n
   return HARM_OK;
 }
 
-extern "C" harm_status harm_step(harm_handle* h, int32_t nsteps)
+namespace {
+harm_status enq_step(harm_handle* h)
 {
-  HH_TRY(check_handle(h));
   double CUf[4][4], Cunew[4][4], w[4];
   const int nto = rk_tables(h->cfg.timeorder, CUf, Cunew);
   rk_flux_weights(h->cfg.timeorder, w);
-  for (int n = 0; n < nsteps; n++) {
-    hipLaunchKernelGGL(k_scal_begin_step, dim3(1), dim3(1), 0, h->stream, h->ds, h->di);
-    for (int to = 0; to < nto; to++) HH_TRY(enq_substep(h, to, nto, CUf, Cunew, w, true, true));
-    hipLaunchKernelGGL(k_scal_end_step, dim3(1), dim3(1), 0, h->stream, h->ds, h->P.safe);
-  }
+  hipLaunchKernelGGL(k_scal_begin_step, dim3(1), dim3(1), 0, h->stream, h->ds, h->di);
+  for (int to = 0; to < nto; to++) HH_TRY(enq_substep(h, to, nto, CUf, Cunew, w, true, true));
+  hipLaunchKernelGGL(k_scal_end_step, dim3(1), dim3(1), 0, h->stream, h->ds, h->P.safe);
   return hh::check_launch("harm_step");
+}
+}  // namespace
+
+// A step is a fixed sequence of ~75 kernel launches with arguments that do not
+// change between steps, so it is captured once in a hipGraph (kernel nodes
+// only) and replayed; with kernel timing enabled the launches go out one by one
+// so that the timing events can bracket them.
+extern "C" harm_status harm_step(harm_handle* h, int32_t nsteps)
+{
+  HH_TRY(check_handle(h));
+  if (nsteps < 0) return hh::set_error(HARM_EINVAL, "harm_step: nsteps < 0");
+  if (h->prof) {
+    for (int n = 0; n < nsteps; n++) HH_TRY(enq_step(h));
+    return HARM_OK;
+  }
+  if (nsteps == 0) return HARM_OK;
+  if (!h->sexec) {
+    HH_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    const harm_status st = enq_step(h);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(h->stream, &g);
+    if (st != HARM_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return st;
+    }
+    HH_HIP(e);
+    h->sgraph = g;
+    HH_HIP(hipGraphInstantiate(&h->sexec, h->sgraph, nullptr, nullptr, 0));
+  }
+  for (int n = 0; n < nsteps; n++) HH_HIP(hipGraphLaunch(h->sexec, h->stream));
+  return HARM_OK;
 }
 
 extern "C" harm_status harm_get_boundary_fluxes(harm_handle* h, double out[4])
@@ -2184,18 +1935,7 @@ extern "C" harm_status harm_tune(harm_handle* h, int32_t reps, int32_t td_out[4]
       const FaceTile T = plan_face_tile(d, dir, td);
       double* F = h->arr[HARM_ARR_F1 + dir - 1];
       const double* G = h->geom[dir];
-      auto launch = [&]() {
-        const dim3 grid((unsigned)T.nblk), blk(kBlock);
-#define TUNE_LAUNCH(DIR)                                                                                           \
-  if (recon == HARM_RECON_WENO5)                                                                                   \
-    hipLaunchKernelGGL((k_flux<DIR, HARM_RECON_WENO5>), grid, blk, 0, s, d, T, pb, F, G, h->P, ft, 1, h->ds);       \
-  else if (recon == HARM_RECON_PARA)                                                                               \
-    hipLaunchKernelGGL((k_flux<DIR, HARM_RECON_PARA>), grid, blk, 0, s, d, T, pb, F, G, h->P, ft, 1, h->ds);        \
-  else                                                                                                             \
-    hipLaunchKernelGGL((k_flux<DIR, HARM_RECON_DONOR>), grid, blk, 0, s, d, T, pb, F, G, h->P, ft, 1, h->ds);
-        if (dir == 2) { TUNE_LAUNCH(2) } else { TUNE_LAUNCH(3) }
-#undef TUNE_LAUNCH
-      };
+      auto launch = [&]() { launch_flux(dir, recon, T, s, d, pb, F, G, h->P, ft, 1, h->ds); };
       launch();  // warm
       (void)hipEventRecord(e0, s);
       for (int r = 0; r < reps; r++) launch();
@@ -2212,6 +1952,7 @@ extern "C" harm_status harm_tune(harm_handle* h, int32_t reps, int32_t td_out[4]
     if (st == HARM_OK && best_td) {
       h->tile_td[dir] = best_td;
       h->ft[dir] = plan_face_tile(d, dir, best_td);
+      drop_step_graph(h);  // the captured step holds the old tile shapes
     }
   }
   (void)hipEventDestroy(e0);

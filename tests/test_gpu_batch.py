@@ -123,3 +123,25 @@ def test_batch_members_match_oracle(scheme):
         assert err <= 1e-12
         assert abs(g.time()[1] - dto) <= 1e-12 * dto
     B.close()
+
+
+def test_single_grid_graph_replays_equal_eager():
+    """harm_step replays a captured graph unless kernel timing is on; both
+    paths must give the same bits, replay after replay, with host work
+    between replays."""
+    c = H.default_config(40, 32, 8, recon="weno5", flux="hll", timeorder=3)
+    A, E = Grid(c), Grid(c)
+    A.init_torus(seed=5)
+    ndt = A.fluxcalc("p")
+    A.set_time(0.0, 0.5 / sum(1.0 / x for x in ndt))
+    E.set("p", A.get("p"))
+    E.set_time(*A.time())
+    E.profile(True)
+    for n in range(4):
+        A.step(1)
+        E.step(1)
+        assert np.array_equal(A.get("p"), E.get("p")), n
+        assert A.time() == E.time()
+        assert np.array_equal(np.array(A.boundary_fluxes()), np.array(E.boundary_fluxes()))
+    A.close()
+    E.close()
