@@ -87,18 +87,17 @@ __device__ __forceinline__ void para4(const double* y, double& lout, double& rou
 }
 
 // limit_weight (reconstructeno.c:3015) for three weights: zero below 0.2/3,
-// a linear ramp to 1/3, flat to 1.2/3, then linear to 1 at w = 1.  The
-// constant quotients are folded into multipliers.
+// a linear ramp to 1/3, flat to 1.2/3, then linear to 1 at w = 1.  Written
+// without branches as the sum of a clamped first ramp and a one-sided second
+// ramp (the two ramps never overlap: w2 < w3).
 __device__ __forceinline__ double limit_weight3(double w)
 {
   constexpr double avg = 1. / 3.;
   constexpr double w1 = 0.2 * avg, w2 = (1.0 - 0.2) * avg, w3 = (1.0 + 0.2) * avg, w4 = 1.0;
-  constexpr double k12 = 1.0 / (w2 - w1), k34 = (w4 - avg) / (w4 - w3);
-  if (w < w1) return 0.0;
-  if (w < w2) return (avg * (w - w1)) * k12;
-  if (w < w3) return avg;
-  return (w - w3) * k34 + avg;
+  constexpr double k12 = avg / (w2 - w1), k34 = (w4 - avg) / (w4 - w3);
+  return fmin(fmax((w - w1) * k12, 0.0), avg) + fmax((w - w3) * k34, 0.0);
 }
+
 
 // WENO5 centre-to-edge reconstruction of one cell from point values
 // u[-2..2] (reconstructeno.c:1693-2030 weights, :2211-2330 indicators, :3015
@@ -125,23 +124,26 @@ __device__ __forceinline__ void weno5(const double* u, double& lout, double& rou
   double b0 = (13. / 12. * d20 * d20 + 1. / 4. * d10 * d10) + err;
   double b1 = (13. / 12. * d21 * d21 + 1. / 4. * d11 * d11) + err;
   double b2 = (13. / 12. * d22 * d22 + 1. / 4. * d12 * d12) + err;
-  const double isb = 1.0 / ((b0 + b2) + b1 + DBL_MIN);
+  const double isb = recip_nr((b0 + b2) + b1 + DBL_MIN);  // >= DBL_MIN
   const double t0 = (b0 * isb + EPS) * (b0 * isb + EPS);
   const double t1 = (b1 * isb + EPS) * (b1 * isb + EPS);
   const double t2 = (b2 * isb + EPS) * (b2 * isb + EPS);
   const double q0 = t1 * t2, q1 = t0 * t2, q2 = t0 * t1;
-  const double iq = 1.0 / ((q0 + q2) + q1);
+  const double iq = recip_nr((q0 + q2) + q1);  // >= EPS^4
   const double w0 = limit_weight3(q0 * iq), w1 = limit_weight3(q1 * iq), w2 = limit_weight3(q2 * iq);
   const double wr0 = w0 * (5.0 / 16.0), wr1 = w1 * (5.0 / 8.0), wr2 = w2 * (1.0 / 16.0);
   const double wl0 = w0 * (1.0 / 16.0), wl1 = w1 * (5.0 / 8.0), wl2 = w2 * (5.0 / 16.0);
-  const double qr0 = (3 * u[0] + 6 * u[1] - u[2]) * 0.125;
-  const double qr1 = (-u[-1] + 6 * u[0] + 3 * u[1]) * 0.125;
-  const double qr2 = (3 * u[-2] - 10 * u[-1] + 15 * u[0]) * 0.125;
-  const double ql0 = (15 * u[0] - 10 * u[1] + 3 * u[2]) * 0.125;
-  const double ql1 = (3 * u[-1] + 6 * u[0] - u[1]) * 0.125;
-  const double ql2 = (-u[-2] + 6 * u[-1] + 3 * u[0]) * 0.125;
-  rout = ((wr0 * qr0 + wr2 * qr2) + wr1 * qr1) / ((wr0 + wr2) + wr1);
-  lout = ((wl0 * ql0 + wl2 * ql2) + wl1 * ql1) / ((wl0 + wl2) + wl1);
+  // the sub-stencil edge values times 8 (the 1/8 is applied with the
+  // normalisation)
+  const double qr0 = 3 * u[0] + 6 * u[1] - u[2];
+  const double qr1 = -u[-1] + 6 * u[0] + 3 * u[1];
+  const double qr2 = 3 * u[-2] - 10 * u[-1] + 15 * u[0];
+  const double ql0 = 15 * u[0] - 10 * u[1] + 3 * u[2];
+  const double ql1 = 3 * u[-1] + 6 * u[0] - u[1];
+  const double ql2 = -u[-2] + 6 * u[-1] + 3 * u[0];
+  // one limited weight is >= 1/3, so each edge's weight sum is >= 1/48
+  rout = ((wr0 * qr0 + wr2 * qr2) + wr1 * qr1) * (0.125 * recip_nr((wr0 + wr2) + wr1));
+  lout = ((wl0 * ql0 + wl2 * ql2) + wl1 * ql1) * (0.125 * recip_nr((wl0 + wl2) + wl1));
 }
 
 template <int RECON>

@@ -53,6 +53,20 @@ __device__ __forceinline__ const double* fresh(const double* p)
   return (const double*)g;
 }
 
+// 1/x for |x| in the normal range and far from overflow: the hardware
+// reciprocal estimate refined by two Newton steps (within an ulp of the IEEE
+// quotient, without the scaling and special-value steps of the IEEE division
+// sequence, which such an x never needs).  Used only by the face-flux
+// kernels (the FAST paths below), whose results are compared with a
+// tolerance; everything bit-exact divides.
+__device__ __forceinline__ double recip_nr(double x)
+{
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(fma(-x, r, 1.0), r, r);
+  r = fma(fma(-x, r, 1.0), r, r);
+  return r;
+}
+
 // Load a record stored SoA with field stride `fs` starting at `base`.
 __device__ __forceinline__ void load_geom(const double* __restrict__ tab, long base, long fs, Geom& g)
 {
@@ -86,10 +100,19 @@ __device__ __forceinline__ void raise(const double* v, const Geom& g, double* w)
 struct State {
   double ucon[4], ucov[4];
   double gamma, qsq, pressure, bsq, p1ud0;
+  double iu0;  // 1/u^0, set by get_state<true> only
 };
+
+// The functions below take FAST = true only in the face-flux kernels: their
+// divisions by quantities that are positive and normal by construction
+// (alpha, 1 + gamma alpha, u^0, the characteristic quadratic's leading
+// coefficient, the fast-speed denominator >= SMALL) become multiplications by
+// recip_nr, and 1/u^0 is formed once per state.  FAST = false is the
+// reference's operation order, bit for bit.
 
 // get_state for VELREL4 primitives (ucon_calc_rel4vel, compute_1plusud0_rel4vel,
 // bsq_calc_fromq_rel4vel, ideal-gas pressure).
+template <bool FAST = false>
 __device__ __forceinline__ void get_state(const double* pr, const Geom& g, double gam, State& q)
 {
   const double u1 = pr[HARM_U1], u2 = pr[HARM_U2], u3 = pr[HARM_U3];
@@ -99,7 +122,7 @@ __device__ __forceinline__ void get_state(const double* pr, const Geom& g, doubl
   const double gamma = sqrt(1. + qsq);
   q.qsq = qsq;
   q.gamma = gamma;
-  q.ucon[0] = gamma / g.alpha;
+  q.ucon[0] = FAST ? gamma * recip_nr(g.alpha) : gamma / g.alpha;
   q.ucon[1] = u1 - q.ucon[0] * g.beta1;
   q.ucon[2] = u2 - q.ucon[0] * g.beta2;
   q.ucon[3] = u3 - q.ucon[0] * g.beta3;
@@ -109,7 +132,8 @@ __device__ __forceinline__ void get_state(const double* pr, const Geom& g, doubl
   ud0tilde += u2 * gco(g, 0, 2);
   ud0tilde += u3 * gco(g, 0, 3);
   const double alpha = g.alpha, alphasq = alpha * alpha;
-  q.p1ud0 = ud0tilde + (g.gpert_tt - alphasq * (g.bsqoa2 + qsq)) / (1.0 + gamma * alpha);
+  q.p1ud0 = FAST ? ud0tilde + (g.gpert_tt - alphasq * (g.bsqoa2 + qsq)) * recip_nr(1.0 + gamma * alpha)
+                 : ud0tilde + (g.gpert_tt - alphasq * (g.bsqoa2 + qsq)) / (1.0 + gamma * alpha);
   const double Bcon1 = pr[HARM_B1], Bcon2 = pr[HARM_B2], Bcon3 = pr[HARM_B3];
   double Bcov[4];
 #pragma unroll
@@ -123,13 +147,19 @@ __device__ __forceinline__ void get_state(const double* pr, const Geom& g, doubl
   Bsq += Bcon1 * Bcov[1]; Bsq += Bcon2 * Bcov[2]; Bsq += Bcon3 * Bcov[3];
   udotB += Bcon1 * q.ucov[1]; udotB += Bcon2 * q.ucov[2]; udotB += Bcon3 * q.ucov[3];
   double bsq = (Bsq + udotB * udotB);
-  bsq *= 1.0 / (q.ucon[0] * q.ucon[0]);
+  if (FAST) {
+    q.iu0 = recip_nr(q.ucon[0]);
+    bsq *= q.iu0 * q.iu0;
+  } else {
+    bsq *= 1.0 / (q.ucon[0] * q.ucon[0]);
+  }
   q.bsq = bsq;
   q.pressure = (gam - 1.0) * pr[HARM_UU];
 }
 
 // One row T^dir_mu of the rest-mass-removed stress tensor: matter + EM parts
 // kept separate so callers can add them in the reference order.
+template <bool FAST = false>
 __device__ __forceinline__ void mhd_parts(const double* pr, int dir, const Geom& g, const State& q, double* ma, double* em)
 {
   const double rho = pr[HARM_RHO], u = pr[HARM_UU], P = q.pressure;
@@ -156,24 +186,31 @@ __device__ __forceinline__ void mhd_parts(const double* pr, int dir, const Geom&
   for (int mu = 0; mu < 4; mu++)
     em[mu] = (Bsq * ud * (q.ucov[mu]) - Bd * Bcov[mu] - udotB * (Bd * (q.ucov[mu]) + Bcov[mu] * ud));
   em[dir] += 0.5 * (Bsq + udotB * udotB);
-  const double ogs = 1.0 / ((q.ucon[0]) * (q.ucon[0]));
+  const double ogs = FAST ? q.iu0 * q.iu0 : 1.0 / ((q.ucon[0]) * (q.ucon[0]));
 #pragma unroll
   for (int mu = 0; mu < 4; mu++) em[mu] *= ogs;
 }
 
 // primtoflux(UEVOLVE): flux (dir>0) or conserved state (dir=0), x gdet.
+template <bool FAST = false>
 __device__ __forceinline__ void primtoflux(const double* pr, const State& q, int dir, const Geom& g, double* F)
 {
   double ma[4], em[4], fl[HARM_NPR];
   fl[HARM_RHO] = pr[HARM_RHO] * q.ucon[dir];
-  mhd_parts(pr, dir, g, q, ma, em);
+  mhd_parts<FAST>(pr, dir, g, q, ma, em);
 #pragma unroll
   for (int j = 0; j < 4; j++) fl[HARM_UU + j] = ma[j] + em[j];
   if (dir > 0) {
     const double Bd = pr[HARM_B1 + dir - 1];
-    fl[HARM_B1] = (pr[HARM_B1] * q.ucon[dir] - Bd * q.ucon[1]) / q.ucon[0];
-    fl[HARM_B2] = (pr[HARM_B2] * q.ucon[dir] - Bd * q.ucon[2]) / q.ucon[0];
-    fl[HARM_B3] = (pr[HARM_B3] * q.ucon[dir] - Bd * q.ucon[3]) / q.ucon[0];
+    if (FAST) {
+      fl[HARM_B1] = (pr[HARM_B1] * q.ucon[dir] - Bd * q.ucon[1]) * q.iu0;
+      fl[HARM_B2] = (pr[HARM_B2] * q.ucon[dir] - Bd * q.ucon[2]) * q.iu0;
+      fl[HARM_B3] = (pr[HARM_B3] * q.ucon[dir] - Bd * q.ucon[3]) * q.iu0;
+    } else {
+      fl[HARM_B1] = (pr[HARM_B1] * q.ucon[dir] - Bd * q.ucon[1]) / q.ucon[0];
+      fl[HARM_B2] = (pr[HARM_B2] * q.ucon[dir] - Bd * q.ucon[2]) / q.ucon[0];
+      fl[HARM_B3] = (pr[HARM_B3] * q.ucon[dir] - Bd * q.ucon[3]) / q.ucon[0];
+    }
   } else {
     fl[HARM_B1] = pr[HARM_B1];
     fl[HARM_B2] = pr[HARM_B2];
@@ -184,6 +221,7 @@ __device__ __forceinline__ void primtoflux(const double* pr, const State& q, int
 }
 
 // vchar + simplefast: fast magnetosonic speeds along dir.
+template <bool FAST = false>
 __device__ __forceinline__ void vchar(const double* pr, const State& q, int dir, const Geom& g, double gam,
                                       double& vmax, double& vmin)
 {
@@ -204,7 +242,7 @@ __device__ __forceinline__ void vchar(const double* pr, const State& q, int dir,
   if (cs2 >= 1.0) cs2 = 1.0 - D_NUMEPSILON * 10.0;
   else if (cs2 <= 0.0) cs2 = D_SMALL;
   const double EF = D_SMALL + fabs(bsq + WW);
-  const double va2 = bsq / EF;
+  const double va2 = FAST ? bsq * recip_nr(EF) : bsq / EF;
   double cms2 = cs2 + va2 - cs2 * va2;
   if (cms2 < 0.0) cms2 = 0.0;
   if (cms2 > 1.0) cms2 = 1.0;
@@ -219,8 +257,15 @@ __device__ __forceinline__ void vchar(const double* pr, const State& q, int dir,
   double discr = 4.0 * cms2 * ((AB * AB - Asq * Bsq) * cms2 + (2.0 * AB * Au * Bu - Asq * Bu2 - Bsq * Au2) * (cms2 - 1.0));
   if (discr < 0.0) discr = 0.0;
   discr = sqrt(discr);
-  const double vm = -(-B + discr) / (2. * A);
-  const double vp = -(-B - discr) / (2. * A);
+  double vm, vp;
+  if (FAST) {  // A >= min(u0^2, -g^00) > 0
+    const double i2A = recip_nr(2. * A);
+    vm = -(-B + discr) * i2A;
+    vp = -(-B - discr) * i2A;
+  } else {
+    vm = -(-B + discr) / (2. * A);
+    vp = -(-B - discr) / (2. * A);
+  }
   if (vp < vm) { vmax = vm; vmin = vp; }
   else { vmax = vp; vmin = vm; }
 }

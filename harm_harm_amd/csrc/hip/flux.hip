@@ -32,17 +32,17 @@ __device__ __forceinline__ double face_riemann(double* p_l, double* p_r, const G
   double cmaxl, cminl, cmaxr, cminr;
   {
     State q;
-    hd::get_state(p_l, G, P.gam, q);
-    hd::primtoflux(p_l, q, DIR, G, F_l);
-    hd::primtoflux(p_l, q, 0, G, U_l);
-    hd::vchar(p_l, q, DIR, G, P.gam, cmaxl, cminl);
+    hd::get_state<true>(p_l, G, P.gam, q);
+    hd::primtoflux<true>(p_l, q, DIR, G, F_l);
+    hd::primtoflux<true>(p_l, q, 0, G, U_l);
+    hd::vchar<true>(p_l, q, DIR, G, P.gam, cmaxl, cminl);
   }
   {
     State q;
-    hd::get_state(p_r, G, P.gam, q);
-    hd::primtoflux(p_r, q, DIR, G, F_r);
-    hd::primtoflux(p_r, q, 0, G, U_r);
-    hd::vchar(p_r, q, DIR, G, P.gam, cmaxr, cminr);
+    hd::get_state<true>(p_r, G, P.gam, q);
+    hd::primtoflux<true>(p_r, q, DIR, G, F_r);
+    hd::primtoflux<true>(p_r, q, 0, G, U_r);
+    hd::vchar<true>(p_r, q, DIR, G, P.gam, cmaxr, cminr);
   }
   double lmin = fmin(cminl, cminr), lmax = fmax(cmaxl, cmaxr);
   lmin = fabs(fmax(0., -lmin));
@@ -102,9 +102,12 @@ __device__ __forceinline__ void flux_body(Dims d, FaceTile T, const double* __re
     }
     ncell = (T.td + 1) * T.tc;
   }
-  for (int it = threadIdx.x; it < ncell * HARM_NPR; it += kBlock) {
-    const int v = it / ncell, slot = it - v * ncell;
-    int i, j, k;
+  // storage offset of every cell of the tile (-1: outside the face box),
+  // decoded once per cell so the loop over (variable, cell) items below needs
+  // no integer division
+  __shared__ int s_off[kCellMax];
+  for (int slot = threadIdx.x; slot < ncell; slot += kBlock) {
+    int i, j, k, off = -1;
     if (DIR == 1) {
       const int g = g_lo + slot;
       const int row = g / rowc;
@@ -112,22 +115,37 @@ __device__ __forceinline__ void flux_body(Dims d, FaceTile T, const double* __re
       const int rj = row / b.nk;
       k = b.ks + (row - rj * b.nk);
       j = b.js + rj;
+      off = (int)hd::cidx(d, i, j, k);
     } else {
       const int dd = slot / T.tc, cc = slot - dd * T.tc;
       i = T.i0 + bc * T.tc + cc;
       const int pd = bd * T.td - 1 + dd;  // sweep index of the cell relative to the first face
-      if (i < b.is || i >= b.is + b.ni || pd > T.nd - 1) continue;
-      if (DIR == 2) { k = b.ks + pl; j = b.js + pd; }
-      else { j = b.js + pl; k = b.ks + pd; }
+      if (i >= b.is && i < b.is + b.ni && pd <= T.nd - 1) {
+        if (DIR == 2) { k = b.ks + pl; j = b.js + pd; }
+        else { j = b.js + pl; k = b.ks + pd; }
+        off = (int)hd::cidx(d, i, j, k);
+      }
     }
-    const double* a = pb + (long)v * d.ncell + hd::cidx(d, i, j, k);
-    double y[5];
+    s_off[slot] = off;
+  }
+  __syncthreads();
+  // item it = v * ncell + slot, advanced by kBlock per pass
+  int v = 0, slot = threadIdx.x;
+  while (slot >= ncell) { slot -= ncell; v++; }
+  while (v < HARM_NPR) {
+    const int off = s_off[slot];
+    if (off >= 0) {
+      const double* a = pb + (long)v * d.ncell + off;
+      double y[5];
 #pragma unroll
-    for (int m = 0; m < 5; m++) y[m] = a[(m - 2) * st];
-    double l, r;
-    hd::recon_cell<RECON>(y + 2, l, r);
-    s_l[v * kCellMax + slot] = l;
-    s_r[v * kCellMax + slot] = r;
+      for (int m = 0; m < 5; m++) y[m] = a[(m - 2) * st];
+      double l, r;
+      hd::recon_cell<RECON>(y + 2, l, r);
+      s_l[v * kCellMax + slot] = l;
+      s_r[v * kCellMax + slot] = r;
+    }
+    slot += kBlock;
+    while (slot >= ncell) { slot -= ncell; v++; }
   }
   __syncthreads();
 
