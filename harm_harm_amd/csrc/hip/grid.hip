@@ -575,13 +575,10 @@ __global__ __launch_bounds__(kBlock) void k_bound_x1(Dims d, Params P, double* _
   bound_x1_body(d, P, p, gcent, blockIdx.x);
 }
 
-__device__ __forceinline__ void x2_reflect_body(Dims d, double* __restrict__ p, int side, const int blk)
+// one ghost zone of the polar reflection: ghost row j = -ng2 + g (side 0) or
+// n2 + g (side 1) of column col = (i + ng1) + ns1 * k
+__device__ __forceinline__ void x2_reflect_zone(const Dims& d, double* __restrict__ p, int side, int g, long col)
 {
-  const long t = (long)blk * kBlock + threadIdx.x;
-  const long ncol = (long)d.ns1 * d.n3;
-  if (t >= ncol * d.ng2) return;
-  const int g = (int)(t / ncol);
-  const long col = t % ncol;
   const int i = (int)(col % d.ns1) - d.ng1, k = (int)(col / d.ns1);
   const int n2 = d.n2, n3 = d.n3;
   const int rk = (n3 > 1) ? (k + n3 / 2) % n3 : k;
@@ -597,19 +594,29 @@ __device__ __forceinline__ void x2_reflect_body(Dims d, double* __restrict__ p, 
   }
 }
 
+__device__ __forceinline__ void x2_reflect_body(Dims d, double* __restrict__ p, int side, const int blk)
+{
+  const long t = (long)blk * kBlock + threadIdx.x;
+  const long ncol = (long)d.ns1 * d.n3;
+  if (t >= ncol * d.ng2) return;
+  x2_reflect_zone(d, p, side, (int)(t / ncol), t % ncol);
+}
+
 __global__ __launch_bounds__(kBlock) void k_x2_reflect(Dims d, double* __restrict__ p, int side)
 {
   x2_reflect_body(d, p, side, blockIdx.x);
 }
 
-__device__ __forceinline__ void x2_poledeath_body(Dims d, Params P, double* __restrict__ p,
-                                                         const double* __restrict__ gcent,
-                                                         const double* __restrict__ rcent, int side, const int blk)
+// pole death of one column col = (i + ng1) + ns1 * k at one pole: the first
+// active row and the first ghost row are rebuilt from the second active row.
+// The field components of the ghost row are kept as the reflection left
+// them; with mirror_ghost they are taken from the mirrored first active row
+// directly (the reflection of that ghost row has not run).
+__device__ __forceinline__ void x2_poledeath_col(const Dims& d, const Params& P, double* __restrict__ p,
+                                                 const double* __restrict__ gcent, const double* __restrict__ rcent,
+                                                 int side, long col, bool mirror_ghost = false)
 {
-  const long t = (long)blk * kBlock + threadIdx.x;
-  const long ncol = (long)d.ns1 * d.n3;
-  if (t >= ncol) return;
-  const int i = (int)(t % d.ns1) - d.ng1, k = (int)(t / d.ns1);
+  const int i = (int)(col % d.ns1) - d.ng1, k = (int)(col / d.ns1);
   const int n2 = d.n2, n3 = d.n3;
   const long nc = d.ncell;
   const double x2c0 = P.startx[2];
@@ -627,6 +634,15 @@ __device__ __forceinline__ void x2_poledeath_body(Dims d, Params P, double* __re
     double pj[HARM_NPR];
 #pragma unroll
     for (int pl = 0; pl < HARM_NPR; pl++) pj[pl] = p[pl * nc + o];
+    if (mirror_ghost && j == (side == 0 ? js : je)) {
+      // x2_reflect_zone of this ghost row, field components only (pole death
+      // replaces the rest, and never writes the fields of the mirrored row)
+      const int rk = (n3 > 1) ? (k + n3 / 2) % n3 : k;
+      const long so = hd::cidx(d, i, side == 0 ? 0 : n2 - 1, rk);
+      pj[HARM_B1] = p[HARM_B1 * nc + so];
+      pj[HARM_B2] = -p[HARM_B2 * nc + so];
+      pj[HARM_B3] = p[HARM_B3 * nc + so];
+    }
     const double X = x2c0 + (j + 0.5) * P.dx[2];
     for (int pl = HARM_RHO; pl <= HARM_U3; pl++) {
       if (pl == HARM_U2) continue;
@@ -652,6 +668,49 @@ __device__ __forceinline__ void x2_poledeath_body(Dims d, Params P, double* __re
 #pragma unroll
     for (int pl = 0; pl < HARM_NPR; pl++) p[pl * nc + o] = pj[pl];
   }
+}
+
+__device__ __forceinline__ void x2_poledeath_body(Dims d, Params P, double* __restrict__ p,
+                                                         const double* __restrict__ gcent,
+                                                         const double* __restrict__ rcent, int side, const int blk)
+{
+  const long t = (long)blk * kBlock + threadIdx.x;
+  const long ncol = (long)d.ns1 * d.n3;
+  if (t >= ncol) return;
+  x2_poledeath_col(d, P, p, gcent, rcent, side, t);
+}
+
+// Both poles in one pass, for n2 > ng2: the pole-death columns (rows -1, 0
+// and n2-1, n2), then the reflection of the ghost rows beyond the first
+// (-ng2..-2 and n2+1..n2+ng2-1).  The reflection reads rows 1..ng2-1 and
+// n2-ng2..n2-2, pole death reads rows 1 and n2-2 and the field components of
+// rows 0 and n2-1 (which it never writes), none of which this pass writes
+// when n2 > ng2, so the result equals reflect-then-pole-death at one pole and
+// then the other: there pole death overwrites all but the fields of the
+// reflected rows -1 and n2, and those fields are mirrored here directly.
+__device__ __forceinline__ void x2_poles_body(Dims d, Params P, double* __restrict__ p,
+                                              const double* __restrict__ gcent, const double* __restrict__ rcent,
+                                              const int blk)
+{
+  const long t = (long)blk * kBlock + threadIdx.x;
+  const long ncol = (long)d.ns1 * d.n3;
+  if (t < 2 * ncol) {
+    const int side = t >= ncol;
+    x2_poledeath_col(d, P, p, gcent, rcent, side, t - side * ncol, true);
+    return;
+  }
+  const long u = t - 2 * ncol, nside = ncol * (d.ng2 - 1);
+  if (u >= 2 * nside) return;
+  const int side = u >= nside;
+  const long v = u - side * nside;
+  x2_reflect_zone(d, p, side, (int)(v / ncol) + side, v % ncol);
+}
+
+__global__ __launch_bounds__(kBlock) void k_x2_poles(Dims d, Params P, double* __restrict__ p,
+                                                     const double* __restrict__ gcent,
+                                                     const double* __restrict__ rcent)
+{
+  x2_poles_body(d, P, p, gcent, rcent, blockIdx.x);
 }
 
 __global__ __launch_bounds__(kBlock) void k_x2_poledeath(Dims d, Params P, double* __restrict__ p,
@@ -1087,7 +1146,10 @@ harm_status enq_bound(harm_handle* h, int which)
   if (d.n1 > 1)
     hipLaunchKernelGGL(k_bound_x1, dim3(nblocks(2L * d.ng1 * d.n2 * d.n3)), dim3(kBlock), 0, s, d, h->P, p,
                        h->geom[HARM_LOC_CENT]);
-  if (d.n2 > 1) {
+  if (d.n2 > d.ng2) {
+    hipLaunchKernelGGL(k_x2_poles, dim3(nblocks(2L * d.ns1 * d.n3 * d.ng2)), dim3(kBlock), 0, s, d, h->P, p,
+                       h->geom[HARM_LOC_CENT], h->rcent);
+  } else if (d.n2 > 1) {  // so few rows that the two poles' stencils overlap: one pole after the other
     for (int side = 0; side < 2; side++) {
       hipLaunchKernelGGL(k_x2_reflect, dim3(nblocks((long)d.ns1 * d.n3 * d.ng2)), dim3(kBlock), 0, s, d, p, side);
       hipLaunchKernelGGL(k_x2_poledeath, dim3(nblocks((long)d.ns1 * d.n3)), dim3(kBlock), 0, s, d, h->P, p,
@@ -1189,7 +1251,7 @@ void rk_flux_weights(int timeorder, double* w)
 // replayed.
 // ---------------------------------------------------------------------------
 enum { BS_FLUX1 = 0, BS_FLUX2, BS_FLUX3, BS_EMF, BS_CT1, BS_CT2, BS_CT3, BS_FACC, BS_ZONE, BS_BX1, BS_X2R, BS_X2P,
-       BS_BX3, BS_ONE, BS_N };
+       BS_X2M, BS_BX3, BS_ONE, BS_N };
 
 __device__ __forceinline__ void batch_decode(const int* __restrict__ pre, int ng, int b, int& gi, int& lb)
 {
@@ -1291,6 +1353,13 @@ __global__ __launch_bounds__(kBlock) void kb_x2_poledeath(const GridRef* __restr
   x2_poledeath_body(g.d, g.P, gmem(g.arr[which]), gmem(g.geom[HARM_LOC_CENT]), gmem(g.rcent), side, lb);
 }
 
+__global__ __launch_bounds__(kBlock) void kb_x2_poles(const GridRef* __restrict__ G, const int* __restrict__ pre,
+                                                      int ng, int which)
+{
+  BATCH_DECODE(false)
+  x2_poles_body(g.d, g.P, gmem(g.arr[which]), gmem(g.geom[HARM_LOC_CENT]), gmem(g.rcent), lb);
+}
+
 __global__ __launch_bounds__(kBlock) void kb_bound_x3(const GridRef* __restrict__ G, const int* __restrict__ pre,
                                                       int ng, int which)
 {
@@ -1357,6 +1426,7 @@ struct harm_batch {
   int* dpre = nullptr;  // BS_N prefix arrays of ng+1 ints
   int ntot[BS_N] = {0};
   int recon = 0, timeorder = 0, dm = 0;
+  bool x2_merged = true;  // every grid has n2 > ng2 (or no polar boundary): both poles in one launch
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
 };
@@ -1386,6 +1456,8 @@ harm_status batch_build(harm_batch* b)
     cnt[BS_BX1] = e0 ? nblocks(2L * d.ng1 * d.n2 * d.n3) : 0;
     cnt[BS_X2R] = e1 ? nblocks((long)d.ns1 * d.n3 * d.ng2) : 0;
     cnt[BS_X2P] = e1 ? nblocks((long)d.ns1 * d.n3) : 0;
+    cnt[BS_X2M] = e1 ? nblocks(2L * d.ns1 * d.n3 * d.ng2) : 0;
+    if (e1 && d.n2 <= d.ng2) b->x2_merged = false;
     cnt[BS_BX3] = e2 ? nblocks(d.nplane * 2L * d.ng3) : 0;
     cnt[BS_ONE] = 1;
     for (int st = 0; st < BS_N; st++) {
@@ -1463,7 +1535,9 @@ harm_status enq_batch_substep(harm_batch* b, int to, int nto, const double CUf[4
     if (pass == 1) hipLaunchKernelGGL(kb_post_fixup, gz, blk, 0, s, G, P(BS_ZONE), ng, pff[to]);
     if (b->ntot[BS_BX1]) hipLaunchKernelGGL(kb_bound_x1, dim3((unsigned)b->ntot[BS_BX1]), blk, 0, s, G, P(BS_BX1), ng,
                                             pff[to]);
-    if (b->ntot[BS_X2R])
+    if (b->ntot[BS_X2M] && b->x2_merged)
+      hipLaunchKernelGGL(kb_x2_poles, dim3((unsigned)b->ntot[BS_X2M]), blk, 0, s, G, P(BS_X2M), ng, pff[to]);
+    else if (b->ntot[BS_X2R])
       for (int side = 0; side < 2; side++) {
         hipLaunchKernelGGL(kb_x2_reflect, dim3((unsigned)b->ntot[BS_X2R]), blk, 0, s, G, P(BS_X2R), ng, pff[to], side);
         hipLaunchKernelGGL(kb_x2_poledeath, dim3((unsigned)b->ntot[BS_X2P]), blk, 0, s, G, P(BS_X2P), ng, pff[to],
