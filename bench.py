@@ -305,6 +305,81 @@ def batch_bench(args):
     print(json.dumps(out))
 
 
+def corner_divb_max(g, geom_cent):
+    """Largest Toth corner divergence of gdet B over the interior corners
+    (the quantity flux-CT keeps at round-off), relative to max|gdet B| / dx."""
+    import numpy as np
+    c = g.cfg
+    p = g.get("p")  # (ns3, ns2, ns1, 8)
+    ng1, ng2, ng3 = g.ng1, g.ng2, g.ng3
+    gd = geom_cent[..., 21][None, :, :]
+    dx = g.dx
+    n1, n2, n3 = c.n1, c.n2, c.n3
+    sl = lambda a, di, dj, dk: a[ng3 + dk:ng3 + dk + n3 - 1, ng2 + dj:ng2 + dj + n2 - 1, ng1 + di:ng1 + di + n1 - 1]
+    div = 0.0
+    scale = 0.0
+    for m, s in ((0, dx[1]), (1, dx[2]), (2, dx[3])):
+        B = p[..., 5 + m] * gd
+        acc = 0.0
+        for di in (0, 1):
+            for dj in (0, 1):
+                for dk in (0, 1):
+                    sign = [di, dj, dk][m] * 2 - 1
+                    acc = acc + sign * sl(B, di, dj, dk)
+        div = div + acc / (4 * s)
+        scale = max(scale, float(np.abs(sl(B, 0, 0, 0)).max()))
+        del B, acc
+    return float(np.abs(div).max()) / (scale / min(dx[1], dx[2], dx[3]))
+
+
+def large_bench(args):
+    """The 3-D torus at 384x192x192 (14.2 M zones) with the plain step:
+    zone-updates/s, inversion failures, the corner divergence of B and the
+    accumulated accretion sums after the timed steps."""
+    import harm_harm_amd as H
+    from harm_harm_amd.grid import Grid
+    cfg = H.default_config(384, 192, 192, recon="weno5", flux="hll", timeorder=3)
+    g = Grid(cfg)
+    g.init_torus(seed=1)
+    dt0 = initial_dt(g)
+    g.set_time(0.0, dt0)
+    g.step(args.warmup)
+    g.sync()
+    t0 = time.perf_counter()
+    g.step(args.steps)
+    g.sync()
+    el = time.perf_counter() - t0
+    zones = cfg.n1 * cfg.n2 * cfg.n3
+    (_, _), (nstroke, nfail) = g.newton_stats()
+    steps_done = g.nstep()
+    t_sim, dt_now = g.time()
+    divb = corner_divb_max(g, g.geometry(0))
+    fa = g.boundary_fluxes()
+    out = {
+        "metric": "zone-updates/s, 3D FM torus 384x192x192 FP64, plain step",
+        "value": zones * args.steps / el,
+        "unit": "zone-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Fishbone-Moncrief torus initial data)",
+        "config": {"workload": "3D FM torus 384x192x192 MKS Kerr-Schild a=0.9375, plain step",
+                   "recon": "WENO5", "flux": "HLL", "ct": "Toth", "rk": 3, "cour": cfg.cour,
+                   "zones": zones, "sim_time_end": t_sim, "dt": dt_now},
+        "steps_total": steps_done,
+        "newton_iterations_per_zone_update": nstroke / (zones * 3.0 * steps_done),
+        "inversion_failures": nfail,
+        "divb_max_relative": divb,
+        "accretion_sums": {"mdot_in": fa[0], "edot_in": fa[1], "mdot_out": fa[2], "edot_out": fa[3]},
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -314,9 +389,10 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--prof-steps", type=int, default=10, help="steps of the separate per-kernel timing run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["step", "inversion", "batch"], default="step",
+    ap.add_argument("--workload", choices=["step", "inversion", "batch", "large"], default="step",
                     help="step: the 256x128x64 torus RK3 step (default); inversion: the batched-inversion "
-                         "microbench, plain vs MLP-seeded; batch: 64 uneven 2-D tori stepped together")
+                         "microbench, plain vs MLP-seeded; batch: 64 uneven 2-D tori stepped together; "
+                         "large: the 384x192x192 torus with failures, div B and accretion sums")
     ap.add_argument("--ngrids", type=int, default=64, help="batch: number of grids")
     ap.add_argument("--n", type=int, default=1 << 24, help="inversion microbench: zones")
     ap.add_argument("--train-n", type=int, default=1 << 18, help="inversion microbench: training pairs")
@@ -334,6 +410,8 @@ def main():
         return inversion_microbench(args)
     if args.workload == "batch":
         return batch_bench(args)
+    if args.workload == "large":
+        return large_bench(args)
 
     import harm_harm_amd as H
     from harm_harm_amd.grid import Grid

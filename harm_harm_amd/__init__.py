@@ -89,6 +89,7 @@ _OPTIONAL_PROTOS = [
     ("harm_get_pflag", _I, [_P, _P]),
     ("harm_set_time", _I, [_P, _D, _D, _L]),
     ("harm_get_time", _I, [_P, _P, _P, _P]),
+    ("harm_get_newton_stats", _I, [_P, _P, _P]),
     ("harm_init_torus", _I, [_P, _D, _D, _D, _D, _D, ctypes.c_uint32]),
     ("harm_fluxcalc", _I, [_P, _I, _I, _P]),
     ("harm_update", _I, [_P, _I, _I, _P, _P, _D, _I, _P]),

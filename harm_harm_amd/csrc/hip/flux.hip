@@ -132,6 +132,11 @@ __device__ __forceinline__ void flux_body(Dims d, FaceTile T, const double* __re
   // item it = v * ncell + slot, advanced by kBlock per pass
   int v = 0, slot = threadIdx.x;
   while (slot >= ncell) { slot -= ncell; v++; }
+  // One item per pass: every cell is reconstructed by the same instruction
+  // sequence whatever the tile shape, so tile shapes never change results.
+  // (Two items per pass in one basic block measured 1.5 % faster, but the
+  // compiler may contract the two copies differently; likewise no unrolling.)
+#pragma clang loop unroll(disable)
   while (v < HARM_NPR) {
     const int off = s_off[slot];
     if (off >= 0) {

@@ -15,9 +15,11 @@ constexpr int kBlock = 256;  // threads per block of every grid kernel
 constexpr double kBIG = 1.e+100;
 
 // device scalar slots
-enum { DS_DT = 0, DS_T = 1, DS_NDT1 = 2, DS_NDT2 = 3, DS_NDT3 = 4, DS_ACCDT = 5, DS_LASTNDT = 6, DS_FLUXACC = 8,
-       DS_N = 16 };
-enum { DI_NSTROKE = 0, DI_NFAIL = 1, DI_N = 4 };
+enum { DS_DT = 0, DS_T = 1, DS_NDT1 = 2, DS_NDT2 = 3, DS_NDT3 = 4, DS_ACCDT = 5, DS_LASTNDT = 6, DS_NSTEP = 7,
+       DS_FLUXACC = 8, DS_N = 16 };
+// Newton counters: the current step's, and their sums over the steps since
+// the time was last set
+enum { DI_NSTROKE = 0, DI_NFAIL = 1, DI_NSTROKE_SUM = 2, DI_NFAIL_SUM = 3, DI_N = 4 };
 
 struct Params {
   double gam, cour, safe, gammamax, gammafail;

@@ -6,7 +6,7 @@
 // of advance() is one kernel:
 //   k_flux<dir>   reconstruct + Riemann flux per face, fused   (flux.c:852;
 //                 in flux.hip, built with FMA contraction)
-//   k_emf/k_ct    Toth corner EMFs and face B fluxes           (fluxct.c:186)
+//   k_emf/k_ct_all Toth corner EMFs and face B fluxes          (fluxct.c:186)
 //   k_update      U(pi), flux divergence, connection source,
 //                 RK combine and the source dt limit           (advance.c:420)
 //   k_invert      Newton inversion + floors                    (advance.c:748)
@@ -91,39 +91,45 @@ __global__ __launch_bounds__(kBlock) void k_emf(Dims d, int ni, int nj, int nk, 
   emf_body(d, ni, nj, nk, F1, F2, F3, gf1, gf2, gc3, emf, xcd_tile(blockIdx.x, gridDim.x));
 }
 
-template <int DIR>
-__device__ __forceinline__ void ct_body(Dims d, int ni, int nj, int nk, const double* __restrict__ emf,
-                                               double* __restrict__ F, const int blk)
+// The three directions' B fluxes from the corner EMFs (fluxct.c:1106-1161)
+// in one pass: thread (i, j, k) of the union of the three face boxes writes
+// face (i, j, k) of every direction whose box holds it.
+__device__ __forceinline__ void ct_all_body(Dims d, int ni, int nj, int nk, const double* __restrict__ emf,
+                                            double* __restrict__ F1, double* __restrict__ F2,
+                                            double* __restrict__ F3, const int blk)
 {
   const int e0 = d.n1 > 1, e1 = d.n2 > 1, e2 = d.n3 > 1;
-  const int fi = d.n1 + (DIR == 1 ? e0 : 0), fj = d.n2 + (DIR == 2 ? e1 : 0), fk = d.n3 + (DIR == 3 ? e2 : 0);
-  const long t = (long)blk * kBlock + threadIdx.x;
-  if (t >= (long)fi * fj * fk) return;
-  const int i = (int)(t % fi), j = (int)((t / fi) % fj), k = (int)(t / ((long)fi * fj));
-  const long n = (long)ni * nj * nk;
-  auto E = [&](int c, int ii, int jj, int kk) { return emf[c * n + ((long)kk * nj + jj) * ni + ii]; };
+  const int fi = d.n1 + e0, fj = d.n2 + e1, fk = d.n3 + e2;
+  const int t = blk * kBlock + threadIdx.x;
+  if (t >= fi * fj * fk) return;
+  const int r = t / fi, i = t - r * fi, k = r / fj, j = r - k * fj;
+  const int n = ni * nj * nk;
+  auto E = [&](int c, int ii, int jj, int kk) { return emf[c * n + (kk * nj + jj) * ni + ii]; };
   const long o = hd::cidx(d, i, j, k);
   const long nc = d.ncell;
-  if (DIR == 1) {
-    F[HARM_B1 * nc + o] = 0.;
-    F[HARM_B2 * nc + o] = 0.5 * (E(2, i, j, k) + E(2, i, j + e1, k));
-    F[HARM_B3 * nc + o] = -0.5 * (E(1, i, j, k) + E(1, i, j, k + e2));
-  } else if (DIR == 2) {
-    F[HARM_B1 * nc + o] = -0.5 * (E(2, i, j, k) + E(2, i + e0, j, k));
-    F[HARM_B2 * nc + o] = 0.;
-    F[HARM_B3 * nc + o] = 0.5 * (E(0, i, j, k) + E(0, i, j, k + e2));
-  } else {
-    F[HARM_B1 * nc + o] = 0.5 * (E(1, i, j, k) + E(1, i + e0, j, k));
-    F[HARM_B2 * nc + o] = -0.5 * (E(0, i, j, k) + E(0, i, j + e1, k));
-    F[HARM_B3 * nc + o] = 0.;
+  const bool in_i = i < d.n1, in_j = j < d.n2, in_k = k < d.n3;
+  if (e0 && in_j && in_k) {
+    F1[HARM_B1 * nc + o] = 0.;
+    F1[HARM_B2 * nc + o] = 0.5 * (E(2, i, j, k) + E(2, i, j + e1, k));
+    F1[HARM_B3 * nc + o] = -0.5 * (E(1, i, j, k) + E(1, i, j, k + e2));
+  }
+  if (e1 && in_i && in_k) {
+    F2[HARM_B1 * nc + o] = -0.5 * (E(2, i, j, k) + E(2, i + e0, j, k));
+    F2[HARM_B2 * nc + o] = 0.;
+    F2[HARM_B3 * nc + o] = 0.5 * (E(0, i, j, k) + E(0, i, j, k + e2));
+  }
+  if (e2 && in_i && in_j) {
+    F3[HARM_B1 * nc + o] = 0.5 * (E(1, i, j, k) + E(1, i + e0, j, k));
+    F3[HARM_B2 * nc + o] = -0.5 * (E(0, i, j, k) + E(0, i, j + e1, k));
+    F3[HARM_B3 * nc + o] = 0.;
   }
 }
 
-template <int DIR>
-__global__ __launch_bounds__(kBlock) void k_ct(Dims d, int ni, int nj, int nk, const double* __restrict__ emf,
-                                               double* __restrict__ F)
+__global__ __launch_bounds__(kBlock) void k_ct_all(Dims d, int ni, int nj, int nk, const double* __restrict__ emf,
+                                                   double* __restrict__ F1, double* __restrict__ F2,
+                                                   double* __restrict__ F3)
 {
-  ct_body<DIR>(d, ni, nj, nk, emf, F, xcd_tile(blockIdx.x, gridDim.x));
+  ct_all_body(d, ni, nj, nk, emf, F1, F2, F3, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 // ---------------------------------------------------------------------------
@@ -762,13 +768,16 @@ __global__ void k_scal_substep_dt(double* ds)
   const double nd = 1.0 * fmin(wavedt, ds[DS_ACCDT]);
   if (nd < ds[DS_LASTNDT]) ds[DS_LASTNDT] = nd;
 }
-__global__ void k_scal_end_step(double* ds, double safe)
+__global__ void k_scal_end_step(double* ds, unsigned long long* di, double safe)
 {
   double fullndt = ds[DS_LASTNDT];
   const double dt = ds[DS_DT];
   if (fullndt > safe * dt) fullndt = safe * dt;
   ds[DS_T] = ds[DS_T] + dt;
   ds[DS_DT] = fullndt;
+  ds[DS_NSTEP] = ds[DS_NSTEP] + 1.0;
+  di[DI_NSTROKE_SUM] += di[DI_NSTROKE];
+  di[DI_NFAIL_SUM] += di[DI_NFAIL];
 }
 
 // Boundary flux sums through the inner and outer radial faces
@@ -1074,18 +1083,11 @@ harm_status enq_fluxcalc(harm_handle* h, int which_p, int do_ct)
     hipLaunchKernelGGL(k_emf, dim3(nblocks((long)ni * nj * nk)), dim3(kBlock), 0, s, d, ni, nj, nk,
                        h->arr[HARM_ARR_F1], h->arr[HARM_ARR_F2], h->arr[HARM_ARR_F3], h->geom[HARM_LOC_FACE1],
                        h->geom[HARM_LOC_FACE2], h->gcorn3, h->emf);
-    const int e0 = d.n1 > 1, e1 = d.n2 > 1, e2 = d.n3 > 1;
-    if (e0)
-      hipLaunchKernelGGL(k_ct<1>, dim3(nblocks((long)(d.n1 + 1) * d.n2 * d.n3)), dim3(kBlock), 0, s, d, ni, nj, nk,
-                         h->emf, h->arr[HARM_ARR_F1]);
-    if (e1)
-      hipLaunchKernelGGL(k_ct<2>, dim3(nblocks((long)d.n1 * (d.n2 + 1) * d.n3)), dim3(kBlock), 0, s, d, ni, nj, nk,
-                         h->emf, h->arr[HARM_ARR_F2]);
-    if (e2)
-      hipLaunchKernelGGL(k_ct<3>, dim3(nblocks((long)d.n1 * d.n2 * (d.n3 + 1))), dim3(kBlock), 0, s, d, ni, nj, nk,
-                         h->emf, h->arr[HARM_ARR_F3]);
+    const long nct = (long)(d.n1 + (d.n1 > 1)) * (d.n2 + (d.n2 > 1)) * (d.n3 + (d.n3 > 1));
+    hipLaunchKernelGGL(k_ct_all, dim3(nblocks(nct)), dim3(kBlock), 0, s, d, ni, nj, nk, h->emf,
+                       h->arr[HARM_ARR_F1], h->arr[HARM_ARR_F2], h->arr[HARM_ARR_F3]);
     prof_end(h, HARM_K_EMFCT);
-    HH_TRY(hh::check_launch("k_emf/k_ct"));
+    HH_TRY(hh::check_launch("k_emf/k_ct_all"));
   }
   return HARM_OK;
 }
@@ -1250,7 +1252,7 @@ void rk_flux_weights(int timeorder, double* w)
 // A whole RK step is captured once in a hipGraph (kernel nodes only) and
 // replayed.
 // ---------------------------------------------------------------------------
-enum { BS_FLUX1 = 0, BS_FLUX2, BS_FLUX3, BS_EMF, BS_CT1, BS_CT2, BS_CT3, BS_FACC, BS_ZONE, BS_BX1, BS_X2R, BS_X2P,
+enum { BS_FLUX1 = 0, BS_FLUX2, BS_FLUX3, BS_EMF, BS_CT, BS_FACC, BS_ZONE, BS_BX1, BS_X2R, BS_X2P,
        BS_X2M, BS_BX3, BS_ONE, BS_N };
 
 __device__ __forceinline__ void batch_decode(const int* __restrict__ pre, int ng, int b, int& gi, int& lb)
@@ -1279,11 +1281,12 @@ __global__ __launch_bounds__(kBlock) void kb_emf(const GridRef* __restrict__ G, 
            gmem(g.geom[HARM_LOC_FACE1]), gmem(g.geom[HARM_LOC_FACE2]), gmem(g.gcorn3), gmem(g.emf), lb);
 }
 
-template <int DIR>
-__global__ __launch_bounds__(kBlock) void kb_ct(const GridRef* __restrict__ G, const int* __restrict__ pre, int ng)
+__global__ __launch_bounds__(kBlock) void kb_ct_all(const GridRef* __restrict__ G, const int* __restrict__ pre,
+                                                    int ng)
 {
   BATCH_DECODE(true)
-  ct_body<DIR>(g.d, g.emf_ni, g.emf_nj, g.emf_nk, gmem(g.emf), gmem(g.arr[HARM_ARR_F1 + DIR - 1]), lb);
+  ct_all_body(g.d, g.emf_ni, g.emf_nj, g.emf_nk, gmem(g.emf), gmem(g.arr[HARM_ARR_F1]), gmem(g.arr[HARM_ARR_F2]),
+              gmem(g.arr[HARM_ARR_F3]), lb);
 }
 
 __global__ __launch_bounds__(kBlock) void kb_fluxacc_part(const GridRef* __restrict__ G, const int* __restrict__ pre,
@@ -1390,6 +1393,10 @@ __global__ void kb_scal(const GridRef* __restrict__ G, int ng, int what)
     if (fullndt > safe * dt) fullndt = safe * dt;
     ds[DS_T] = ds[DS_T] + dt;
     ds[DS_DT] = fullndt;
+    ds[DS_NSTEP] = ds[DS_NSTEP] + 1.0;
+    unsigned long long* di = G[gi].di;
+    di[DI_NSTROKE_SUM] += di[DI_NSTROKE];
+    di[DI_NFAIL_SUM] += di[DI_NFAIL];
   }
 }
 
@@ -1448,9 +1455,7 @@ harm_status batch_build(harm_batch* b)
     cnt[BS_FLUX2] = e1 ? h->ft[2].nblk : 0;
     cnt[BS_FLUX3] = e2 ? h->ft[3].nblk : 0;
     cnt[BS_EMF] = nblocks((long)h->emf_ni * h->emf_nj * h->emf_nk);
-    cnt[BS_CT1] = e0 ? nblocks((long)(d.n1 + 1) * d.n2 * d.n3) : 0;
-    cnt[BS_CT2] = e1 ? nblocks((long)d.n1 * (d.n2 + 1) * d.n3) : 0;
-    cnt[BS_CT3] = e2 ? nblocks((long)d.n1 * d.n2 * (d.n3 + 1)) : 0;
+    cnt[BS_CT] = nblocks((long)(d.n1 + e0) * (d.n2 + e1) * (d.n3 + e2));
     cnt[BS_FACC] = e0 ? kFaccBlocks : 0;
     cnt[BS_ZONE] = nblocks((long)d.n1 * d.n2 * d.n3);
     cnt[BS_BX1] = e0 ? nblocks(2L * d.ng1 * d.n2 * d.n3) : 0;
@@ -1499,9 +1504,7 @@ harm_status enq_batch_substep(harm_batch* b, int to, int nto, const double CUf[4
   BFLUX(3)
 #undef BFLUX
   hipLaunchKernelGGL(kb_emf, dim3((unsigned)b->ntot[BS_EMF]), blk, 0, s, G, P(BS_EMF), ng);
-  if (b->ntot[BS_CT1]) hipLaunchKernelGGL(kb_ct<1>, dim3((unsigned)b->ntot[BS_CT1]), blk, 0, s, G, P(BS_CT1), ng);
-  if (b->ntot[BS_CT2]) hipLaunchKernelGGL(kb_ct<2>, dim3((unsigned)b->ntot[BS_CT2]), blk, 0, s, G, P(BS_CT2), ng);
-  if (b->ntot[BS_CT3]) hipLaunchKernelGGL(kb_ct<3>, dim3((unsigned)b->ntot[BS_CT3]), blk, 0, s, G, P(BS_CT3), ng);
+  if (b->ntot[BS_CT]) hipLaunchKernelGGL(kb_ct_all, dim3((unsigned)b->ntot[BS_CT]), blk, 0, s, G, P(BS_CT), ng);
   if (b->ntot[BS_FACC]) {
     hipLaunchKernelGGL(kb_fluxacc_part, dim3((unsigned)b->ntot[BS_FACC]), blk, 0, s, G, P(BS_FACC), ng);
     hipLaunchKernelGGL(kb_fluxacc_final, dim3((unsigned)ng), dim3(64), 0, s, G, wflux[to]);
@@ -1744,10 +1747,15 @@ extern "C" harm_status harm_get_pflag(harm_handle* h, int32_t* host_flags)
 extern "C" harm_status harm_set_time(harm_handle* h, double t, double dt, int64_t nstep)
 {
   HH_TRY(check_handle(h));
-  (void)nstep;
-  double v[2] = {dt, t};
+  double v[DS_N];
   HH_HIP(hipStreamSynchronize(h->stream));
-  return hh::check_hip(hipMemcpy(h->ds, v, sizeof(v), hipMemcpyHostToDevice), "set_time");
+  HH_HIP(hipMemcpy(v, h->ds, sizeof(v), hipMemcpyDeviceToHost));
+  v[DS_DT] = dt;
+  v[DS_T] = t;
+  v[DS_NSTEP] = (double)nstep;
+  HH_HIP(hipMemcpy(h->ds, v, sizeof(v), hipMemcpyHostToDevice));
+  const unsigned long long z[2] = {0, 0};  // the Newton sums restart with the clock
+  return hh::check_hip(hipMemcpy(h->di + DI_NSTROKE_SUM, z, sizeof(z), hipMemcpyHostToDevice), "set_time");
 }
 
 extern "C" harm_status harm_get_time(harm_handle* h, double* t, double* dt, int64_t* nstep)
@@ -1758,7 +1766,24 @@ extern "C" harm_status harm_get_time(harm_handle* h, double* t, double* dt, int6
   HH_HIP(hipMemcpy(v, h->ds, sizeof(v), hipMemcpyDeviceToHost));
   if (t) *t = v[DS_T];
   if (dt) *dt = v[DS_DT];
-  if (nstep) *nstep = 0;
+  if (nstep) *nstep = (int64_t)v[DS_NSTEP];
+  return HARM_OK;
+}
+
+extern "C" harm_status harm_get_newton_stats(harm_handle* h, harm_newton_stats* last_step, harm_newton_stats* total)
+{
+  HH_TRY(check_handle(h));
+  unsigned long long z[DI_N];
+  HH_HIP(hipStreamSynchronize(h->stream));
+  HH_HIP(hipMemcpy(z, h->di, sizeof(z), hipMemcpyDeviceToHost));
+  if (last_step) {
+    last_step->nstroke = (int64_t)z[DI_NSTROKE];
+    last_step->nfail = (int64_t)z[DI_NFAIL];
+  }
+  if (total) {
+    total->nstroke = (int64_t)z[DI_NSTROKE_SUM];
+    total->nfail = (int64_t)z[DI_NFAIL_SUM];
+  }
   return HARM_OK;
 }
 
@@ -1873,7 +1898,7 @@ harm_status enq_step(harm_handle* h)
   rk_flux_weights(h->cfg.timeorder, w);
   hipLaunchKernelGGL(k_scal_begin_step, dim3(1), dim3(1), 0, h->stream, h->ds, h->di);
   for (int to = 0; to < nto; to++) HH_TRY(enq_substep(h, to, nto, CUf, Cunew, w, true, true));
-  hipLaunchKernelGGL(k_scal_end_step, dim3(1), dim3(1), 0, h->stream, h->ds, h->P.safe);
+  hipLaunchKernelGGL(k_scal_end_step, dim3(1), dim3(1), 0, h->stream, h->ds, h->di, h->P.safe);
   return hh::check_launch("harm_step");
 }
 }  // namespace

@@ -77,6 +77,14 @@ class Grid:
         check(lib().harm_get_corner_gdet(self.h, out.ctypes.data), "harm_get_corner_gdet")
         return out
 
+    @property
+    def dx(self):
+        """[0, dx1, dx2, dx3] of the uniform internal coordinates"""
+        X0, X1, V = (ctypes.c_double * 4)(), (ctypes.c_double * 4)(), (ctypes.c_double * 4)()
+        check(lib().harm_get_coords(self.h, 0, 0, 0, X0, V), "harm_get_coords")
+        check(lib().harm_get_coords(self.h, 1, 1, 1, X1, V), "harm_get_coords")
+        return [0.0] + [X1[a] - X0[a] for a in (1, 2, 3)]
+
     def rcent(self):
         r = np.empty(self.ns1)
         X = (ctypes.c_double * 4)()
@@ -91,8 +99,19 @@ class Grid:
         check(lib().harm_get_time(self.h, ctypes.byref(t), ctypes.byref(dt), ctypes.byref(n)), "harm_get_time")
         return t.value, dt.value
 
-    def set_time(self, t, dt):
-        check(lib().harm_set_time(self.h, t, dt, 0), "harm_set_time")
+    def set_time(self, t, dt, nstep=0):
+        check(lib().harm_set_time(self.h, t, dt, nstep), "harm_set_time")
+
+    def nstep(self):
+        t, dt, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
+        check(lib().harm_get_time(self.h, ctypes.byref(t), ctypes.byref(dt), ctypes.byref(n)), "harm_get_time")
+        return n.value
+
+    def newton_stats(self):
+        """((nstroke, nfail) of the last step, (nstroke, nfail) summed since set_time)"""
+        last, tot = NewtonStats(), NewtonStats()
+        check(lib().harm_get_newton_stats(self.h, ctypes.byref(last), ctypes.byref(tot)), "harm_get_newton_stats")
+        return (last.nstroke, last.nfail), (tot.nstroke, tot.nfail)
 
     # ---- problem setup ----
     def init_torus(self, rin=6.0, rmax=12.0, kappa=1e-3, beta=100.0, randfact=4e-2, seed=1):

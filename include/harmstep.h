@@ -138,8 +138,15 @@ harm_status harm_set_array(harm_handle* h, int32_t which, const double* host_aos
 harm_status harm_get_array(harm_handle* h, int32_t which, double* host_aos);
 /* pflag[k][j][i] (FLAGUTOPRIMFAIL) over the storage box */
 harm_status harm_get_pflag(harm_handle* h, int32_t* host_flags);
+/* the clock of step() (t, dt and the step count nstep, cf. global t, dt,
+ * nstep); setting it also restarts the Newton sums below */
 harm_status harm_set_time(harm_handle* h, double t, double dt, int64_t nstep);
 harm_status harm_get_time(harm_handle* h, double* t, double* dt, int64_t* nstep);
+/* Newton statistics of the last whole step and summed over the steps since
+ * harm_set_time (either pointer may be NULL); the failure counts are the
+ * zones left with pflag > 0 by each substep's inversion, as counted into the
+ * reference's failure diagnostics (fail.c) */
+harm_status harm_get_newton_stats(harm_handle* h, harm_newton_stats* last_step, harm_newton_stats* total);
 
 /* ------------------------------------------------------------------------ */
 /* Problem setup (host init, then upload)                                    */

@@ -304,3 +304,24 @@ def test_divb_stays_at_roundoff(case):
     scale = Bscale / min(og.dx[1], og.dx[2])
     print(case, "divB initial", np.abs(d0).max() / scale, "after 20 steps", np.abs(d1).max() / scale)
     assert np.abs(d1).max() / scale < 1e-12
+
+
+def test_step_clock_and_newton_sums():
+    """harm_get_time counts steps from the value given to harm_set_time, and
+    the Newton sums since then are the sums of the per-step counts."""
+    cfg, og, g = make("3d_para_hll_rk2")
+    p = perturbed_torus(og)
+    dt = 0.5 * min(og.fluxcalc(p, og.zeros(), og.zeros(), og.zeros()))
+    g.set("p", p)
+    g.set_time(0.0, dt, nstep=5)
+    tot_s, tot_f = 0, 0
+    for _ in range(3):
+        g.step(1)
+        (ns, nf), _ = g.newton_stats()
+        tot_s, tot_f = tot_s + ns, tot_f + nf
+    _, (ns_sum, nf_sum) = g.newton_stats()
+    zones = cfg.n1 * cfg.n2 * cfg.n3
+    print("steps", g.nstep(), "Newton iterations per zone-substep", ns_sum / (3 * 2 * zones), "failures", nf_sum)
+    assert g.nstep() == 8
+    assert (ns_sum, nf_sum) == (tot_s, tot_f)
+    assert ns_sum >= 3 * 2 * zones
