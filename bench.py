@@ -210,11 +210,14 @@ def inversion_microbench(args):
 
 
 def batch_bench(args):
-    """64 independent 2-D Fishbone-Moncrief tori of uneven sizes (64^2 ... 256^2),
-    spins 0-0.99, beta 30-300, seeds 0-63, PARA / LAX-F / RK2, stepped together
-    (one launch per stage over all grids, the RK step replayed from a hipGraph,
-    tile shapes autotuned per grid) against the same grids stepped one after
-    another with the single-grid step."""
+    """64 independent 2-D problems of uneven sizes (64^2 ... 256^2): Fishbone-
+    Moncrief tori (even seeds; spins 0-0.99, beta 30-300) and Bondi flows (odd
+    seeds; spin 0, the reference's Bondi boundary set), PARA / LAX-F / RK2,
+    stepped together (one launch per stage over all grids, the RK step
+    replayed from a hipGraph, tile shapes autotuned per grid) against the same
+    grids stepped one after another with the single-grid step; the smallest
+    torus and the smallest Bondi flow are each checked against their own CPU
+    solve."""
     import numpy as np
     import harm_harm_amd as H
     from harm_harm_amd.batch import Batch
@@ -225,17 +228,25 @@ def batch_bench(args):
     members = []
     for seed in range(args.ngrids):
         n = int(rng.choice(sizes))
-        members.append((n, float(rng.uniform(0.0, 0.99)), float(np.exp(rng.uniform(np.log(30), np.log(300)))), seed))
+        kind = "torus" if seed % 2 == 0 else "bondi"
+        members.append((kind, n, float(rng.uniform(0.0, 0.99)), float(np.exp(rng.uniform(np.log(30), np.log(300)))),
+                        seed))
     cfgs = []
-    for n, a, beta, seed in members:
-        c = H.default_config(n, n, 1, recon="para", flux="laxf", timeorder=2)
-        c.a = a
+    for kind, n, a, beta, seed in members:
+        if kind == "torus":
+            c = H.default_config(n, n, 1, recon="para", flux="laxf", timeorder=2)
+            c.a = a
+        else:
+            c = H.bondi_config(n, n, recon="para", flux="laxf", timeorder=2)
         cfgs.append(c)
-    zones = sum(n * n for n, _, _, _ in members)
+    zones = sum(n * n for _, n, _, _, _ in members)
     B = Batch(cfgs)
     init = []
-    for g, (n, a, beta, seed) in zip(B.grids, members):
-        g.init_torus(beta=beta, seed=seed)
+    for g, (kind, n, a, beta, seed) in zip(B.grids, members):
+        if kind == "torus":
+            g.init_torus(beta=beta, seed=seed)
+        else:
+            g.init_bondi()
         ndt = g.fluxcalc("p")
         dt = 0.5 / sum(1.0 / x for x in ndt)
         g.set_time(0.0, dt)
@@ -272,35 +283,46 @@ def batch_bench(args):
     el_s = time.perf_counter() - t0
     log(f"one by one: {el_s:.3f} s")
     same = all(np.array_equal(g.get("p"), s_.get("p")) for g, s_ in zip(B.grids, singles))
-    # one member against its own CPU solve
-    check = None
+    # the smallest member of each kind against its own CPU solve
+    checks = []
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import _oracle
-        k = min(range(len(members)), key=lambda i: members[i][0])
-        og = _oracle.OGrid(cfgs[k], nthreads=min(16, os.cpu_count() or 1))
-        st = og.new_state(init[k][0].copy())
-        dt = init[k][1]
-        t0 = time.time()
-        for _ in range(args.steps):
-            dt = og.step(st, dt)
-        el_c = time.time() - t0
-        a = B.grids[k].get("p")[:, og.ng2:og.ng2 + og.cfg.n2, og.ng1:og.ng1 + og.cfg.n1]
-        b = st["p"][:, og.ng2:og.ng2 + og.cfg.n2, og.ng1:og.ng1 + og.cfg.n1]
-        rel = max(float(np.abs(a[..., v] - b[..., v]).max() / max(np.abs(b[..., v]).max(), 1e-300)) for v in range(8))
-        n = members[k][0]
-        check = {"grid": k, "size": f"{n}x{n}", "steps": args.steps, "max_rel_difference": rel,
-                 "cpu_zone_updates_per_s": n * n * args.steps / el_c, "cpu_threads": min(16, os.cpu_count() or 1)}
+        for kind in ("torus", "bondi"):
+            ks = [i for i in range(len(members)) if members[i][0] == kind]
+            if not ks:
+                continue
+            k = min(ks, key=lambda i: members[i][1])
+            og = _oracle.OGrid(cfgs[k], nthreads=min(16, os.cpu_count() or 1))
+            st = og.new_state(init[k][0].copy())
+            dt = init[k][1]
+            t0 = time.time()
+            for _ in range(args.steps):
+                dt = og.step(st, dt)
+            el_c = time.time() - t0
+            a = B.grids[k].get("p")[:, og.ng2:og.ng2 + og.cfg.n2, og.ng1:og.ng1 + og.cfg.n1]
+            b = st["p"][:, og.ng2:og.ng2 + og.cfg.n2, og.ng1:og.ng1 + og.cfg.n1]
+            # velocities against the largest velocity component (the Bondi flow's
+            # polar velocity is rounding noise), the rest per variable
+            vs = max(float(np.abs(b[..., 2:5]).max()), 1e-300)
+            rel = max(float(np.abs(a[..., v] - b[..., v]).max()
+                            / (vs if 2 <= v <= 4 else max(np.abs(b[..., v]).max(), 1e-300))) for v in range(8))
+            n = members[k][1]
+            checks.append({"grid": k, "problem": kind, "size": f"{n}x{n}", "steps": args.steps,
+                           "max_rel_difference": rel, "cpu_zone_updates_per_s": n * n * args.steps / el_c,
+                           "cpu_threads": min(16, os.cpu_count() or 1)})
+    nt = sum(1 for m in members if m[0] == "torus")
     out = {
-        "metric": "zone-updates/s, 64 uneven 2-D tori stepped together (batched launches + hipGraph) vs one by one",
+        "metric": "zone-updates/s, 64 uneven 2-D tori and Bondi flows stepped together (batched launches + hipGraph)"
+                  " vs one by one",
         "value": zones * args.steps / el_b, "unit": "zone-updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": el_b / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded torus initial data)",
-        "config": {"workload": f"{len(members)} 2-D FM tori, sizes {sizes}, PARA/LAX-F/RK2, {zones} zones",
-                   "tune_seconds": t_tune},
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded torus initial data; exact Bondi flow)",
+        "config": {"workload": f"{nt} 2-D FM tori and {len(members) - nt} Bondi flows, sizes {sizes}, "
+                               f"PARA/LAX-F/RK2, {zones} zones", "tune_seconds": t_tune},
         "one_by_one": {"value": zones * args.steps / el_s, "ms_per_step": el_s / args.steps * 1e3,
                        "speedup_of_batch": el_s / el_b, "identical_results": same},
-        "oracle_check": check,
+        "oracle_check": checks,
     }
     print(json.dumps(out))
 
@@ -391,7 +413,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", choices=["step", "inversion", "batch", "large"], default="step",
                     help="step: the 256x128x64 torus RK3 step (default); inversion: the batched-inversion "
-                         "microbench, plain vs MLP-seeded; batch: 64 uneven 2-D tori stepped together; "
+                         "microbench, plain vs MLP-seeded; batch: 64 uneven 2-D tori and Bondi flows stepped together; "
                          "large: the 384x192x192 torus with failures, div B and accretion sums")
     ap.add_argument("--ngrids", type=int, default=64, help="batch: number of grids")
     ap.add_argument("--n", type=int, default=1 << 24, help="inversion microbench: zones")

@@ -33,7 +33,7 @@ class HarmConfig(ctypes.Structure):
     _fields_ = [
         ("n1", ctypes.c_int32), ("n2", ctypes.c_int32), ("n3", ctypes.c_int32), ("ng", ctypes.c_int32),
         ("recon", ctypes.c_int32), ("flux", ctypes.c_int32), ("timeorder", ctypes.c_int32),
-        ("pad0", ctypes.c_int32),
+        ("problem", ctypes.c_int32),
         ("a", ctypes.c_double), ("gam", ctypes.c_double), ("R0", ctypes.c_double), ("Rin", ctypes.c_double),
         ("Rout", ctypes.c_double), ("hslope", ctypes.c_double), ("cour", ctypes.c_double),
         ("safe", ctypes.c_double), ("rhomin", ctypes.c_double), ("uumin", ctypes.c_double),
@@ -55,6 +55,7 @@ _DP = ctypes.POINTER(ctypes.c_double)
 _PROTOS = [
     ("harm_last_error", ctypes.c_char_p, []),
     ("harm_default_config", None, [ctypes.POINTER(HarmConfig), _I, _I, _I]),
+    ("harm_bondi_config", None, [ctypes.POINTER(HarmConfig), _I, _I]),
     ("harm_geom_ks_record", None, [_D, _D, _D, _P]),
     ("harm_geom_mks_record", None, [ctypes.POINTER(HarmConfig), _P, _P]),
     ("harm_geom_ks_batch", None, [_L, _P, _P, _P, _P]),
@@ -89,6 +90,9 @@ _OPTIONAL_PROTOS = [
     ("harm_get_pflag", _I, [_P, _P]),
     ("harm_set_time", _I, [_P, _D, _D, _L]),
     ("harm_get_time", _I, [_P, _P, _P, _P]),
+    ("harm_init_bondi", _I, [_P]),
+    ("harm_host_init_bondi", _I, [_P, _P]),
+    ("harm_bondi_state", _I, [_D, _D, _P]),
     ("harm_get_newton_stats", _I, [_P, _P, _P]),
     ("harm_init_torus", _I, [_P, _D, _D, _D, _D, _D, ctypes.c_uint32]),
     ("harm_fluxcalc", _I, [_P, _I, _I, _P]),
@@ -159,6 +163,27 @@ def default_config(n1, n2, n3, **kw):
             v = FLUX[v]
         setattr(c, k, v)
     return c
+
+
+def bondi_config(n1, n2, **kw):
+    """The Bondi problem's setup on an n1 x n2 (r, theta) grid; keyword
+    arguments override fields as in default_config."""
+    c = HarmConfig()
+    lib().harm_bondi_config(ctypes.byref(c), n1, n2)
+    for k, v in kw.items():
+        if k == "recon" and isinstance(v, str):
+            v = RECON[v]
+        if k == "flux" and isinstance(v, str):
+            v = FLUX[v]
+        setattr(c, k, v)
+    return c
+
+
+def bondi_state(r, gam=4.0 / 3.0):
+    """(rho, u, u^r) of the Bondi flow at radius r (host computation)."""
+    out = (ctypes.c_double * 3)()
+    check(lib().harm_bondi_state(gam, r, out), "harm_bondi_state")
+    return tuple(out)
 
 
 def device_count():

@@ -74,6 +74,22 @@ extern "C" void harm_default_config(harm_config* c, int32_t n1, int32_t n2, int3
   c->gammafail = 2500.0;
 }
 
+extern "C" void harm_bondi_config(harm_config* c, int32_t n1, int32_t n2)
+{
+  harm_default_config(c, n1, n2, 1);
+  c->problem = HARM_PROBLEM_BONDI;
+  c->a = 0.0;
+  c->R0 = 0.0;
+  c->Rin = 1.9;
+  c->Rout = 20.0;
+  c->hslope = 1.0;
+  c->cour = 0.5;
+  c->rhomin = 0.0;
+  c->uumin = 0.0;
+  c->gammamax = 100.0;
+  c->gammafail = 100.0 * c->gammamax;
+}
+
 extern "C" harm_status harm_device_count(int32_t* n)
 {
   int c = 0;

@@ -24,6 +24,8 @@ enum { DI_NSTROKE = 0, DI_NFAIL = 1, DI_NSTROKE_SUM = 2, DI_NFAIL_SUM = 3, DI_N 
 struct Params {
   double gam, cour, safe, gammamax, gammafail;
   double dx[4], startx[4];
+  double R0;  // r = R0 + exp(X1)
+  int bc;     // HARM_PROBLEM_*: which radial and polar boundary conditions
 };
 
 __device__ __forceinline__ void atomic_min_pos(double* addr, double v)

@@ -562,6 +562,43 @@ __device__ void extrap_line(const Dims& d, const Params& P, double* p, const dou
   }
 }
 
+// Bondi radial boundaries (bounds.sasha.c:182-198, :341-353; BONDIEXTRAPORDER
+// 1): the integrals of the boundary zone (inner: C1, C2 and K; outer: C1 with
+// the exact C2 and K) give the state of ghost zone `gsel` of the (j, k) line;
+// the fields are zero there (the reference zeroes every primitive first).
+__device__ void bondi_line(const Dims& d, const Params& P, double* p, const double* gcent, int up, int j, int k,
+                           int gsel)
+{
+  const long nc = d.ncell;
+  const int ri = up ? d.n1 - 1 : 0;
+  const int i = up ? d.n1 + gsel : -1 - gsel;
+  double pr[HARM_NPR];
+  const long o1 = hd::cidx(d, ri, j, k);
+#pragma unroll
+  for (int pl = 0; pl < HARM_NPR; pl++) pr[pl] = p[pl * nc + o1];
+  Geom Gr;
+  hd::load_geom(gcent, hd::pidx(d, ri, j), d.nplane, Gr);
+  const double rr = P.R0 + exp(P.startx[1] + (ri + 0.5) * P.dx[1]);
+  double C1, C2, K;
+  hd::bondi_integrals(pr, Gr, rr, P.R0, P.gam, C1, C2, K);
+  if (up) {
+    C2 = hd::kBondiC2x;
+    K = hd::kBondiKx;
+  }
+  const double r = P.R0 + exp(P.startx[1] + (i + 0.5) * P.dx[1]);
+  double pi[HARM_NPR];
+#pragma unroll
+  for (int pl = 0; pl < HARM_NPR; pl++) pi[pl] = 0.0;
+  double ur = 0.0;
+  hd::bondi_invert(1 / (P.gam - 1), C1, C2, K, r, pi[HARM_RHO], pi[HARM_UU], ur);
+  Geom Gi;
+  hd::load_geom(gcent, hd::pidx(d, i, j), d.nplane, Gi);
+  hd::radial_ks_to_rel4(ur, r, P.R0, Gi, pi);
+  const long oi = hd::cidx(d, i, j, k);
+#pragma unroll
+  for (int pl = 0; pl < HARM_NPR; pl++) p[pl * nc + oi] = pi[pl];
+}
+
 __device__ __forceinline__ void bound_x1_body(Dims d, Params P, double* __restrict__ p,
                                                      const double* __restrict__ gcent, const int blk)
 {
@@ -572,7 +609,8 @@ __device__ __forceinline__ void bound_x1_body(Dims d, Params P, double* __restri
   const int gsel = (int)(l % d.ng1);
   const long line = l / d.ng1;
   const int j = (int)(line % d.n2), k = (int)(line / d.n2);
-  extrap_line(d, P, p, gcent, up, j, k, gsel);
+  if (P.bc == HARM_PROBLEM_BONDI) bondi_line(d, P, p, gcent, up, j, k, gsel);
+  else extrap_line(d, P, p, gcent, up, j, k, gsel);
 }
 
 __global__ __launch_bounds__(kBlock) void k_bound_x1(Dims d, Params P, double* __restrict__ p,
@@ -682,7 +720,7 @@ __device__ __forceinline__ void x2_poledeath_body(Dims d, Params P, double* __re
 {
   const long t = (long)blk * kBlock + threadIdx.x;
   const long ncol = (long)d.ns1 * d.n3;
-  if (t >= ncol) return;
+  if (t >= ncol || P.bc == HARM_PROBLEM_BONDI) return;  // Bondi: plain reflection only
   x2_poledeath_col(d, P, p, gcent, rcent, side, t);
 }
 
@@ -702,7 +740,10 @@ __device__ __forceinline__ void x2_poles_body(Dims d, Params P, double* __restri
   const long ncol = (long)d.ns1 * d.n3;
   if (t < 2 * ncol) {
     const int side = t >= ncol;
-    x2_poledeath_col(d, P, p, gcent, rcent, side, t - side * ncol, true);
+    if (P.bc == HARM_PROBLEM_BONDI)  // plain polar reflection (POLARAXIS): rows -1 and n2 too
+      x2_reflect_zone(d, p, side, side == 0 ? d.ng2 - 1 : 0, t - side * ncol);
+    else
+      x2_poledeath_col(d, P, p, gcent, rcent, side, t - side * ncol, true);
     return;
   }
   const long u = t - 2 * ncol, nside = ncol * (d.ng2 - 1);
@@ -888,6 +929,8 @@ void make_dims(const harm_config* cfg, Dims& d, Params& P)
   d.nplane = (long)d.ps1 * d.ns2;
   d.ncell = d.nplane * d.ns3;
   P.gam = cfg->gam; P.cour = cfg->cour; P.safe = cfg->safe; P.gammamax = cfg->gammamax; P.gammafail = cfg->gammafail;
+  P.R0 = cfg->R0;
+  P.bc = cfg->problem;
   P.startx[0] = 0.0; P.dx[0] = 0.0;
   P.startx[1] = log(cfg->Rin - cfg->R0);
   P.dx[1] = (log(cfg->Rout - cfg->R0) - log(cfg->Rin - cfg->R0)) / cfg->n1;
@@ -1579,6 +1622,21 @@ extern "C" harm_status harm_create(const harm_config* cfg, harm_handle** out)
   if (cfg->n3 > 1 && (cfg->n3 % 2 != 0 || cfg->n3 / 2 < cfg->ng))
     return hh::set_error(HARM_EINVAL, "harm_create: n3 must be even and n3/2 >= ng for the polar rotation");
   if (cfg->timeorder != 2 && cfg->timeorder != 3) return hh::set_error(HARM_EINVAL, "harm_create: timeorder 2 or 3");
+  if (cfg->problem != HARM_PROBLEM_TORUS && cfg->problem != HARM_PROBLEM_BONDI)
+    return hh::set_error(HARM_EINVAL, "harm_create: unknown problem %d", cfg->problem);
+  if (cfg->problem == HARM_PROBLEM_BONDI) {
+    // the boundary integrals and their inversion hold for this flow only
+    if (cfg->a != 0.0 || fabs(cfg->gam - 4.0 / 3.0) > 1e-15 || cfg->n3 != 1 || cfg->n1 < 2)
+      return hh::set_error(HARM_EINVAL, "harm_create: the Bondi problem needs a = 0, gam = 4/3, n3 = 1");
+    const double dx1 = (log(cfg->Rout - cfg->R0) - log(cfg->Rin - cfg->R0)) / cfg->n1;
+    const double r_in_lo = cfg->R0 + (cfg->Rin - cfg->R0) * exp(-(cfg->ng - 0.5) * dx1);
+    const double r_in_hi = cfg->R0 + (cfg->Rin - cfg->R0) * exp(-0.5 * dx1);
+    const double r_out_lo = cfg->R0 + (cfg->Rout - cfg->R0) * exp(0.5 * dx1);
+    if (!(r_in_lo > 0.35 && r_in_hi < 2.5 && r_out_lo > 9.0))
+      return hh::set_error(HARM_EINVAL,
+                           "harm_create: Bondi ghost zones at r in [%g, %g] and beyond %g; the integral inversion "
+                           "needs 0.35 < r < 2.5 inside and r > 9 outside", r_in_lo, r_in_hi, r_out_lo);
+  }
   if (cfg->recon < HARM_RECON_DONOR || cfg->recon > HARM_RECON_WENO5 || cfg->recon == HARM_RECON_MC)
     return hh::set_error(HARM_EINVAL, "harm_create: recon %d not supported", cfg->recon);
   int ndev = 0;
@@ -1978,6 +2036,42 @@ extern "C" harm_status harm_host_init_torus(const harm_config* cfg, double rin, 
   memset(p_aos, 0, sizeof(double) * HARM_NPR * d.ns1 * d.ns2 * d.ns3);
   if (hi_init_torus(cfg, &G, rin, rmax, kappa, beta, randfact, seed, p_aos) != 0)
     return hh::set_error(HARM_EINVAL, "harm_host_init_torus: no torus zones on this grid");
+  return HARM_OK;
+}
+
+extern "C" harm_status harm_host_init_bondi(const harm_config* cfg, double* p_aos)
+{
+  if (!cfg || !p_aos) return hh::set_error(HARM_EINVAL, "harm_host_init_bondi: null argument");
+  Dims d;
+  Params P;
+  make_dims(cfg, d, P);
+  hi_grid G;
+  G.ng1 = d.ng1; G.ng2 = d.ng2; G.ng3 = d.ng3;
+  for (int a = 0; a < 4; a++) { G.startx[a] = P.startx[a]; G.dx[a] = P.dx[a]; }
+  memset(p_aos, 0, sizeof(double) * HARM_NPR * d.ns1 * d.ns2 * d.ns3);
+  if (hi_init_bondi(cfg, &G, p_aos) != 0)
+    return hh::set_error(HARM_EINVAL, "harm_host_init_bondi: no Bondi flow through some zone of this grid");
+  return HARM_OK;
+}
+
+extern "C" harm_status harm_init_bondi(harm_handle* h)
+{
+  HH_TRY(check_handle(h));
+  if (h->cfg.problem != HARM_PROBLEM_BONDI)
+    return hh::set_error(HARM_EINVAL, "harm_init_bondi: the handle was not created for the Bondi problem");
+  const Dims& d = h->d;
+  std::vector<double> p((size_t)HARM_NPR * d.ns1 * d.ns2 * d.ns3, 0.0);
+  HH_TRY(harm_host_init_bondi(&h->cfg, p.data()));
+  HH_TRY(harm_set_array(h, HARM_ARR_P, p.data()));
+  HH_TRY(enq_bound(h, HARM_ARR_P));
+  return harm_sync(h);
+}
+
+extern "C" harm_status harm_bondi_state(double gam, double r, double out[3])
+{
+  if (!out) return hh::set_error(HARM_EINVAL, "harm_bondi_state: null argument");
+  if (hi_bondi_state(gam, r, &out[0], &out[1], &out[2]) != 0)
+    return hh::set_error(HARM_EINVAL, "harm_bondi_state: no flow through r = %g", r);
   return HARM_OK;
 }
 

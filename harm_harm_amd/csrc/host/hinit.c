@@ -240,3 +240,111 @@ int hi_init_torus(const harm_config* c, const hi_grid* G, double rin, double rma
 #undef PIX
   return 0;
 }
+
+
+/* ---------------------------------------------------------------------- */
+/* Bondi accretion (TESTNUMBER 153, initboundcode/init.sasha.c:865, :2977) */
+/* ---------------------------------------------------------------------- */
+/* The reference reads the flow from initbounddata/bondigrid<N>.dat (written
+ * by a Mathematica notebook for Gamma = 4/3, sonic point r_c = 8, rho_c =
+ * 0.000421875).  Here it is computed: at the sonic point u_c^2 = 1/(2 r_c)
+ * and the sound speed a_c^2 = u_c^2/(1 - 3 u_c^2); with a^2 = Gamma T/(1 +
+ * (n+1) T), T = p/rho, n = 1/(Gamma-1), that fixes T_c; the polytropic
+ * constant is K = 1 (rho_c = T_c^n), and the integrals C1 = T^n u^r r^2 and
+ * C2 = (1 + (1+n) T)^2 (1 - 2/r + (u^r)^2) are those of the sonic point.  At
+ * each r the temperature is the root of
+ *   F(T) = 2 ln(1 + (1+n) T) + ln(1 - 2/r + (C1 / (T^n r^2))^2) - ln C2,
+ * which has one minimum in T: the larger root (subsonic) outside r_c, the
+ * smaller (supersonic) inside, both by bisection to the last bit. */
+static const double kBondiRc = 8.0;
+
+static void bondi_constants(double gam, double* n, double* C1, double* C2, double* Tc)
+{
+  *n = 1.0 / (gam - 1.0);
+  const double uc2 = 1.0 / (2.0 * kBondiRc), uc = -sqrt(uc2);
+  const double ac2 = uc2 / (1.0 - 3.0 * uc2);
+  *Tc = ac2 / (gam - (*n + 1.0) * ac2);
+  *C1 = pow(*Tc, *n) * uc * kBondiRc * kBondiRc;
+  *C2 = pow(1.0 + (1.0 + *n) * (*Tc), 2.0) * (1.0 - 2.0 / kBondiRc + uc2);
+}
+
+static double bondi_F(double T, double n, double C1, double C2, double r)
+{
+  const double u = C1 / (pow(T, n) * r * r);
+  return 2.0 * log(1.0 + (1.0 + n) * T) + log(1.0 - 2.0 / r + u * u) - log(C2);
+}
+
+static double bondi_dF(double T, double n, double C1, double r)
+{
+  const double u = C1 / (pow(T, n) * r * r);
+  const double w = 1.0 - 2.0 / r + u * u;
+  return 2.0 * (1.0 + n) / (1.0 + (1.0 + n) * T) - 2.0 * n * u * u / (T * w);
+}
+
+/* bisection of f over [lo, hi] (f(lo), f(hi) of opposite sign) until the
+ * interval stops shrinking */
+static double bisect(double (*f)(double, const double*), const double* par, double lo, double hi)
+{
+  double flo = f(lo, par);
+  for (int it = 0; it < 2000; it++) {
+    const double mid = 0.5 * (lo + hi);
+    if (mid <= lo || mid >= hi) break;
+    const double fm = f(mid, par);
+    if ((fm < 0) == (flo < 0)) { lo = mid; flo = fm; }
+    else hi = mid;
+  }
+  return 0.5 * (lo + hi);
+}
+static double f_F(double T, const double* q) { return bondi_F(T, q[0], q[1], q[2], q[3]); }
+static double f_dF(double T, const double* q) { return bondi_dF(T, q[0], q[1], q[3]); }
+
+int hi_bondi_state(double gam, double r, double* rho, double* uu, double* ur)
+{
+  double n, C1, C2, Tc;
+  bondi_constants(gam, &n, &C1, &C2, &Tc);
+  const double q[4] = {n, C1, C2, r};
+  double lo = 1e-12, hi = 1e3;
+  if (!(bondi_dF(lo, n, C1, r) < 0.0 && bondi_dF(hi, n, C1, r) > 0.0)) return 1;
+  const double Tmin = bisect(f_dF, q, lo, hi);
+  double T;
+  if (r == kBondiRc) T = Tc;
+  else if (bondi_F(Tmin, n, C1, C2, r) > 0.0) return 1; /* no flow through this radius */
+  else if (r > kBondiRc) T = bisect(f_F, q, Tmin, hi);
+  else T = bisect(f_F, q, lo, Tmin);
+  *rho = pow(T, n); /* K = 1 */
+  *uu = *rho * T * n;
+  *ur = C1 / (pow(T, n) * r * r);
+  return 0;
+}
+
+int hi_init_bondi(const harm_config* c, const hi_grid* G, double* p)
+{
+  const int n1 = c->n1, n2 = c->n2, n3 = c->n3;
+  const int ng1 = G->ng1, ng2 = G->ng2, ng3 = G->ng3;
+  const int ns1 = n1 + 2 * ng1, ns2 = n2 + 2 * ng2;
+#define PIX(i, j, k) ((((long)((k) + ng3) * ns2 + ((j) + ng2)) * ns1 + ((i) + ng1)) * HARM_NPR)
+  for (int k = 0; k < n3; k++)
+    for (int j = 0; j < n2; j++)
+      for (int i = 0; i < n1; i++) {
+        double X[4], V[4], J[4][4], rec[HARM_G_NFIELD], rho, uu, ur;
+        zone_X(c, G, i, j, k, X);
+        hg_mks_V(c, X, V);
+        if (hi_bondi_state(c->gam, V[1], &rho, &uu, &ur) != 0) return 1;
+        hg_mks_dxdxp(c, X, J);
+        hg_mks_record(c, X, rec);
+        /* radial KS 4-velocity -> MKS relative 4-velocity (u^t > 0) */
+        const double u1 = ur / J[1][1];
+        const double A = rec[0], B = 2.0 * rec[1] * u1, C = 1.0 + rec[4] * u1 * u1;
+        const double ut = (-B - sqrt(B * B - 4.0 * A * C)) / (2.0 * A);
+        double* pr = p + PIX(i, j, k);
+        for (int pl = 0; pl < HARM_NPR; pl++) pr[pl] = 0.0;
+        pr[HARM_RHO] = rho;
+        pr[HARM_UU] = uu;
+        pr[HARM_U1] = u1 + ut * rec[HARM_G_BETA1];
+        pr[HARM_U2] = ut * rec[HARM_G_BETA2];
+        pr[HARM_U3] = ut * rec[HARM_G_BETA3];
+      }
+#undef PIX
+  return 0;
+}
+

@@ -18,6 +18,12 @@ typedef struct hi_grid {
 int hi_init_torus(const harm_config* c, const hi_grid* G, double rin, double rmax, double kappa, double beta,
                   double randfact, uint32_t seed, double* p);
 
+/* Bondi flow (Gamma, sonic point r = 8, K = 1) at radius r: rho, u and the
+ * Kerr-Schild radial 4-velocity; nonzero when no flow passes r. */
+int hi_bondi_state(double gam, double r, double* rho, double* uu, double* ur);
+/* Fills the active zones with the Bondi flow (fields zero). */
+int hi_init_bondi(const harm_config* c, const hi_grid* G, double* p);
+
 #ifdef __cplusplus
 }
 #endif

@@ -114,6 +114,9 @@ class Grid:
         return (last.nstroke, last.nfail), (tot.nstroke, tot.nfail)
 
     # ---- problem setup ----
+    def init_bondi(self):
+        check(lib().harm_init_bondi(self.h), "harm_init_bondi")
+
     def init_torus(self, rin=6.0, rmax=12.0, kappa=1e-3, beta=100.0, randfact=4e-2, seed=1):
         check(lib().harm_init_torus(self.h, rin, rmax, kappa, beta, randfact, seed), "harm_init_torus")
 

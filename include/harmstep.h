@@ -67,6 +67,20 @@ enum { HARM_NCONN = 40 };
 /* grid locations with geometry tables */
 enum { HARM_LOC_CENT = 0, HARM_LOC_FACE1 = 1, HARM_LOC_FACE2 = 2, HARM_LOC_FACE3 = 3, HARM_NLOC = 4 };
 
+/* ---- problems: the boundary-condition sets of the two problem setups ----
+ * TORUS: the FM torus (initboundcode/bounds.fishmon.c:15-216): radial outflow
+ *        extrapolation with the inflow check, polar reflection with pole
+ *        death, periodic phi.
+ * BONDI: spherical Bondi accretion (TESTNUMBER 153,
+ *        initboundcode/init.sasha.c:865, :1502; bounds.sasha.c:182, :341):
+ *        inner BONDIINTOUTFLOW (the Bondi integrals C1, C2, K of the first
+ *        active zone carried into the ghost zones), outer BONDIMDOTOUTFLOW
+ *        (C1 of the last active zone with the exact C2 and K), plain polar
+ *        reflection (POLARAXIS).  Spin 0, Gamma 4/3, 2-D (n3 = 1), and the
+ *        radial ghost zones must lie in 0.35 < r < 2.5 (inner) and r > 9
+ *        (outer), the ranges of the reference's integral inversion.        */
+enum { HARM_PROBLEM_TORUS = 0, HARM_PROBLEM_BONDI = 1 };
+
 /* ---- runtime configuration (HARM's init_global/init_grid globals) ---- */
 typedef struct harm_config {
   int32_t n1, n2, n3;       /* active zones (N1,N2,N3)                        */
@@ -74,7 +88,7 @@ typedef struct harm_config {
   int32_t recon;            /* HARM_RECON_*  (lim[1..3])                      */
   int32_t flux;             /* HARM_FLUX_*   (fluxmethod)                     */
   int32_t timeorder;        /* 2 (RK2 midpoint) or 3 (RK3 TVD)               */
-  int32_t pad0;
+  int32_t problem;          /* HARM_PROBLEM_*: boundary conditions (BCtype[]) */
   double a;                 /* Kerr spin                                      */
   double gam;               /* ideal-gas adiabatic index                      */
   double R0, Rin, Rout;     /* MKS radial map r = R0 + exp(X1)                */
@@ -101,6 +115,11 @@ typedef struct harm_newton_stats {
 /* Fills cfg with HARM's torus defaults for an n1 x n2 x n3 grid
  * (init.fishmon.c:194-290, init.tools.c:89-263, rescaletype 0 floors). */
 void harm_default_config(harm_config* cfg, int32_t n1, int32_t n2, int32_t n3);
+/* Fills cfg with the Bondi problem's setup on an n1 x n2 (r, theta) grid
+ * (TESTNUMBER 153: init.sasha.c:865-895, :1084-1102, :1222-1228): spin 0,
+ * Gamma 4/3, MKS with R0 = 0 and hslope = 1, Rin = 1.9, Rout = 20, Courant
+ * 0.5, no floors (RHOMIN = UUMIN = 0), GAMMAMAX 100. */
+void harm_bondi_config(harm_config* cfg, int32_t n1, int32_t n2);
 
 /* Replaces init()'s grid/geometry part (initbase.c:7 -> set_grid.c:114):
  * builds MKS Kerr-Schild geometry and connection tables on the host and
@@ -156,6 +175,11 @@ harm_status harm_get_newton_stats(harm_handle* h, harm_newton_stats* last_step, 
  * init.tools.c:295-894).  Leaves p bounded and sets dt from set_dt(). */
 harm_status harm_init_torus(harm_handle* h, double rin, double rmax, double kappa,
                             double beta, double randfact, uint32_t seed);
+/* Bondi accretion flow through the sonic point r = 8 (the flow of
+ * initbounddata/bondigrid<N>.dat, init.sasha.c:2977-2988): rho, u and the
+ * radial Kerr-Schild 4-velocity on every active zone, fields zero; leaves p
+ * bounded.  The handle must have been created with problem BONDI. */
+harm_status harm_init_bondi(harm_handle* h);
 
 /* Host-only forms (no GPU needed): the geometry tables harm_create uploads
  * (geom_aos[loc][j][i][27] for the 4 locations, conn_aos[j][i][40],
@@ -166,6 +190,10 @@ harm_status harm_host_tables(const harm_config* cfg, double* geom_aos, double* c
                              double* rcent, double startx[4], double dx[4]);
 harm_status harm_host_init_torus(const harm_config* cfg, double rin, double rmax, double kappa, double beta,
                                  double randfact, uint32_t seed, double* p_aos);
+harm_status harm_host_init_bondi(const harm_config* cfg, double* p_aos);
+/* The Bondi flow at Boyer-Lindquist/Kerr-Schild radius r (spin 0):
+ * out = {rho, u, u^r}. */
+harm_status harm_bondi_state(double gam, double r, double out[3]);
 
 /* ------------------------------------------------------------------------ */
 /* Hot path                                                                  */

@@ -200,8 +200,115 @@ static void inflow_check_x1up(double* pr, const o_geom* G, double gam, double ga
   pr[O_U3] *= gnew;
 }
 
+/* ---- Bondi problem (TESTNUMBER 153) ------------------------------------ */
+/* bondi_compute_integrals (initboundcode/bounds.sasha.c:896): the zone's
+ * primitives as Kerr-Schild VEL4 (metp2met2bl: u^r = (dr/dX1) u^X1 with
+ * dr/dX1 = r - R0), T = p/rho, C1 = T^n u^r r^2, C2 = (1+(1+n)T)^2
+ * (1 - 2/r + (u^r)^2), K = p/rho^Gamma. */
+static void bondi_integrals(const o_grid* g, const double* pr, const o_geom* G, double r, double* C1, double* C2,
+                            double* K)
+{
+  o_state q;
+  o_get_state(pr, G, g->gam, &q);
+  const double uu1 = (r - g->R0) * q.ucon[1];
+  const double n = 1 / (g->gam - 1);
+  const double p = (g->gam - 1) * pr[O_UU];
+  const double rho = pr[O_RHO];
+  const double T = p / rho;
+  *C1 = pow(T, n) * uu1 * r * r;
+  *C2 = pow(1 + (1 + n) * T, 2.) * (1 - 2. / r + uu1 * uu1);
+  *K = p / pow(rho, g->gam);
+}
+
+/* bondi_invert_integrals (bounds.sasha.c:1101-1160), its iterations as
+ * written (the max_iter test after the loops never fires there either). */
+static int bondi_invert(double n, double C1, double C2, double K, double r, double* primout)
+{
+  const double relative_accuracy = 1.0e-15;
+  const int max_iter = 250;
+  double T0 = 0.2, T1 = 0.1, u1;
+  int iter;
+  if (r > 0.35 && r < 2.5) {
+    for (iter = 0; iter < max_iter && fabs((T0 - T1) / T1) > relative_accuracy; iter++) {
+      T0 = T1;
+      u1 = C1 / (r * r * pow(T0, n));
+      T1 = T0 + 0.02 * (T0 - (sqrt(C2 / (1 - 2 / r + u1 * u1)) - 1) / (1 + n));
+    }
+  } else if (r > 9) {
+    for (iter = 0; iter < max_iter && (T0 - T1) / T1 > relative_accuracy; iter++) {
+      T0 = T1;
+      u1 = C1 / (r * r * pow(T0, n));
+      T1 = (sqrt(C2 / (1 - 2 / r + u1 * u1)) - 1) / (1 + n);
+    }
+    if (T1 > T0) return 1;
+  } else
+    return 1;
+  for (int pl = 0; pl < NPR; pl++) primout[pl] = 0.0;
+  const double rho = pow(T1 / K, n);
+  primout[O_RHO] = rho;
+  primout[O_UU] = rho * T1 * n;
+  primout[O_U1] = C1 / (pow(T1, n) * r * r);
+  return 0;
+}
+
+/* bondi_generate_solution (bounds.sasha.c:946) at ghost zone (i, j): the
+ * Kerr-Schild (spin 0) u^t from the normalisation (pr2ucon VEL4: g_tt =
+ * -(1-2/r), g_tr = 2/r, g_rr = 1+2/r), the 4-velocity taken to the MKS
+ * coordinates (mettometpface: u^X1 = u^r/(r - R0)), then the relative
+ * 4-velocity u~^i = u^i - u^t g^ti/g^tt (ucon2pr). */
+static void bondi_ghost(const o_grid* g, const dims_t* d, double C1, double C2, double K, int i, int j,
+                        double* primout)
+{
+  const double r = g->R0 + exp(g->startx[1] + (i + 0.5) * g->dx[1]);
+  if (bondi_invert(1 / (g->gam - 1), C1, C2, K, r, primout) != 0) {
+    for (int pl = 0; pl < NPR; pl++) primout[pl] = nan("");
+    return;
+  }
+  const double ur = primout[O_U1];
+  const double gtt = -(1. - 2. / r), gtr = 2. / r, grr = 1. + 2. / r;
+  const double AA = gtt, BB = 2. * gtr * ur, CC = 1. + grr * ur * ur;
+  const double ut = (-BB - sqrt(BB * BB - 4. * AA * CC)) / (2. * AA);
+  const double u1 = ur / (r - g->R0);
+  o_geom G;
+  geom_at(g, d, 0, i, j, &G);
+  primout[O_U1] = u1 - ut * G.gcon[0][1] / G.gcon[0][0];
+  primout[O_U2] = 0.0 - ut * G.gcon[0][2] / G.gcon[0][0];
+  primout[O_U3] = 0.0 - ut * G.gcon[0][3] / G.gcon[0][0];
+}
+
+/* X1DN BONDIINTOUTFLOW and X1UP BONDIMDOTOUTFLOW (bounds.sasha.c:182-198,
+ * :341-353) with BONDIEXTRAPORDER 1: the integrals of the boundary zone are
+ * carried unchanged into every ghost zone (outside: C1 only, with the exact
+ * C2 = 1.373125 and K = 1). */
+static void bondi_x1(const o_grid* g, const dims_t* d, double* p)
+{
+#pragma omp parallel for collapse(2) num_threads(g->nthreads > 0 ? g->nthreads : 1)
+  for (int k = 0; k < d->n3; k++)
+    for (int j = 0; j < d->n2; j++)
+      for (int up = 0; up < 2; up++) {
+        const int ri = up ? d->n1 - 1 : 0;
+        o_geom Gr;
+        geom_at(g, d, 0, ri, j, &Gr);
+        const double rr = g->R0 + exp(g->startx[1] + (ri + 0.5) * g->dx[1]);
+        double C1, C2, K;
+        bondi_integrals(g, p + ix(d, ri, j, k), &Gr, rr, &C1, &C2, &K);
+        if (up) {
+          C2 = 1.373125;
+          K = 1.;
+        }
+        for (int gs = 0; gs < d->ng1; gs++) {
+          const int i = up ? d->n1 + gs : -1 - gs;
+          bondi_ghost(g, d, C1, C2, K, i, j, p + ix(d, i, j, k));
+        }
+      }
+}
+
 static void bound_x1(const o_grid* g, const dims_t* d, double* p)
 {
+  if (g->problem == 1) {
+    bondi_x1(g, d, p);
+    return;
+  }
 #pragma omp parallel for collapse(2) num_threads(g->nthreads > 0 ? g->nthreads : 1)
   for (int k = 0; k < d->n3; k++)
     for (int j = 0; j < d->n2; j++) {
@@ -286,6 +393,11 @@ static void x2_poledeath(const o_grid* g, const dims_t* d, double* p, int side)
 
 static void bound_x2(const o_grid* g, const dims_t* d, double* p)
 {
+  if (g->problem == 1) {  /* Bondi: POLARAXIS, a plain reflection (bounds.sasha.c:170-180) */
+    x2_reflect(g, d, p, 0);
+    x2_reflect(g, d, p, 1);
+    return;
+  }
   x2_reflect(g, d, p, 0);
   x2_poledeath(g, d, p, 0);
   x2_reflect(g, d, p, 1);

@@ -26,6 +26,8 @@ typedef struct o_grid {
   const double* gdet_corn3; /* (j*ns1+i) */
   const double* conn;       /* (j*ns1+i)*40 + c */
   const double* rcent;      /* r at zone centre per i (ns1) */
+  int problem;              /* 0 torus, 1 Bondi: the boundary-condition set */
+  double R0;                /* r = R0 + exp(X1) */
 } o_grid;
 
 /* storage helpers */

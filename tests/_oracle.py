@@ -129,7 +129,7 @@ class OGridStruct(ctypes.Structure):
         ("dx", ctypes.c_double * 4), ("startx", ctypes.c_double * 4),
         ("rhomin", ctypes.c_double), ("uumin", ctypes.c_double),
         ("geom", ctypes.c_void_p * 4), ("gdet_corn3", ctypes.c_void_p), ("conn", ctypes.c_void_p),
-        ("rcent", ctypes.c_void_p),
+        ("rcent", ctypes.c_void_p), ("problem", ctypes.c_int), ("R0", ctypes.c_double),
     ]
 
 
@@ -184,6 +184,7 @@ class OGrid:
         g.gdet_corn3 = self.gc3.ctypes.data
         g.conn = self.conn.ctypes.data
         g.rcent = self.rc.ctypes.data
+        g.problem, g.R0 = cfg.problem, cfg.R0
         self.g = g
         self.dx = [dx[a] for a in range(4)]
 
@@ -199,6 +200,15 @@ class OGrid:
         p = self.zeros()
         H.check(H.lib().harm_host_init_torus(ctypes.byref(self.cfg), rin, rmax, kappa, beta, randfact, seed,
                                              p.ctypes.data), "host_init_torus")
+        self.bound(p)
+        return p
+
+    def init_bondi(self):
+        """The Bondi flow on the active zones (the library's host generator,
+        pinned against the reference's tables), then bounded by the oracle."""
+        import harm_harm_amd as H
+        p = self.zeros()
+        H.check(H.lib().harm_host_init_bondi(ctypes.byref(self.cfg), p.ctypes.data), "host_init_bondi")
         self.bound(p)
         return p
 
