@@ -357,28 +357,91 @@ def corner_divb_max(g, geom_cent):
 def large_bench(args):
     """The 3-D torus at 384x192x192 (14.2 M zones) with the plain step:
     zone-updates/s, inversion failures, the corner divergence of B and the
-    accumulated accretion sums after the timed steps."""
+    accumulated accretion sums after the timed steps.  With --mlp, the chosen
+    form's learned guess: the network is trained on the GPU from zones of the
+    state after the first 200 plain steps, then the same number of steps is
+    timed from the same state plainly and with every inversion seeded by the
+    network, and one step of each is compared."""
+    import numpy as np
     import harm_harm_amd as H
     from harm_harm_amd.grid import Grid
+    log = lambda *a: print(*a, file=sys.stderr, flush=True)
     cfg = H.default_config(384, 192, 192, recon="weno5", flux="hll", timeorder=3)
     g = Grid(cfg)
     g.init_torus(seed=1)
     dt0 = initial_dt(g)
     g.set_time(0.0, dt0)
+    g.tune(reps=3)
+    zones = cfg.n1 * cfg.n2 * cfg.n3
+    mlp_out = None
+    if args.mlp:
+        from harm_harm_amd.mlp import MLP
+        t0 = time.time()
+        g.step(args.mlp_plain_steps)
+        g.sync()
+        n = args.mlp_samples
+        dU, dgeo, dp = g.sample_zones(n, seed=11)
+        X, T, ok = MLP.pairs(dU, dgeo, dp, n, gammafail=cfg.gammafail)
+        okh = ok.numpy()
+        order = np.nonzero(okh)[0].astype(np.int32)
+        np.random.default_rng(0).shuffle(order)
+        m = MLP(seed=1)
+        m.fit_norm(X, T, ok, n)
+        losses = m.train(X, T, n, H.DeviceArray.from_numpy(order), args.mlp_train_steps, lr=3e-3, lr_end=3e-5,
+                         chunk=max(1, args.mlp_train_steps // 4))
+        t_train = time.time() - t0
+        log(f"mlp: {args.mlp_plain_steps} plain steps, {n} sampled zones, training losses {losses}")
     g.step(args.warmup)
     g.sync()
+    p_start, (t_start, dt_start) = g.get("p"), g.time()
+    n_start = g.nstep()
+    g.set_time(t_start, dt_start, n_start)
     t0 = time.perf_counter()
     g.step(args.steps)
     g.sync()
     el = time.perf_counter() - t0
-    zones = cfg.n1 * cfg.n2 * cfg.n3
     (_, _), (nstroke, nfail) = g.newton_stats()
     steps_done = g.nstep()
     t_sim, dt_now = g.time()
     divb = corner_divb_max(g, g.geometry(0))
     fa = g.boundary_fluxes()
+    if args.mlp:
+        # the same steps from the same state with the learned guess
+        g.set("p", p_start)
+        g.set_time(t_start, dt_start, n_start)
+        g.set_inversion_guess(m)
+        g.step(1)  # capture
+        g.set("p", p_start)
+        g.set_time(t_start, dt_start, n_start)
+        g.sync()
+        t0 = time.perf_counter()
+        g.step(args.steps)
+        g.sync()
+        el_m = time.perf_counter() - t0
+        (_, _), (ns_m, nf_m) = g.newton_stats()
+        _, fb_m = g.guess_stats()
+        # one step of each from the same state
+        g.set("p", p_start)
+        g.set_time(t_start, dt_start, n_start)
+        g.step(1)
+        a = g.active(g.get("p"))
+        g.set_inversion_guess(None)
+        g.set("p", p_start)
+        g.set_time(t_start, dt_start, n_start)
+        g.step(1)
+        b = g.active(g.get("p"))
+        rel = max(float(np.abs(a[..., v] - b[..., v]).max() / max(np.abs(b[..., v]).max(), 1e-300)) for v in range(8))
+        mlp_out = {"value": zones * args.steps / el_m, "ms_per_step": el_m / args.steps * 1e3,
+                   "speedup_over_plain": el / el_m,
+                   "newton_iterations_per_zone_update": ns_m / (zones * 3.0 * args.steps),
+                   "inversion_failures": nf_m, "fallbacks_to_plain_guess": fb_m,
+                   "one_step_max_rel_difference_vs_plain": rel,
+                   "training": {"plain_steps_before": args.mlp_plain_steps, "sampled_zones": args.mlp_samples,
+                                "adam_steps": args.mlp_train_steps, "final_loss": losses[-1],
+                                "seconds": t_train}}
     out = {
-        "metric": "zone-updates/s, 3D FM torus 384x192x192 FP64, plain step",
+        "metric": "zone-updates/s, 3D FM torus 384x192x192 FP64, plain step" + (" vs MLP-guess inversion"
+                                                                                if args.mlp else ""),
         "value": zones * args.steps / el,
         "unit": "zone-updates/s",
         "n_gpus": 1,
@@ -394,10 +457,11 @@ def large_bench(args):
                    "recon": "WENO5", "flux": "HLL", "ct": "Toth", "rk": 3, "cour": cfg.cour,
                    "zones": zones, "sim_time_end": t_sim, "dt": dt_now},
         "steps_total": steps_done,
-        "newton_iterations_per_zone_update": nstroke / (zones * 3.0 * steps_done),
+        "newton_iterations_per_zone_update": nstroke / (zones * 3.0 * args.steps),
         "inversion_failures": nfail,
         "divb_max_relative": divb,
         "accretion_sums": {"mdot_in": fa[0], "edot_in": fa[1], "mdot_out": fa[2], "edot_out": fa[3]},
+        "mlp_guess": mlp_out,
     }
     print(json.dumps(out))
 
@@ -419,6 +483,10 @@ def main():
     ap.add_argument("--n", type=int, default=1 << 24, help="inversion microbench: zones")
     ap.add_argument("--train-n", type=int, default=1 << 18, help="inversion microbench: training pairs")
     ap.add_argument("--train-steps", type=int, default=20000, help="inversion microbench: Adam steps")
+    ap.add_argument("--mlp", action="store_true", help="large: also time the step with the learned inversion guess")
+    ap.add_argument("--mlp-plain-steps", type=int, default=200, help="large --mlp: plain steps before training")
+    ap.add_argument("--mlp-samples", type=int, default=1 << 20, help="large --mlp: zones sampled for training")
+    ap.add_argument("--mlp-train-steps", type=int, default=3000, help="large --mlp: Adam steps")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="before the run, one device-to-device copy of this many MiB (the reference dispatch "
                          "for scaling HBM byte counters)")
@@ -452,6 +520,7 @@ def main():
     g.init_torus(seed=1)
     dt0 = initial_dt(g)
     g.set_time(0.0, dt0)
+    tiles = g.tune(reps=5)  # face-tile depths of directions 2 and 3 (results do not depend on them)
     g.step(args.warmup)
     g.sync()
 
@@ -518,6 +587,7 @@ def main():
         "config": {"workload": f"3D FM torus {n1}x{n2}x{n3} MKS Kerr-Schild a=0.9375, plain step",
                    "recon": "WENO5", "flux": "HLL", "ct": "Toth", "rk": 3, "cour": cfg.cour,
                    "form": "plain (multi-kernel step, Newton from previous primitives)",
+                   "tile_depth_dir2_dir3": tiles[2:4],
                    "sim_time_end": t_sim, "dt": dt_now},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS if achieved else None,

@@ -93,6 +93,9 @@ _OPTIONAL_PROTOS = [
     ("harm_init_bondi", _I, [_P]),
     ("harm_host_init_bondi", _I, [_P, _P]),
     ("harm_bondi_state", _I, [_D, _D, _P]),
+    ("harm_set_inversion_guess", _I, [_P, _P]),
+    ("harm_get_guess_stats", _I, [_P, _P, _P]),
+    ("harm_grid_sample_zones", _I, [_P, _I, _I, _L, ctypes.c_uint64, _P, _P, _P]),
     ("harm_get_newton_stats", _I, [_P, _P, _P]),
     ("harm_init_torus", _I, [_P, _D, _D, _D, _D, _D, ctypes.c_uint32]),
     ("harm_fluxcalc", _I, [_P, _I, _I, _P]),

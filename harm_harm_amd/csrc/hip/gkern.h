@@ -19,7 +19,9 @@ enum { DS_DT = 0, DS_T = 1, DS_NDT1 = 2, DS_NDT2 = 3, DS_NDT3 = 4, DS_ACCDT = 5,
        DS_FLUXACC = 8, DS_N = 16 };
 // Newton counters: the current step's, and their sums over the steps since
 // the time was last set
-enum { DI_NSTROKE = 0, DI_NFAIL = 1, DI_NSTROKE_SUM = 2, DI_NFAIL_SUM = 3, DI_N = 4 };
+// and the zones whose learned-guess solve fell back to the plain guess
+enum { DI_NSTROKE = 0, DI_NFAIL = 1, DI_NSTROKE_SUM = 2, DI_NFAIL_SUM = 3, DI_NFALLBACK = 4, DI_NFALLBACK_SUM = 5,
+       DI_N = 8 };
 
 struct Params {
   double gam, cour, safe, gammamax, gammafail;
@@ -147,6 +149,24 @@ __device__ __forceinline__ T* gmem(T* p)
   asm("" : "+s"(g));
   return (T*)g;
 }
+
+// Active-zone order for the per-zone kernels: i fastest (coalesced), then k,
+// then j.  Consecutive rows are consecutive phi planes of the same (i, j)
+// row, so the 2-D geometry and connection rows stay in L2 across phi; blocks
+// are remapped (xcd_tile) so each XCD works on a contiguous range of rows.
+__device__ __forceinline__ void zone_ijk(const Dims& d, int t, int& i, int& j, int& k)
+{
+  const int r = t / d.n1;
+  i = t - r * d.n1;
+  j = r / d.n3;
+  k = r - j * d.n3;
+}
+
+// The step's inversion started from a learned W' (mlp.hip): harm_step's
+// k_invert with the network in front of the Newton solve.
+harm_status launch_invert_grid_mlp(const harm_mlp* m, const hd::Dims& d, const Params& P, const double* U,
+                                   const double* pg, double* pf, int32_t* pflag, const double* gcent,
+                                   const double* floors, int finalstep, unsigned long long* di, hipStream_t s);
 
 // Launchers of the face-flux kernels (flux.hip): one direction of one grid,
 // or one direction of every grid of a batch.

@@ -23,6 +23,7 @@
 #include <algorithm>
 #include "dgrid.h"
 #include "gkern.h"
+#include "dinvert.h"
 #include "hhost.h"
 #include "../host/hgeom.h"
 #include "../host/hinit.h"
@@ -34,17 +35,6 @@ using namespace hk;
 
 namespace {
 
-// Active-zone order for the per-zone kernels: i fastest (coalesced), then k,
-// then j.  Consecutive rows are consecutive phi planes of the same (i, j)
-// row, so the 2-D geometry and connection rows stay in L2 across phi; blocks
-// are remapped (xcd_tile) so each XCD works on a contiguous range of rows.
-__device__ __forceinline__ void zone_ijk(const Dims& d, int t, int& i, int& j, int& k)
-{
-  const int r = t / d.n1;
-  i = t - r * d.n1;
-  j = r / d.n3;
-  k = r - j * d.n3;
-}
 
 // Toth EMFs at corners, each multiplied by the corner gdet (CORNGDETVERSION)
 __device__ __forceinline__ void emf_body(Dims d, int ni, int nj, int nk, const double* __restrict__ F1,
@@ -130,83 +120,6 @@ __global__ __launch_bounds__(kBlock) void k_ct_all(Dims d, int ni, int nj, int n
                                                    double* __restrict__ F3)
 {
   ct_all_body(d, ni, nj, nk, emf, F1, F2, F3, xcd_tile(blockIdx.x, gridDim.x));
-}
-
-// ---------------------------------------------------------------------------
-// inversion + fixup1zone (floors via ZAMO-frame mass/energy addition)
-// ---------------------------------------------------------------------------
-// One zone: conserved Uz (with gdet) in registers, the guess read from pg at
-// offset o, the result written to pf and pflag.  Returns the flag; the Newton
-// stroke count is added to ns.
-__device__ __forceinline__ int invert_zone(const Dims& d, const Params& P, double* Uz,
-                                           const double* __restrict__ pg, double* __restrict__ pf,
-                                           int32_t* __restrict__ pflag, const double* __restrict__ gcent,
-                                           const double* __restrict__ floors, int finalstep, long o, long pp, int& ns)
-{
-  const long nc = d.ncell;
-  Geom G;
-  hd::load_geom(gcent, pp, d.nplane, G);
-  double pz[HARM_NPR];
-#pragma unroll
-  for (int pl = 0; pl < HARM_NPR; pl++) pz[pl] = pg[pl * nc + o];  // guess: the substep's starting primitives
-  int fl = hd::utoprimgen(Uz, G, P.gam, P.gammafail, pz, ns, __builtin_nan(""));
-  // The geometry is needed again only for the floors below.  Reloading it
-  // through a pointer the compiler cannot see through frees its 54
-  // registers during the Newton iterations (they come back from L1/L2).
-  hd::load_geom(hd::fresh(gcent), pp, d.nplane, G);
-  if (finalstep) {
-    double prfloor[2] = {floors[pp], floors[d.nplane + pp]};
-    for (int pl = 0; pl < 2; pl++)
-      if (prfloor[pl] < D_SMALL) prfloor[pl] = D_SMALL;
-    double prnew[HARM_NPR];
-    bool didchange = false;
-#pragma unroll
-    for (int pl = 0; pl < HARM_NPR; pl++) {
-      if (pl <= HARM_UU && prfloor[pl] > pz[pl]) { didchange = true; prnew[pl] = prfloor[pl]; }
-      else prnew[pl] = pz[pl];
-    }
-    if (didchange) {
-      Geom G1 = G;
-      G1.gdet = 1.0;
-      State q, dq;
-      double Ua[HARM_NPR], dpr[HARM_NPR], dUa[HARM_NPR];
-      hd::get_state(pz, G, P.gam, q);
-      hd::primtoflux(pz, q, 0, G1, Ua);
-#pragma unroll
-      for (int pl = 0; pl < HARM_NPR; pl++) dpr[pl] = 0.0;
-      for (int pl = HARM_RHO; pl <= HARM_UU; pl++) dpr[pl] = prnew[pl] - pz[pl];
-      hd::get_state(dpr, G, P.gam, dq);
-      hd::primtoflux(dpr, dq, 0, G1, dUa);
-#pragma unroll
-      for (int pl = 0; pl < HARM_NPR; pl++) Ua[pl] += dUa[pl];
-      fl = hd::jon(Ua, G, P.gam, P.gammafail, pz, ns, __builtin_nan(""));
-      if (fl > 0) {
-#pragma unroll
-        for (int pl = 0; pl < HARM_NPR; pl++) pz[pl] = prnew[pl];
-      }
-    }
-    hd::limit_gamma(P.gammamax, pz, G);
-  }
-#pragma unroll
-  for (int pl = 0; pl < HARM_NPR; pl++) pf[pl * nc + o] = pz[pl];
-  pflag[o] = fl;
-  return fl;
-}
-
-// Block sums of the Newton strokes and failures into di (integer sums are
-// order independent).  Every thread of the block calls it.
-__device__ __forceinline__ void newton_counts_to(int ns, int nf, unsigned long long* __restrict__ di)
-{
-  __shared__ unsigned long long red[2];
-  if (threadIdx.x == 0) { red[0] = 0; red[1] = 0; }
-  __syncthreads();
-  if (ns) atomicAdd(&red[0], (unsigned long long)ns);
-  if (nf) atomicAdd(&red[1], 1ull);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    if (red[0]) atomicAdd(&di[DI_NSTROKE], red[0]);
-    if (red[1]) atomicAdd(&di[DI_NFAIL], red[1]);
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -802,6 +715,7 @@ __global__ void k_scal_begin_step(double* ds, unsigned long long* di)
   ds[DS_LASTNDT] = kBIG;
   di[DI_NSTROKE] = 0;
   di[DI_NFAIL] = 0;
+  di[DI_NFALLBACK] = 0;
 }
 __global__ void k_scal_substep_dt(double* ds)
 {
@@ -819,6 +733,7 @@ __global__ void k_scal_end_step(double* ds, unsigned long long* di, double safe)
   ds[DS_NSTEP] = ds[DS_NSTEP] + 1.0;
   di[DI_NSTROKE_SUM] += di[DI_NSTROKE];
   di[DI_NFAIL_SUM] += di[DI_NFAIL];
+  di[DI_NFALLBACK_SUM] += di[DI_NFALLBACK];
 }
 
 // Boundary flux sums through the inner and outer radial faces
@@ -889,6 +804,7 @@ struct harm_handle {
   double* rcent = nullptr;
   double* emf = nullptr;
   double* facc = nullptr;  // boundary-flux partial sums
+  const harm_mlp* guess = nullptr;  // learned inversion guess (harm_set_inversion_guess), or none
   int tile_td[4] = {0, 0, 0, 0};  // sweep depth of the face tiles of dirs 2,3 (0: cost model)
   hipGraph_t sgraph = nullptr;      // one full step, captured on first use when not profiling
   hipGraphExec_t sexec = nullptr;
@@ -1175,9 +1091,14 @@ harm_status enq_invert(harm_handle* h, int which_pb, int which_pf, int finalstep
     HH_HIP(hipMemcpyAsync(h->arr[which_pf], h->arr[which_pb], bytes, hipMemcpyDeviceToDevice, h->stream));
   const double* U = h->arr[finalstep ? HARM_ARR_UCUM : HARM_ARR_UF];
   prof_begin(h);
-  hipLaunchKernelGGL(k_invert, dim3(nblocks((long)d.n1 * d.n2 * d.n3)), dim3(kBlock), 0, h->stream, d, h->P, U,
-                     h->arr[which_pb], h->arr[which_pf], h->pflag, h->geom[HARM_LOC_CENT], h->floors, finalstep,
-                     h->di);
+  if (h->guess) {
+    HH_TRY(launch_invert_grid_mlp(h->guess, d, h->P, U, h->arr[which_pb], h->arr[which_pf], h->pflag,
+                                  h->geom[HARM_LOC_CENT], h->floors, finalstep, h->di, h->stream));
+  } else {
+    hipLaunchKernelGGL(k_invert, dim3(nblocks((long)d.n1 * d.n2 * d.n3)), dim3(kBlock), 0, h->stream, d, h->P, U,
+                       h->arr[which_pb], h->arr[which_pf], h->pflag, h->geom[HARM_LOC_CENT], h->floors, finalstep,
+                       h->di);
+  }
   prof_end(h, HARM_K_INVERT);
   return hh::check_launch("k_invert");
 }
@@ -1425,6 +1346,7 @@ __global__ void kb_scal(const GridRef* __restrict__ G, int ng, int what)
     ds[DS_LASTNDT] = kBIG;
     G[gi].di[DI_NSTROKE] = 0;
     G[gi].di[DI_NFAIL] = 0;
+    G[gi].di[DI_NFALLBACK] = 0;
   } else if (what == 2) {  // substep dt
     const double wavedt = 1. / (1. / ds[DS_NDT1] + 1. / ds[DS_NDT2] + 1. / ds[DS_NDT3]);
     const double nd = 1.0 * fmin(wavedt, ds[DS_ACCDT]);
@@ -1440,6 +1362,7 @@ __global__ void kb_scal(const GridRef* __restrict__ G, int ng, int what)
     unsigned long long* di = G[gi].di;
     di[DI_NSTROKE_SUM] += di[DI_NSTROKE];
     di[DI_NFAIL_SUM] += di[DI_NFAIL];
+    di[DI_NFALLBACK_SUM] += di[DI_NFALLBACK];
   }
 }
 
@@ -1812,8 +1735,10 @@ extern "C" harm_status harm_set_time(harm_handle* h, double t, double dt, int64_
   v[DS_T] = t;
   v[DS_NSTEP] = (double)nstep;
   HH_HIP(hipMemcpy(h->ds, v, sizeof(v), hipMemcpyHostToDevice));
-  const unsigned long long z[2] = {0, 0};  // the Newton sums restart with the clock
-  return hh::check_hip(hipMemcpy(h->di + DI_NSTROKE_SUM, z, sizeof(z), hipMemcpyHostToDevice), "set_time");
+  unsigned long long z[DI_N];  // the Newton sums restart with the clock
+  HH_HIP(hipMemcpy(z, h->di, sizeof(z), hipMemcpyDeviceToHost));
+  z[DI_NSTROKE_SUM] = z[DI_NFAIL_SUM] = z[DI_NFALLBACK_SUM] = 0;
+  return hh::check_hip(hipMemcpy(h->di, z, sizeof(z), hipMemcpyHostToDevice), "set_time");
 }
 
 extern "C" harm_status harm_get_time(harm_handle* h, double* t, double* dt, int64_t* nstep)
@@ -1842,6 +1767,67 @@ extern "C" harm_status harm_get_newton_stats(harm_handle* h, harm_newton_stats* 
     total->nstroke = (int64_t)z[DI_NSTROKE_SUM];
     total->nfail = (int64_t)z[DI_NFAIL_SUM];
   }
+  return HARM_OK;
+}
+
+// n zones drawn (with replacement, a seeded hash of the draw index) from the
+// active zones: U and p at the zone, and its 27-field geometry record, into
+// SoA buffers of n entries (the layout of harm_invert_batch and
+// harm_mlp_pairs).
+__global__ __launch_bounds__(kBlock) void k_sample_zones(Dims d, const double* __restrict__ U,
+                                                         const double* __restrict__ p,
+                                                         const double* __restrict__ gcent, long n, uint64_t seed,
+                                                         double* __restrict__ Uo, double* __restrict__ go,
+                                                         double* __restrict__ po)
+{
+  const long s = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (s >= n) return;
+  uint64_t x = seed + 0x9E3779B97F4A7C15ull * (uint64_t)(s + 1);  // splitmix64 of the draw index
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  x ^= x >> 31;
+  const int nact = d.n1 * d.n2 * d.n3;
+  int i, j, k;
+  zone_ijk(d, (int)(x % (uint64_t)nact), i, j, k);
+  const long o = hd::cidx(d, i, j, k), pp = hd::pidx(d, i, j);
+#pragma unroll
+  for (int pl = 0; pl < HARM_NPR; pl++) {
+    Uo[pl * n + s] = U[pl * d.ncell + o];
+    po[pl * n + s] = p[pl * d.ncell + o];
+  }
+  for (int f = 0; f < HARM_G_NFIELD; f++) go[f * n + s] = gcent[f * d.nplane + pp];
+}
+
+extern "C" harm_status harm_grid_sample_zones(harm_handle* h, int32_t which_U, int32_t which_p, int64_t n,
+                                              uint64_t seed, double* U, double* geom, double* p)
+{
+  HH_TRY(check_handle(h));
+  HH_TRY(check_which(which_U));
+  HH_TRY(check_which(which_p));
+  if (n <= 0 || !U || !geom || !p) return hh::set_error(HARM_EINVAL, "harm_grid_sample_zones: bad arguments");
+  hipLaunchKernelGGL(k_sample_zones, dim3(nblocks(n)), dim3(kBlock), 0, h->stream, h->d, h->arr[which_U],
+                     h->arr[which_p], h->geom[HARM_LOC_CENT], (long)n, seed, U, geom, p);
+  HH_TRY(hh::check_launch("k_sample_zones"));
+  return harm_sync(h);
+}
+
+extern "C" harm_status harm_set_inversion_guess(harm_handle* h, const harm_mlp* m)
+{
+  HH_TRY(check_handle(h));
+  HH_HIP(hipStreamSynchronize(h->stream));
+  h->guess = m;
+  drop_step_graph(h);  // the captured step holds the other inversion kernel
+  return HARM_OK;
+}
+
+extern "C" harm_status harm_get_guess_stats(harm_handle* h, int64_t* fallbacks_last_step, int64_t* fallbacks_total)
+{
+  HH_TRY(check_handle(h));
+  unsigned long long z[DI_N];
+  HH_HIP(hipStreamSynchronize(h->stream));
+  HH_HIP(hipMemcpy(z, h->di, sizeof(z), hipMemcpyDeviceToHost));
+  if (fallbacks_last_step) *fallbacks_last_step = (int64_t)z[DI_NFALLBACK];
+  if (fallbacks_total) *fallbacks_total = (int64_t)z[DI_NFALLBACK_SUM];
   return HARM_OK;
 }
 
@@ -1904,8 +1890,11 @@ extern "C" harm_status harm_invert_grid(harm_handle* h, int32_t which_pb, int32_
   HH_TRY(check_handle(h));
   HH_TRY(check_which(which_pb));
   HH_TRY(check_which(which_pf));
-  unsigned long long z[DI_N] = {0};
-  HH_HIP(hipMemcpyAsync(h->di, z, sizeof(z), hipMemcpyHostToDevice, h->stream));
+  unsigned long long z[DI_N];
+  HH_HIP(hipStreamSynchronize(h->stream));
+  HH_HIP(hipMemcpy(z, h->di, sizeof(z), hipMemcpyDeviceToHost));
+  z[DI_NSTROKE] = z[DI_NFAIL] = z[DI_NFALLBACK] = 0;  // this call's counts (the step sums stay)
+  HH_HIP(hipMemcpy(h->di, z, sizeof(z), hipMemcpyHostToDevice));
   HH_TRY(enq_invert(h, which_pb, which_pf, finalstep, true));
   if (stats) {
     HH_HIP(hipStreamSynchronize(h->stream));

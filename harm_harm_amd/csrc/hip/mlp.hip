@@ -20,6 +20,8 @@
 #include <algorithm>
 #include "mlp.h"
 #include "hhost.h"
+#include "gkern.h"
+#include "dinvert.h"
 
 using hm::v4f;
 
@@ -500,6 +502,74 @@ harm_status pack(harm_mlp* m, hipStream_t s)
   return hh::check_launch("k_mlp_pack");
 }
 
+// The step's inversion with the learned guess (harm_set_inversion_guess):
+// the zones of k_invert in the same order; per wave the features of 64 zones
+// go through the network on the matrix cores, each zone's Newton solve then
+// starts from the predicted W' (falling back to the previous primitives when
+// it fails), and the final substep's floors follow, as in k_invert.
+__global__ __launch_bounds__(kBlock) void k_invert_grid_mlp(const uint16_t* __restrict__ wbf,
+                                                            const float* __restrict__ w32,
+                                                            const float* __restrict__ norm, hd::Dims d, hk::Params P,
+                                                            const double* __restrict__ U,
+                                                            const double* __restrict__ pg, double* __restrict__ pf,
+                                                            int32_t* __restrict__ pflag,
+                                                            const double* __restrict__ gcent,
+                                                            const double* __restrict__ floors, int finalstep,
+                                                            unsigned long long* __restrict__ di)
+{
+  __shared__ Fwd S;
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  stage_weights(wbf, w32, S.sW, S.sBias);
+  const int t = hk::xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  const bool act = t < d.n1 * d.n2 * d.n3;
+  long o = 0, pp = 0;
+  double D = 1.0;
+  {
+    float f[hm::kIn] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (act) {
+      int i, j, k;
+      hk::zone_ijk(d, t, i, j, k);
+      o = hd::cidx(d, i, j, k);
+      pp = hd::pidx(d, i, j);
+      hd::Geom g;
+      hd::load_geom(gcent, pp, d.nplane, g);
+      const double ig = (g.gdet >= 0.0 ? 1.0 : -1.0) / (fabs(g.gdet) + D_SMALL);
+      double Ut[HARM_NPR];
+#pragma unroll
+      for (int pl = 0; pl < HARM_NPR; pl++) Ut[pl] = g.alpha * (U[pl * d.ncell + o] * ig);
+      hd::Inv s;
+      double Qtcon[4], Bcon[4];
+      hd::inv_setup(Ut, g, P.gam, P.gammafail, s, Qtcon, Bcon);
+      hm::features(s, f);
+      D = s.D;
+    }
+#pragma unroll
+    for (int k = 0; k < hm::kInPad; k++) {
+      float v = 0.0f;
+      if (act && k < hm::kIn) v = (f[k] - norm[k]) * norm[hm::kIn + k];
+      S.bufA[w][l * hm::kInPad + k] = hm::f2bf(v);
+    }
+  }
+  __syncthreads();
+  wave_forward(S.bufA[w], S.bufB[w], S.sW, S.sBias, S.y[w]);
+  int ns = 0, nf = 0, fb = 0;
+  if (act) {
+    const float y0 = S.y[w][l * 2] * norm[2 * hm::kIn + hm::kOut] + norm[2 * hm::kIn];
+    const double seed = D * exp((double)fminf(fmaxf(y0, -60.0f), 60.0f));
+    double Uz[HARM_NPR];
+#pragma unroll
+    for (int pl = 0; pl < HARM_NPR; pl++) Uz[pl] = U[pl * d.ncell + o];
+    nf = hk::invert_zone<true>(d, P, Uz, pg, pf, pflag, gcent, floors, finalstep, o, pp, ns, seed, &fb) > 0;
+  }
+  hk::newton_counts_to(ns, nf, di);
+  __shared__ unsigned int nfb;
+  if (threadIdx.x == 0) nfb = 0;
+  __syncthreads();
+  if (fb) atomicAdd(&nfb, 1u);
+  __syncthreads();
+  if (threadIdx.x == 0 && nfb) atomicAdd(&di[hk::DI_NFALLBACK], (unsigned long long)nfb);
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -676,4 +746,14 @@ extern "C" harm_status harm_invert_batch_mlp(harm_mlp* m, const double* U, const
   hipLaunchKernelGGL(k_invert_batch_mlp, dim3((unsigned)nb), dim3(kBlock), 0, (hipStream_t)stream, m->wbf, m->w32,
                      m->norm, U, geom, p_inout, pflag, niter, (long)n, gam, gammafail, nfallback);
   return hh::check_launch("k_invert_batch_mlp");
+}
+
+harm_status hk::launch_invert_grid_mlp(const harm_mlp* m, const hd::Dims& d, const hk::Params& P, const double* U,
+                                       const double* pg, double* pf, int32_t* pflag, const double* gcent,
+                                       const double* floors, int finalstep, unsigned long long* di, hipStream_t s)
+{
+  const long nb = ((long)d.n1 * d.n2 * d.n3 + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(k_invert_grid_mlp, dim3((unsigned)nb), dim3(kBlock), 0, s, m->wbf, m->w32, m->norm, d, P, U, pg,
+                     pf, pflag, gcent, floors, finalstep, di);
+  return hh::check_launch("k_invert_grid_mlp");
 }

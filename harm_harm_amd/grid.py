@@ -107,6 +107,28 @@ class Grid:
         check(lib().harm_get_time(self.h, ctypes.byref(t), ctypes.byref(dt), ctypes.byref(n)), "harm_get_time")
         return n.value
 
+    def set_inversion_guess(self, mlp):
+        """Start every inversion of this grid's steps from the network's W'
+        (an MLP), or from the previous primitives again (None)."""
+        self._guess = mlp  # kept alive while in use
+        check(lib().harm_set_inversion_guess(self.h, mlp.h if mlp is not None else None),
+              "harm_set_inversion_guess")
+
+    def guess_stats(self):
+        """(fallbacks in the last step, fallbacks since set_time)"""
+        a, b = ctypes.c_int64(), ctypes.c_int64()
+        check(lib().harm_get_guess_stats(self.h, ctypes.byref(a), ctypes.byref(b)), "harm_get_guess_stats")
+        return a.value, b.value
+
+    def sample_zones(self, n, seed=1, which_U="ucum", which_p="p"):
+        """n active zones (seeded draw): device SoA arrays U (8, n), geometry
+        (27, n), p (8, n) for MLP training pairs."""
+        from . import DeviceArray
+        U, geom, p = DeviceArray((8, n), np.float64), DeviceArray((G_NFIELD, n), np.float64), DeviceArray((8, n), np.float64)
+        check(lib().harm_grid_sample_zones(self.h, self._which(which_U), self._which(which_p), n, seed, U.ptr,
+                                           geom.ptr, p.ptr), "harm_grid_sample_zones")
+        return U, geom, p
+
     def newton_stats(self):
         """((nstroke, nfail) of the last step, (nstroke, nfail) summed since set_time)"""
         last, tot = NewtonStats(), NewtonStats()

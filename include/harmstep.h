@@ -366,6 +366,24 @@ harm_status harm_invert_batch_mlp(harm_mlp* m, const double* U, const double* ge
                                   int32_t* pflag, int32_t* niter, int64_t n, double gam, double gammafail,
                                   unsigned long long* nfallback, void* stream);
 
+/* The learned guess inside the step (replaces the per-zone Utoprimgen call of
+ * advance(), advance.c:748, as harm_invert_batch_mlp does for a batch): with
+ * m set, every inversion of the handle's steps starts its Newton solve from
+ * the network's W' and falls back to the previous primitives when that
+ * fails; floors, flags and counters are those of the plain inversion.  m
+ * NULL restores the plain inversion.  m must outlive its use by h; batches
+ * (harm_batch_*) always invert plainly. */
+harm_status harm_set_inversion_guess(harm_handle* h, const harm_mlp* m);
+/* zones whose seeded solve fell back, in the last whole step and summed
+ * since harm_set_time (either pointer may be NULL) */
+harm_status harm_get_guess_stats(harm_handle* h, int64_t* fallbacks_last_step, int64_t* fallbacks_total);
+/* Training data from a grid: n active zones drawn with a seeded hash (with
+ * replacement) with their conserved state from array which_U, primitives from
+ * which_p and geometry record, into device SoA buffers U[8][n], geom[27][n],
+ * p[8][n] (the layout of harm_mlp_pairs and harm_invert_batch). */
+harm_status harm_grid_sample_zones(harm_handle* h, int32_t which_U, int32_t which_p, int64_t n, uint64_t seed,
+                                   double* U, double* geom, double* p);
+
 #ifdef __cplusplus
 }
 #endif

@@ -131,3 +131,51 @@ def test_seeded_newton_converges_like_plain(trained):
         e_seed = np.abs(seeded[v] - p[v]).max() / np.abs(p[v]).max()
         print("var", v, "error vs truth: plain", e_plain, "seeded", e_seed)
         assert e_seed <= 10 * e_plain + 1e-12
+
+
+def test_seeded_step_matches_plain_step():
+    """The learned guess inside the step (harm_set_inversion_guess): a network
+    trained on zones sampled from a grid after plain steps seeds every
+    inversion of the next step; the step's primitives agree with the plain
+    step's to 1e-12 (both Newton solves converge to the same root within the
+    inverter's tolerance), with no extra inversion failures."""
+    from harm_harm_amd.grid import Grid
+    cfg = H.default_config(24, 16, 8, recon="weno5", flux="hll", timeorder=3)
+    g = Grid(cfg)
+    g.init_torus(seed=2)
+    ndt = g.fluxcalc("p")
+    g.set_time(0.0, 0.5 / sum(1.0 / x for x in ndt))
+    g.step(3)
+    n = 1 << 15
+    dU, dg, dp = g.sample_zones(n, seed=5)
+    X, T, ok = MLP.pairs(dU, dg, dp, n, gammafail=cfg.gammafail)
+    okh = ok.numpy()
+    order = np.nonzero(okh)[0].astype(np.int32)
+    np.random.default_rng(0).shuffle(order)
+    m = MLP(seed=1)
+    m.fit_norm(X, T, ok, n)
+    m.train(X, T, n, D.from_numpy(order), 2000, lr=3e-3, lr_end=3e-5, chunk=500)
+    p0, (t0, dt0) = g.get("p"), g.time()
+    g.set_time(t0, dt0)
+    g.step(1)
+    plain = g.get("p")
+    (ns_plain, nf_plain), _ = g.newton_stats()
+    g.set("p", p0)
+    g.set_time(t0, dt0)
+    g.set_inversion_guess(m)
+    g.step(1)
+    seeded = g.get("p")
+    (ns_seed, nf_seed), _ = g.newton_stats()
+    fb, _ = g.guess_stats()
+    g.set_inversion_guess(None)
+    zones = cfg.n1 * cfg.n2 * cfg.n3 * 3
+    err = 0.0
+    a, b = g.active(seeded), g.active(plain)
+    for v in range(8):
+        s = np.abs(b[..., v]).max()
+        if s > 0:
+            err = max(err, float(np.abs(a[..., v] - b[..., v]).max() / s))
+    print("iterations per zone-substep: plain", ns_plain / zones, "seeded", ns_seed / zones,
+          "failures", nf_plain, nf_seed, "fallbacks", fb, "relmax", err)
+    assert nf_seed <= nf_plain
+    assert err <= 1e-12
