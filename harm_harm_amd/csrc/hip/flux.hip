@@ -150,7 +150,11 @@ __device__ __forceinline__ void flux_body(Dims d, FaceTile T, const double* __re
       s_r[v * kCellMax + slot] = r;
     }
     slot += kBlock;
-    while (slot >= ncell) { slot -= ncell; v++; }
+    if (slot >= ncell) {  // ncell >= kBlock except in small tiles: one subtraction
+      slot -= ncell;
+      v++;
+      while (slot >= ncell) { slot -= ncell; v++; }
+    }
   }
   __syncthreads();
 
