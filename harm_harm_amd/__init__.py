@@ -45,6 +45,15 @@ class NewtonStats(ctypes.Structure):
     _fields_ = [("nstroke", ctypes.c_int64), ("nfail", ctypes.c_int64)]
 
 
+class HarmEner(ctypes.Structure):
+    """harm_ener: one energy-file record (dump_ener.c:391-407)."""
+    _fields_ = [("t", ctypes.c_double), ("nstep", ctypes.c_int64), ("U", ctypes.c_double * 8),
+                ("pdot", (ctypes.c_double * 8) * 6), ("pcum", (ctypes.c_double * 8) * 6),
+                ("fladd", ctypes.c_double * 8), ("fladdterms", (ctypes.c_double * 8) * 12),
+                ("sourceadd", ctypes.c_double * 8),
+                ("divbmax", ctypes.c_double), ("divbavg", ctypes.c_double)]
+
+
 _P = ctypes.c_void_p
 _D = ctypes.c_double
 _I = ctypes.c_int32
@@ -106,6 +115,9 @@ _OPTIONAL_PROTOS = [
     ("harm_advance", _I, [_P, _I, _P]),
     ("harm_step", _I, [_P, _I]),
     ("harm_get_boundary_fluxes", _I, [_P, _P]),
+    ("harm_set_ener_tracking", _I, [_P, _I]),
+    ("harm_reset_ener", _I, [_P]),
+    ("harm_get_ener", _I, [_P, _I, _P]),
     ("harm_get_stream", _I, [_P, ctypes.POINTER(_P)]),
     ("harm_profile", _I, [_P, _I]),
     ("harm_profile_read", _I, [_P, _P, _P]),

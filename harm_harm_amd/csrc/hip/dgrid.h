@@ -38,9 +38,10 @@ __device__ __forceinline__ double gamma_of(const double* pr, const Geom& G)
   return sqrt(1. + qsq);
 }
 
-__device__ __forceinline__ void limit_gamma(double gammamax, double* pr, const Geom& G)
+// returns whether the velocity was rescaled
+__device__ __forceinline__ bool limit_gamma(double gammamax, double* pr, const Geom& G)
 {
-  if (gammamax <= 1.0) return;
+  if (gammamax <= 1.0) return false;
   const double gamma = gamma_of(pr, G);
   if ((gamma > gammamax && (gamma != 1.0))) {
     const double pref = (gammamax * gammamax - 1.) / (gamma * gamma - 1.);
@@ -48,7 +49,9 @@ __device__ __forceinline__ void limit_gamma(double gammamax, double* pr, const G
     pr[HARM_U1] *= f;
     pr[HARM_U2] *= f;
     pr[HARM_U3] *= f;
+    return true;
   }
+  return false;
 }
 
 // para4 with MC-limited slopes and the CW84 monotonicity conditions.

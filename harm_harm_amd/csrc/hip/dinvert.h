@@ -24,12 +24,33 @@ using hd::State;
 // that solve fails, the zone is solved again from the primitives' guess as
 // the plain inversion does (Newton from the previous primitives is the
 // fallback and decides), and *fellback is set.
-template <bool SEEDED = false>
+// DIAG: on the final substep, the conserved change of each fix is added as
+// diag_fixup does (fixup.c:220, finalstep > 0): dV (U(after) - U(before)),
+// with U = primtoU with gdet and `before` the primitives ahead of the floors,
+// for the floor step (fixup.c:766) to dfl[0..7] and for the gamma limit
+// (fixup.c:784) to dfl[8..15]; changed[0], changed[1] say whether each was.
+// dfl[pl] += dV (U(pa)[pl] - U(pb)[pl]), U with gdet (diag_fixup, fixup.c:303-405)
+__device__ __forceinline__ void fix_delta_to(const Params& P, const Geom& G, const double* pb, const double* pa,
+                                             double* dfl)
+{
+  const double dV = P.dx[1] * P.dx[2] * P.dx[3];
+  State q;
+  double Ub[HARM_NPR], Ua[HARM_NPR];
+  hd::get_state(pb, G, P.gam, q);
+  hd::primtoflux(pb, q, 0, G, Ub);
+  hd::get_state(pa, G, P.gam, q);
+  hd::primtoflux(pa, q, 0, G, Ua);
+#pragma unroll
+  for (int pl = 0; pl < HARM_NPR; pl++) dfl[pl] += dV * (Ua[pl] - Ub[pl]);
+}
+
+template <bool SEEDED = false, bool DIAG = false>
 __device__ __forceinline__ int invert_zone(const Dims& d, const Params& P, double* Uz,
                                            const double* __restrict__ pg, double* __restrict__ pf,
                                            int32_t* __restrict__ pflag, const double* __restrict__ gcent,
                                            const double* __restrict__ floors, int finalstep, long o, long pp, int& ns,
-                                           double wp_seed = 0.0, int* fellback = nullptr)
+                                           double wp_seed = 0.0, int* fellback = nullptr, double* dfl = nullptr,
+                                           bool* changed = nullptr)
 {
   const long nc = d.ncell;
   Geom G;
@@ -49,6 +70,11 @@ __device__ __forceinline__ int invert_zone(const Dims& d, const Params& P, doubl
   // registers during the Newton iterations (they come back from L1/L2).
   hd::load_geom(hd::fresh(gcent), pp, d.nplane, G);
   if (finalstep) {
+    double p0[HARM_NPR];
+    if (DIAG) {
+#pragma unroll
+      for (int pl = 0; pl < HARM_NPR; pl++) p0[pl] = pz[pl];
+    }
     double prfloor[2] = {floors[pp], floors[d.nplane + pp]};
     for (int pl = 0; pl < 2; pl++)
       if (prfloor[pl] < D_SMALL) prfloor[pl] = D_SMALL;
@@ -78,8 +104,10 @@ __device__ __forceinline__ int invert_zone(const Dims& d, const Params& P, doubl
 #pragma unroll
         for (int pl = 0; pl < HARM_NPR; pl++) pz[pl] = prnew[pl];
       }
+      if (DIAG) { fix_delta_to(P, G, p0, pz, dfl); changed[0] = true; }
     }
-    hd::limit_gamma(P.gammamax, pz, G);
+    const bool lg = hd::limit_gamma(P.gammamax, pz, G);
+    if (DIAG && lg) { fix_delta_to(P, G, p0, pz, dfl + HARM_NPR); changed[1] = true; }
   }
 #pragma unroll
   for (int pl = 0; pl < HARM_NPR; pl++) pf[pl * nc + o] = pz[pl];

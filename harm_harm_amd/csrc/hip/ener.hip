@@ -1,0 +1,315 @@
+// ener.hip -- the energy-file diagnostics of one grid on the device
+// (dump_ener.c:5-420, diag.c:1020-1250, 1958-2000, fixup.c:220-420):
+//
+//   k_ener_pdot      boundary flux sums of the substep (diag_flux_pureflux)
+//                    over the six boundaries and eight variables, and their
+//                    time integral pcum
+//   k_ener_dt4       the weight of the sampled source (dt4diag,
+//                    advance.c:251-262)
+//   k_ener_source    the sampled geometric source (diag_source_all)
+//   k_ener_totals    conserved totals sum U dV (constotal, dump_ener.c:882)
+//   k_ener_divb      max and sum of |div B| at the corners (divbmaxavg,
+//                    dump_ener.c:680, SETFDIVBFLUXCTTOTHPRIM)
+//   k_ener_reduce    fixed-order sums of per-block partials
+//
+// The floor, gamma-limit and failure-averaging changes (fladd) are summed by
+// the inversion and failure-averaging kernels themselves (their DIAG
+// variants) into per-block partials that k_ener_reduce adds up.  Every sum
+// is reduced in a fixed order, so the same state gives the same bits.
+#include <hip/hip_runtime.h>
+#include "gkern.h"
+
+namespace hk {
+
+using hd::Geom;
+using hd::State;
+
+namespace {
+
+inline unsigned nblk(long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+// One block per (boundary, chunk): boundary bd = X1UP, X1DN, X2UP, X2DN,
+// X3UP, X3DN (the reference's order, global.nondepnmemonics.h:699); each face adds F[pl] * surface (non-finite terms count as 0, as
+// with PRODUCTION > 0 at diag.c:1196-1201).  A direction with one zone has no
+// faces to sum (its fluxes are not computed).
+__global__ __launch_bounds__(kBlock) void k_ener_pdot(Dims d, Params P, const double* __restrict__ F1,
+                                                      const double* __restrict__ F2,
+                                                      const double* __restrict__ F3, double* __restrict__ part)
+{
+  const int bd = blockIdx.x / kEnFaceBlocks, chunk = blockIdx.x % kEnFaceBlocks;
+  const int dir = bd / 2 + 1, up = !(bd & 1);
+  double v[HARM_NPR] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const int nd = dir == 1 ? d.n1 : dir == 2 ? d.n2 : d.n3;
+  if (nd > 1) {
+    const double* F = dir == 1 ? F1 : dir == 2 ? F2 : F3;
+    const double surface = dir == 1 ? P.dx[2] * P.dx[3] : dir == 2 ? P.dx[1] * P.dx[3] : P.dx[1] * P.dx[2];
+    // the face plane: (a, b) = (j, k), (i, k) or (i, j)
+    const int na = dir == 1 ? d.n2 : d.n1, nb = dir == 3 ? d.n2 : d.n3;
+    const long nf = (long)na * nb, per = (nf + kEnFaceBlocks - 1) / kEnFaceBlocks;
+    const long f0 = chunk * per, f1 = min(nf, f0 + per);
+    const long nc = d.ncell;
+    for (long f = f0 + threadIdx.x; f < f1; f += kBlock) {
+      const int a = (int)(f % na), b = (int)(f / na);
+      int i, j, k;
+      if (dir == 1) { i = up ? d.n1 : 0; j = a; k = b; }
+      else if (dir == 2) { i = a; j = up ? d.n2 : 0; k = b; }
+      else { i = a; j = b; k = up ? d.n3 : 0; }
+      const long o = hd::cidx(d, i, j, k);
+#pragma unroll
+      for (int pl = 0; pl < HARM_NPR; pl++) {
+        const double x = F[pl * nc + o] * surface;
+        v[pl] += isfinite(x) ? x : 0.0;
+      }
+    }
+  }
+  block_sum8_part(v, true, part, blockIdx.x);
+}
+
+// pdot of each boundary from its kEnFaceBlocks partials; pcum += pdot * Dt
+// with Dt = dt * weight, the substep's share of the step (diag.c:1209).
+__global__ void k_ener_pdot_final(const double* __restrict__ part, double* __restrict__ en,
+                                  const double* __restrict__ ds, double weight)
+{
+  const int t = threadIdx.x;
+  if (t >= 6 * HARM_NPR) return;
+  const int bd = t / HARM_NPR, c = t % HARM_NPR;
+  double s = 0.0;
+  for (int q = 0; q < kEnFaceBlocks; q++) s += part[(long)(bd * kEnFaceBlocks + q) * 8 + c];
+  en[EN_PDOT + t] = s;
+  en[EN_PCUM + t] += s * (ds[DS_DT] * weight);
+}
+
+// dt4diag (advance.c:251-262): on the last substep of every
+// kDiagSourceStep-th step the source is sampled with the time gathered
+// since the previous sample; every other substep adds dt to that time.
+__global__ void k_ener_dt4(double* __restrict__ en, const double* __restrict__ ds, int last)
+{
+  if (threadIdx.x != 0) return;
+  const long nstep = (long)ds[DS_NSTEP];
+  if (last && nstep % kDiagSourceStep == 0) {
+    en[EN_DT4] = en[EN_WILLBE];
+    en[EN_WILLBE] = 0.0;
+  } else {
+    en[EN_WILLBE] += ds[DS_DT];
+    en[EN_DT4] = -1.0;
+  }
+}
+
+// diag_source_all: sourceadd += Dt dV dUgeom with Dt = dt4diag > 0; dUgeom
+// is the connection source of pb (source_conn, as the update adds it), zero
+// for the density and the field.
+__global__ __launch_bounds__(kBlock) void k_ener_source(Dims d, Params P, const double* __restrict__ pb,
+                                                        const double* __restrict__ gcent,
+                                                        const double* __restrict__ conn,
+                                                        const double* __restrict__ en, double* __restrict__ part)
+{
+  const double Dt = en[EN_DT4];
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  double v[HARM_NPR] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const bool on = Dt > 0.0 && t < d.n1 * d.n2 * d.n3;
+  if (on) {
+    int i, j, k;
+    zone_ijk(d, t, i, j, k);
+    const long o = hd::cidx(d, i, j, k), pp = hd::pidx(d, i, j), nc = d.ncell, np = d.nplane;
+    Geom G;
+    hd::load_geom(gcent, pp, np, G);
+    double pz[HARM_NPR];
+#pragma unroll
+    for (int pl = 0; pl < HARM_NPR; pl++) pz[pl] = pb[pl * nc + o];
+    State q;
+    hd::get_state(pz, G, P.gam, q);
+    double dUg[4], T00;
+    const double* cz = conn + pp;
+    hd::source_rows(pz, G, q, [&](int c) { return cz[(long)c * np]; }, dUg, T00);
+    const double w = Dt * (P.dx[1] * P.dx[2] * P.dx[3]);
+#pragma unroll
+    for (int l = 0; l < 4; l++) v[HARM_UU + l] = w * dUg[l];
+  }
+  block_sum8_part(v, on, part, blockIdx.x);
+}
+
+__global__ __launch_bounds__(kBlock) void k_ener_totals(Dims d, Params P, const double* __restrict__ p,
+                                                        const double* __restrict__ gcent,
+                                                        double* __restrict__ part)
+{
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  double v[HARM_NPR] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const bool on = t < d.n1 * d.n2 * d.n3;
+  if (on) {
+    int i, j, k;
+    zone_ijk(d, t, i, j, k);
+    const long o = hd::cidx(d, i, j, k), nc = d.ncell;
+    Geom G;
+    hd::load_geom(gcent, hd::pidx(d, i, j), d.nplane, G);
+    double pz[HARM_NPR], U[HARM_NPR];
+#pragma unroll
+    for (int pl = 0; pl < HARM_NPR; pl++) pz[pl] = p[pl * nc + o];
+    State q;
+    hd::get_state(pz, G, P.gam, q);
+    hd::primtoflux(pz, q, 0, G, U);
+    const double dV = P.dx[1] * P.dx[2] * P.dx[3];
+#pragma unroll
+    for (int pl = 0; pl < HARM_NPR; pl++) v[pl] = U[pl] * dV;
+  }
+  block_sum8_part(v, on, part, blockIdx.x);
+}
+
+// |div B| at the corner (i-1/2, j-1/2, k-1/2) of zone (i, j, k)
+// (DIVBFLUXCTTOTHPRIM and its normalisation, global.fieldmacros.h:223-244):
+// gdet B averaged onto the corner-centred faces, differenced, divided by the
+// same average of |gdet B|.  A direction with one zone uses the zone itself
+// as its neighbour (im1mac and friends), which zeroes its difference.
+__device__ __forceinline__ double corner_divb(const Dims& d, const Params& P, const double* __restrict__ p,
+                                              const double* __restrict__ gcent, int i, int j, int k)
+{
+  const long nc = d.ncell, gd = (long)HARM_G_GDET * d.nplane;
+  const int im = d.n1 > 1 ? i - 1 : i, jm = d.n2 > 1 ? j - 1 : j, km = d.n3 > 1 ? k - 1 : k;
+  // Fg over the 8 zones (ii, jj, kk) in {im, i} x {jm, j} x {km, k}
+  double Fb[3][2][2][2], Fa[3][2][2][2];
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int b = 0; b < 2; b++)
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+          const int ii = a ? i : im, jj = b ? j : jm, kk = e ? k : km;
+          const double g = gcent[gd + hd::pidx(d, ii, jj)];
+          const double x = p[(HARM_B1 + c) * nc + hd::cidx(d, ii, jj, kk)] * g;
+          Fb[c][a][b][e] = x;
+          Fa[c][a][b][e] = fabs(p[(HARM_B1 + c) * nc + hd::cidx(d, ii, jj, kk)] * g);
+        }
+  // AVG_2 / AVG_1 then the second average, as the macros nest them
+  auto for1 = [&](const double (&F)[3][2][2][2], int a) {  // AVG_for1 at i (a=1) or i-1 (a=0)
+    const double s_k = 0.5 * (F[0][a][1][1] + F[0][a][0][1]);
+    const double s_km = 0.5 * (F[0][a][1][0] + F[0][a][0][0]);
+    return 0.5 * (s_k + s_km);
+  };
+  auto for2 = [&](const double (&F)[3][2][2][2], int b) {  // AVG_for2 at j (b=1) or j-1 (b=0)
+    const double s_k = 0.5 * (F[1][1][b][1] + F[1][0][b][1]);
+    const double s_km = 0.5 * (F[1][1][b][0] + F[1][0][b][0]);
+    return 0.5 * (s_k + s_km);
+  };
+  auto for3 = [&](const double (&F)[3][2][2][2], int e) {  // AVG_for3 at k (e=1) or k-1 (e=0)
+    const double s_i = 0.5 * (F[2][1][1][e] + F[2][1][0][e]);
+    const double s_im = 0.5 * (F[2][0][1][e] + F[2][0][0][e]);
+    return 0.5 * (s_i + s_im);
+  };
+  const double dfx = for1(Fb, 1) - for1(Fb, 0), dfy = for2(Fb, 1) - for2(Fb, 0), dfz = for3(Fb, 1) - for3(Fb, 0);
+  const double nx = for1(Fa, 1) + for1(Fa, 0), ny = for2(Fa, 1) + for2(Fa, 0), nz = for3(Fa, 1) + for3(Fa, 0);
+  const double norm = 1.0 / (fabs(nx / P.dx[1] + ny / P.dx[2] + nz / P.dx[3]) + D_SMALL);
+  return fabs((dfx / P.dx[1] + dfy / P.dx[2] + dfz / P.dx[3]) * norm);
+}
+
+// per block: part[blk][0] = max, part[blk][1] = sum of |div B| over its zones
+__global__ __launch_bounds__(kBlock) void k_ener_divb(Dims d, Params P, const double* __restrict__ p,
+                                                      const double* __restrict__ gcent, double* __restrict__ part)
+{
+  __shared__ double rm[kBlock / 64], rs[kBlock / 64];
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  double m = 0.0, s = 0.0;
+  if (t < d.n1 * d.n2 * d.n3) {
+    int i, j, k;
+    zone_ijk(d, t, i, j, k);
+    m = s = corner_divb(d, P, p, gcent, i, j, k);
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    m = fmax(m, __shfl_down(m, off, 64));
+    s += __shfl_down(s, off, 64);
+  }
+  if (lane == 0) { rm[wid] = m; rs[wid] = s; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kBlock / 64; w++) { m = fmax(m, rm[w]); s += rs[w]; }
+    part[(long)blockIdx.x * 2] = m;
+    part[(long)blockIdx.x * 2 + 1] = s;
+  }
+}
+
+// en[dst + c] += sum_b part[b][c]: 32 groups of 8 lanes each sum every
+// 32nd block in increasing order, then the 32 group sums in order.
+__global__ __launch_bounds__(256) void k_ener_reduce(const double* __restrict__ part, long n, double* __restrict__ en,
+                                                     int dst, int assign)
+{
+  __shared__ double red[32][8];
+  const int c = threadIdx.x & 7, g = threadIdx.x >> 3;
+  double s = 0.0;
+  for (long b = g; b < n; b += 32) s += part[b * 8 + c];
+  red[g][c] = s;
+  __syncthreads();
+  if (threadIdx.x < 8) {
+    double x = red[0][c];
+    for (int q = 1; q < 32; q++) x += red[q][c];
+    if (assign) en[dst + c] = x;
+    else en[dst + c] += x;
+  }
+}
+
+// divb max and sum from the per-block pairs: 64 lanes each take every 64th
+// block in increasing order, then lane sums are combined in lane order
+__global__ void k_ener_divb_final(const double* __restrict__ part, long n, double* __restrict__ out)
+{
+  __shared__ double rm[64], rs[64];
+  const int t = threadIdx.x;
+  double m = 0.0, s = 0.0;
+  for (long b = t; b < n; b += 64) {
+    m = fmax(m, part[b * 2]);
+    s += part[b * 2 + 1];
+  }
+  rm[t] = m;
+  rs[t] = s;
+  __syncthreads();
+  if (t == 0) {
+    for (int q = 1; q < 64; q++) { m = fmax(m, rm[q]); s += rs[q]; }
+    out[8] = m;
+    out[9] = s;
+  }
+}
+
+}  // namespace
+
+long ener_part_blocks(const hd::Dims& d)
+{
+  const long nz = nblk((long)d.n1 * d.n2 * d.n3);
+  return nz > 6 * kEnFaceBlocks ? nz : 6 * kEnFaceBlocks;
+}
+
+void launch_ener_pdot(const hd::Dims& d, const Params& P, const double* F1, const double* F2, const double* F3,
+                      double* part, double* en, const double* ds, double weight, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_ener_pdot, dim3(6 * kEnFaceBlocks), dim3(kBlock), 0, s, d, P, F1, F2, F3, part);
+  hipLaunchKernelGGL(k_ener_pdot_final, dim3(1), dim3(64), 0, s, part, en, ds, weight);
+}
+
+void launch_ener_reduce(const double* part, long nb, double* en, int dst, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_ener_reduce, dim3(1), dim3(256), 0, s, part, nb, en, dst, 0);
+}
+
+void launch_ener_dt4(double* en, const double* ds, int last, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_ener_dt4, dim3(1), dim3(1), 0, s, en, ds, last);
+}
+
+void launch_ener_source(const hd::Dims& d, const Params& P, const double* pb, const double* gcent,
+                        const double* conn, double* part, double* en, hipStream_t s)
+{
+  const unsigned nb = nblk((long)d.n1 * d.n2 * d.n3);
+  hipLaunchKernelGGL(k_ener_source, dim3(nb), dim3(kBlock), 0, s, d, P, pb, gcent, conn, en, part);
+  hipLaunchKernelGGL(k_ener_reduce, dim3(1), dim3(256), 0, s, part, (long)nb, en, (int)EN_SRCADD, 0);
+}
+
+void launch_ener_totals(const hd::Dims& d, const Params& P, const double* p, const double* gcent, double* part,
+                        double* out, hipStream_t s)
+{
+  const unsigned nb = nblk((long)d.n1 * d.n2 * d.n3);
+  hipLaunchKernelGGL(k_ener_totals, dim3(nb), dim3(kBlock), 0, s, d, P, p, gcent, part);
+  hipLaunchKernelGGL(k_ener_reduce, dim3(1), dim3(256), 0, s, part, (long)nb, out, 0, 1);
+  hipLaunchKernelGGL(k_ener_divb, dim3(nb), dim3(kBlock), 0, s, d, P, p, gcent, part);
+  hipLaunchKernelGGL(k_ener_divb_final, dim3(1), dim3(64), 0, s, part, (long)nb, out);
+}
+
+}  // namespace hk

@@ -22,6 +22,24 @@ enum { DS_DT = 0, DS_T = 1, DS_NDT1 = 2, DS_NDT2 = 3, DS_NDT3 = 4, DS_ACCDT = 5,
 // and the zones whose learned-guess solve fell back to the plain guess
 enum { DI_NSTROKE = 0, DI_NFAIL = 1, DI_NSTROKE_SUM = 2, DI_NFAIL_SUM = 3, DI_NFALLBACK = 4, DI_NFALLBACK_SUM = 5,
        DI_N = 8 };
+// Energy-file accumulators (dump_ener.c:391-407), per grid: boundary flux
+// sums of the last substep (pdot) and their time integral (pcum), six
+// boundaries X1UP, X1DN, X2UP, X2DN, X3UP, X3DN by eight variables; the
+// conserved change from the fix-ups by category (fladdterms, twelve
+// categories by eight variables, their sum being fladd); the sampled
+// geometric source (sourceadd); the source sample's weight (dt4diag) and the
+// time gathered towards the next sample.
+enum { EN_PDOT = 0, EN_PCUM = 48, EN_FLTERMS = 96, EN_SRCADD = 192, EN_DT4 = 200, EN_WILLBE = 201, EN_N = 256 };
+// fix-up categories (NUMFAILFLOORFLAGS, global.nondepnmemonics.h:801-814)
+enum { FL_CONV = 0, FL_FLOOR = 1, FL_GAMMA = 2, FL_INFLOW = 3, FL_RHONEG = 4, FL_UNEG = 5, FL_RHOUNEG = 6, FL_N = 12 };
+// the failure categories failure averaging accounts to (fixuputoprim_accounting, fixup.c:1603-1720)
+constexpr int kFailCats[4] = {FL_CONV, FL_RHONEG, FL_UNEG, FL_RHOUNEG};
+__host__ __device__ inline int fail_slot(int pflag)  // index into kFailCats
+{
+  return pflag == HARM_UTOPRIMFAILRHONEG ? 1 : pflag == HARM_UTOPRIMFAILUNEG ? 2 : pflag == HARM_UTOPRIMFAILRHOUNEG ? 3 : 0;
+}
+constexpr int kEnFaceBlocks = 32;  // blocks per boundary of the boundary-flux sums
+constexpr int kDiagSourceStep = 4;  // DIAGSOURCECOMPSTEP (definit.h:19): full steps between source samples
 
 struct Params {
   double gam, cour, safe, gammamax, gammafail;
@@ -49,6 +67,33 @@ __device__ __forceinline__ void block_min_to(double v, double* addr)
     double m = red[0];
     for (int w = 1; w < kBlock / 64; w++) m = fmin(m, red[w]);
     atomic_min_pos(addr, m);
+  }
+}
+
+// Per-block sums of eight values into part[blk][8], in a fixed tree order (the
+// same inputs give the same bits).  `any`: whether this thread holds a nonzero
+// value; a block where no thread does writes zeros without the tree.  Every
+// thread of the block calls it.
+__device__ __forceinline__ void block_sum8_part(const double* v, bool any, double* __restrict__ part, long blk)
+{
+  __shared__ double red[8][kBlock / 64];
+  if (!__syncthreads_or(any)) {
+    if (threadIdx.x < 8) part[blk * 8 + threadIdx.x] = 0.0;
+    return;
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int c = 0; c < 8; c++) {
+    double x = v[c];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+    if (lane == 0) red[c][wid] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < 8) {
+    double s = red[threadIdx.x][0];
+    for (int w = 1; w < kBlock / 64; w++) s += red[threadIdx.x][w];
+    part[blk * 8 + threadIdx.x] = s;
   }
 }
 
@@ -164,9 +209,31 @@ __device__ __forceinline__ void zone_ijk(const Dims& d, int t, int& i, int& j, i
 
 // The step's inversion started from a learned W' (mlp.hip): harm_step's
 // k_invert with the network in front of the Newton solve.
+// With fl_part, the final substep's conserved changes from floors and gamma
+// limits are summed per block (block_sum8_part) into fl_part[0][blk] and
+// fl_part[1][blk] (each nblk x 8).
 harm_status launch_invert_grid_mlp(const harm_mlp* m, const hd::Dims& d, const Params& P, const double* U,
                                    const double* pg, double* pf, int32_t* pflag, const double* gcent,
-                                   const double* floors, int finalstep, unsigned long long* di, hipStream_t s);
+                                   const double* floors, int finalstep, unsigned long long* di, hipStream_t s,
+                                   double* fl_part = nullptr);
+
+// Energy-file diagnostics (ener.hip).  `part` holds per-block partial sums:
+// at least 4 * ener_part_blocks(d) * 8 doubles.
+long ener_part_blocks(const hd::Dims& d);
+// boundary flux sums of F1..F3 into en[EN_PDOT..], and en[EN_PCUM..] += pdot * dt * weight
+void launch_ener_pdot(const hd::Dims& d, const Params& P, const double* F1, const double* F2, const double* F3,
+                      double* part, double* en, const double* ds, double weight, hipStream_t s);
+// en[dst + c] += sum over nblk blocks of part[b][c] (fixed order), c < 8
+void launch_ener_reduce(const double* part, long nblk, double* en, int dst, hipStream_t s);
+// source sample weight at the end of substep `to` of nto (advance.c:251-262)
+void launch_ener_dt4(double* en, const double* ds, int last, hipStream_t s);
+// sourceadd += dt4diag * dV * dUgeom(pb) over active zones when dt4diag > 0
+void launch_ener_source(const hd::Dims& d, const Params& P, const double* pb, const double* gcent,
+                        const double* conn, double* part, double* en, hipStream_t s);
+// totals over active zones: U[8] = sum U(p) dV into out[0..7]; divb max and
+// sum into out[8], out[9] (device array of 10 doubles)
+void launch_ener_totals(const hd::Dims& d, const Params& P, const double* p, const double* gcent, double* part,
+                        double* out, hipStream_t s);
 
 // Launchers of the face-flux kernels (flux.hip): one direction of one grid,
 // or one direction of every grid of a batch.

@@ -281,15 +281,21 @@ __global__ __launch_bounds__(kBlock) void k_update(Dims d, Params P, const doubl
   update_body<DM, MODE>(d, P, pi, pb, F1, F2, F3, ui, uf, ucum, gcent, conn, cuf0, cuf1, cuf2, cun0, cun1, cun2, timeorder, ds, xcd_tile(blockIdx.x, gridDim.x));
 }
 
+// DIAG: also the final substep's floor and gamma-limit changes, summed per
+// block into fl_part (floors) and fl_part + nblk * 8 (gamma limits).
+template <bool DIAG = false>
 __device__ __forceinline__ void invert_body(Dims d, Params P, const double* __restrict__ U,
                                                    const double* __restrict__ pg, double* __restrict__ pf,
                                                    int32_t* __restrict__ pflag,
                                                    const double* __restrict__ gcent, const double* __restrict__ floors,
-                                                   int finalstep, unsigned long long* __restrict__ di, const int blk)
+                                                   int finalstep, unsigned long long* __restrict__ di, const int blk,
+                                                   double* __restrict__ fl_part = nullptr)
 {
   const int t = blk * kBlock + threadIdx.x;
   const int nact = d.n1 * d.n2 * d.n3;
   int ns = 0, nf = 0;
+  double dfl[2 * HARM_NPR] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  bool changed[2] = {false, false};
   if (t < nact) {
     int i, j, k;
     zone_ijk(d, t, i, j, k);
@@ -299,35 +305,48 @@ __device__ __forceinline__ void invert_body(Dims d, Params P, const double* __re
     double Uz[HARM_NPR];
 #pragma unroll
     for (int pl = 0; pl < HARM_NPR; pl++) Uz[pl] = U[pl * nc + o];
-    nf = invert_zone(d, P, Uz, pg, pf, pflag, gcent, floors, finalstep, o, pp, ns) > 0;
+    nf = invert_zone<false, DIAG>(d, P, Uz, pg, pf, pflag, gcent, floors, finalstep, o, pp, ns, 0.0, nullptr, dfl,
+                                  changed) > 0;
   }
   newton_counts_to(ns, nf, di);
+  if (DIAG) {
+    block_sum8_part(dfl, changed[0], fl_part, blk);
+    block_sum8_part(dfl + HARM_NPR, changed[1], fl_part + (long)gridDim.x * 8, blk);
+  }
 }
 
 // Four waves per SIMD (128 VGPRs, a few bytes of spill outside the Newton
 // loop): measured faster than three on the torus step.
+template <bool DIAG>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_invert(Dims d, Params P, const double* __restrict__ U,
                                                    const double* __restrict__ pg, double* __restrict__ pf,
                                                    int32_t* __restrict__ pflag,
                                                    const double* __restrict__ gcent, const double* __restrict__ floors,
-                                                   int finalstep, unsigned long long* __restrict__ di)
+                                                   int finalstep, unsigned long long* __restrict__ di,
+                                                   double* __restrict__ fl_part)
 {
-  invert_body(d, P, U, pg, pf, pflag, gcent, floors, finalstep, di, xcd_tile(blockIdx.x, gridDim.x));
+  invert_body<DIAG>(d, P, U, pg, pf, pflag, gcent, floors, finalstep, di, xcd_tile(blockIdx.x, gridDim.x), fl_part);
 }
 
 // ---------------------------------------------------------------------------
 // failure averaging (fixup_utoprim with the simple_average cascade)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void post_fixup_body(Dims d, Params P, const double* pa, double* pf,
+// DIAG (final substep): the change of each averaged zone, dV (U(after) -
+// U(before)), is added to dfl and `true` returned (fixuputoprim_accounting,
+// fixup.c:1370, 1595; diag_fixup with pr0 the primitives before averaging);
+// *cat is the zone's failure slot (fail_slot of its flag).
+template <bool DIAG = false>
+__device__ __forceinline__ bool post_fixup_body(Dims d, Params P, const double* pa, double* pf,
                                                        const int32_t* __restrict__ pflag,
-                                                       const double* __restrict__ gcent, const int blk)
+                                                       const double* __restrict__ gcent, const int blk,
+                                                       double* dfl = nullptr, int* cat = nullptr)
 {
   const long t = (long)blk * kBlock + threadIdx.x;
   const long nact = (long)d.n1 * d.n2 * d.n3;
-  if (t >= nact) return;
+  if (t >= nact) return false;
   const int i = (int)(t % d.n1), j = (int)((t / d.n1) % d.n2), k = (int)(t / ((long)d.n1 * d.n2));
   const long o = hd::cidx(d, i, j, k);
-  if (pflag[o] <= 0) return;
+  if (pflag[o] <= 0) return false;
   const int e1 = d.n2 > 1;
   auto good = [&](int ii, int jj) {
     if (ii < 0 || ii >= d.n1 || jj < 0 || jj >= d.n2) return true;
@@ -357,15 +376,35 @@ __device__ __forceinline__ void post_fixup_body(Dims d, Params P, const double* 
   Geom G;
   hd::load_geom(gcent, hd::pidx(d, i, j), d.nplane, G);
   hd::limit_gamma(P.gammamax, pz, G);
+  if (DIAG) {
+    double p0[HARM_NPR];
+#pragma unroll
+    for (int pl = 0; pl < HARM_NPR; pl++) p0[pl] = pa[pl * nc + o];
+    fix_delta_to(P, G, p0, pz, dfl);
+    *cat = fail_slot(pflag[o]);
+  }
 #pragma unroll
   for (int pl = 0; pl < HARM_NPR; pl++) pf[pl * nc + o] = pz[pl];
+  return true;
 }
 
+template <bool DIAG>
 __global__ __launch_bounds__(kBlock) void k_post_fixup(Dims d, Params P, const double* pa, double* pf,
                                                        const int32_t* __restrict__ pflag,
-                                                       const double* __restrict__ gcent)
+                                                       const double* __restrict__ gcent,
+                                                       double* __restrict__ fl_part)
 {
-  post_fixup_body(d, P, pa, pf, pflag, gcent, blockIdx.x);
+  double dfl[HARM_NPR] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int cat = 0;
+  const bool ch = post_fixup_body<DIAG>(d, P, pa, pf, pflag, gcent, blockIdx.x, dfl, &cat);
+  if (DIAG)  // one partial per failure category: fl_part + c * nblk * 8
+    for (int c = 0; c < 4; c++) {
+      const bool mine = ch && cat == c;
+      double v[HARM_NPR];
+#pragma unroll
+      for (int pl = 0; pl < HARM_NPR; pl++) v[pl] = mine ? dfl[pl] : 0.0;
+      block_sum8_part(v, mine, fl_part + (long)c * gridDim.x * 8, blockIdx.x);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -804,6 +843,10 @@ struct harm_handle {
   double* rcent = nullptr;
   double* emf = nullptr;
   double* facc = nullptr;  // boundary-flux partial sums
+  double* en = nullptr;      // energy-file accumulators (EN_*)
+  double* enpart = nullptr;  // their per-block partial sums
+  double* entot = nullptr;   // on-demand totals: U[8], divb max, divb sum
+  bool ener_on = false;      // steps add to the energy-file accumulators
   const harm_mlp* guess = nullptr;  // learned inversion guess (harm_set_inversion_guess), or none
   int tile_td[4] = {0, 0, 0, 0};  // sweep depth of the face tiles of dirs 2,3 (0: cost model)
   hipGraph_t sgraph = nullptr;      // one full step, captured on first use when not profiling
@@ -1083,23 +1126,33 @@ harm_status enq_update(harm_handle* h, int which_pi, int which_pb, const double*
 // (advance.c:654).  Inside a step the copy is skipped: the inversion reads its
 // guess from pb and writes every active zone of pf, and the boundary pass that
 // follows rewrites every ghost zone of pf before anything reads it.
-harm_status enq_invert(harm_handle* h, int which_pb, int which_pf, int finalstep, bool copy_all)
+// ener: the final substep's floor and gamma-limit changes go to their fladdterms
+harm_status enq_invert(harm_handle* h, int which_pb, int which_pf, int finalstep, bool copy_all, bool ener = false)
 {
   const Dims& d = h->d;
   const size_t bytes = sizeof(double) * HARM_NPR * d.ncell;
   if (copy_all && which_pf != which_pb)
     HH_HIP(hipMemcpyAsync(h->arr[which_pf], h->arr[which_pb], bytes, hipMemcpyDeviceToDevice, h->stream));
   const double* U = h->arr[finalstep ? HARM_ARR_UCUM : HARM_ARR_UF];
+  const bool diag = ener && finalstep;
+  const unsigned nb = nblocks((long)d.n1 * d.n2 * d.n3);
   prof_begin(h);
   if (h->guess) {
     HH_TRY(launch_invert_grid_mlp(h->guess, d, h->P, U, h->arr[which_pb], h->arr[which_pf], h->pflag,
-                                  h->geom[HARM_LOC_CENT], h->floors, finalstep, h->di, h->stream));
+                                  h->geom[HARM_LOC_CENT], h->floors, finalstep, h->di, h->stream,
+                                  diag ? h->enpart : nullptr));
+  } else if (diag) {
+    hipLaunchKernelGGL(k_invert<true>, dim3(nb), dim3(kBlock), 0, h->stream, d, h->P, U, h->arr[which_pb],
+                       h->arr[which_pf], h->pflag, h->geom[HARM_LOC_CENT], h->floors, finalstep, h->di, h->enpart);
   } else {
-    hipLaunchKernelGGL(k_invert, dim3(nblocks((long)d.n1 * d.n2 * d.n3)), dim3(kBlock), 0, h->stream, d, h->P, U,
-                       h->arr[which_pb], h->arr[which_pf], h->pflag, h->geom[HARM_LOC_CENT], h->floors, finalstep,
-                       h->di);
+    hipLaunchKernelGGL(k_invert<false>, dim3(nb), dim3(kBlock), 0, h->stream, d, h->P, U, h->arr[which_pb],
+                       h->arr[which_pf], h->pflag, h->geom[HARM_LOC_CENT], h->floors, finalstep, h->di, nullptr);
   }
   prof_end(h, HARM_K_INVERT);
+  if (diag) {
+    launch_ener_reduce(h->enpart, nb, h->en, EN_FLTERMS + FL_FLOOR * HARM_NPR, h->stream);
+    launch_ener_reduce(h->enpart + (long)nb * 8, nb, h->en, EN_FLTERMS + FL_GAMMA * HARM_NPR, h->stream);
+  }
   return hh::check_launch("k_invert");
 }
 
@@ -1127,15 +1180,24 @@ harm_status enq_bound(harm_handle* h, int which)
   return hh::check_launch("bounds");
 }
 
-harm_status enq_post_fixup(harm_handle* h, int which)
+// ener: the averaged zones' changes go to their fladdterms (final substep only)
+harm_status enq_post_fixup(harm_handle* h, int which, bool ener = false)
 {
   const Dims& d = h->d;
+  const unsigned nb = nblocks((long)d.n1 * d.n2 * d.n3);
   prof_begin(h);
   // No snapshot is needed: a failed zone averages only good neighbours (or
   // ghost zones, or itself), and only failed zones are written.
-  hipLaunchKernelGGL(k_post_fixup, dim3(nblocks((long)d.n1 * d.n2 * d.n3)), dim3(kBlock), 0, h->stream, d, h->P,
-                     h->arr[which], h->arr[which], h->pflag, h->geom[HARM_LOC_CENT]);
+  if (ener)
+    hipLaunchKernelGGL(k_post_fixup<true>, dim3(nb), dim3(kBlock), 0, h->stream, d, h->P, h->arr[which],
+                       h->arr[which], h->pflag, h->geom[HARM_LOC_CENT], h->enpart);
+  else
+    hipLaunchKernelGGL(k_post_fixup<false>, dim3(nb), dim3(kBlock), 0, h->stream, d, h->P, h->arr[which],
+                       h->arr[which], h->pflag, h->geom[HARM_LOC_CENT], nullptr);
   prof_end(h, HARM_K_FIXUP);
+  if (ener)
+    for (int c = 0; c < 4; c++)
+      launch_ener_reduce(h->enpart + (long)c * nb * 8, nb, h->en, EN_FLTERMS + kFailCats[c] * HARM_NPR, h->stream);
   return hh::check_launch("k_post_fixup");
 }
 
@@ -1169,7 +1231,18 @@ harm_status enq_substep(harm_handle* h, int to, int nto, const double CUf[4][4],
   int pff[3] = {HARM_ARR_PK0, HARM_ARR_PK1, HARM_ARR_P};
   if (nto == 2) { pbb[1] = HARM_ARR_PK0; pff[1] = HARM_ARR_P; }
   const int finalstep = (to == nto - 1);
+  const bool ener = h->ener_on;
+  if (ener) {
+    // the source sample of this substep, if any, from its starting state pb
+    launch_ener_dt4(h->en, h->ds, finalstep, h->stream);
+    if (finalstep)
+      launch_ener_source(h->d, h->P, h->arr[pbb[to]], h->geom[HARM_LOC_CENT], h->conn, h->enpart, h->en,
+                         h->stream);
+  }
   HH_TRY(enq_fluxcalc(h, pbb[to], 1));
+  if (ener)
+    launch_ener_pdot(h->d, h->P, h->arr[HARM_ARR_F1], h->arr[HARM_ARR_F2], h->arr[HARM_ARR_F3], h->enpart, h->en,
+                     h->ds, wflux[to], h->stream);
   if (diag && h->d.n1 > 1) {
     hipLaunchKernelGGL(k_fluxacc_part, dim3(kFaccBlocks), dim3(kBlock), 0, h->stream, h->d, h->arr[HARM_ARR_F1],
                        h->facc);
@@ -1183,10 +1256,10 @@ harm_status enq_substep(harm_handle* h, int to, int nto, const double CUf[4][4],
     zero_ucum = zero_ucum && Cunew[s][0] == 0.0 && Cunew[s][1] == 0.0 && Cunew[s][2] == 0.0;
   const int mode = !zero_ucum ? kUpdAll : to == 0 ? kUpdFirst : finalstep ? kUpdLast : kUpdMiddle;
   HH_TRY(enq_update(h, pii[to], pbb[to], CUf[to], Cunew[to], to, mode));
-  HH_TRY(enq_invert(h, pbb[to], pff[to], finalstep, false));
+  HH_TRY(enq_invert(h, pbb[to], pff[to], finalstep, false, ener));
   hipLaunchKernelGGL(k_scal_substep_dt, dim3(1), dim3(1), 0, h->stream, h->ds);
   HH_TRY(enq_bound(h, pff[to]));
-  HH_TRY(enq_post_fixup(h, pff[to]));
+  HH_TRY(enq_post_fixup(h, pff[to], ener && finalstep));
   HH_TRY(enq_bound(h, pff[to]));
   return hh::check_launch("substep");
 }
@@ -1587,6 +1660,10 @@ extern "C" harm_status harm_create(const harm_config* cfg, harm_handle** out)
   TRYC(alloc((void**)&h->emf, sizeof(double) * 3 * (size_t)h->emf_ni * h->emf_nj * h->emf_nk));
   TRYC(alloc((void**)&h->ds, sizeof(double) * DS_N));
   TRYC(alloc((void**)&h->facc, sizeof(double) * 4 * kFaccBlocks));
+  TRYC(alloc((void**)&h->en, sizeof(double) * EN_N));
+  TRYC(hh::check_hip(hipMemset(h->en, 0, sizeof(double) * EN_N), "hipMemset"));
+  TRYC(alloc((void**)&h->enpart, sizeof(double) * 4 * 8 * ener_part_blocks(d)));
+  TRYC(alloc((void**)&h->entot, sizeof(double) * 10));
   TRYC(hh::check_hip(hipMemset(h->ds, 0, sizeof(double) * DS_N), "hipMemset"));
   TRYC(alloc((void**)&h->di, sizeof(unsigned long long) * DI_N));
   TRYC(hh::check_hip(hipMemset(h->di, 0, sizeof(unsigned long long) * DI_N), "hipMemset"));
@@ -1613,7 +1690,8 @@ extern "C" harm_status harm_destroy(harm_handle* h)
     (void)hipEventDestroy(r.second.second);
   }
   for (hipEvent_t e : h->evpool) (void)hipEventDestroy(e);
-  void* bufs[] = {h->pflag, h->gcorn3, h->conn, h->floors, h->rcent, h->emf, h->facc, h->ds, h->di};
+  void* bufs[] = {h->pflag, h->gcorn3, h->conn, h->floors, h->rcent, h->emf, h->facc, h->en, h->enpart, h->entot,
+                  h->ds, h->di};
   for (void* b : bufs) (void)hipFree(b);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
@@ -1987,6 +2065,54 @@ extern "C" harm_status harm_get_boundary_fluxes(harm_handle* h, double out[4])
   HH_HIP(hipStreamSynchronize(h->stream));
   HH_HIP(hipMemcpy(v, h->ds, sizeof(v), hipMemcpyDeviceToHost));
   for (int c = 0; c < 4; c++) out[c] = v[DS_FLUXACC + c];
+  return HARM_OK;
+}
+
+extern "C" harm_status harm_set_ener_tracking(harm_handle* h, int32_t on)
+{
+  HH_TRY(check_handle(h));
+  HH_HIP(hipStreamSynchronize(h->stream));
+  if (h->ener_on != (on != 0)) drop_step_graph(h);  // the captured step holds the other kernel set
+  h->ener_on = on != 0;
+  return harm_reset_ener(h);
+}
+
+extern "C" harm_status harm_reset_ener(harm_handle* h)
+{
+  HH_TRY(check_handle(h));
+  HH_HIP(hipStreamSynchronize(h->stream));
+  return hh::check_hip(hipMemset(h->en, 0, sizeof(double) * EN_N), "harm_reset_ener");
+}
+
+extern "C" harm_status harm_get_ener(harm_handle* h, int32_t which, harm_ener* out)
+{
+  HH_TRY(check_handle(h));
+  if (!out) return hh::set_error(HARM_EINVAL, "harm_get_ener: null argument");
+  if (which < 0 || which >= HARM_NARR) return hh::set_error(HARM_EINVAL, "harm_get_ener: bad array %d", which);
+  launch_ener_totals(h->d, h->P, h->arr[which], h->geom[HARM_LOC_CENT], h->enpart, h->entot, h->stream);
+  HH_TRY(hh::check_launch("harm_get_ener"));
+  double en[EN_N], tot[10], v[DS_N];
+  HH_HIP(hipStreamSynchronize(h->stream));
+  HH_HIP(hipMemcpy(en, h->en, sizeof(en), hipMemcpyDeviceToHost));
+  HH_HIP(hipMemcpy(tot, h->entot, sizeof(tot), hipMemcpyDeviceToHost));
+  HH_HIP(hipMemcpy(v, h->ds, sizeof(v), hipMemcpyDeviceToHost));
+  out->t = v[DS_T];
+  out->nstep = (int64_t)v[DS_NSTEP];
+  for (int pl = 0; pl < HARM_NPR; pl++) {
+    out->U[pl] = tot[pl];
+    out->fladd[pl] = 0.0;
+    for (int c = 0; c < FL_N; c++) {
+      out->fladdterms[c][pl] = en[EN_FLTERMS + c * HARM_NPR + pl];
+      out->fladd[pl] += out->fladdterms[c][pl];
+    }
+    out->sourceadd[pl] = en[EN_SRCADD + pl];
+    for (int b = 0; b < 6; b++) {
+      out->pdot[b][pl] = en[EN_PDOT + b * HARM_NPR + pl];
+      out->pcum[b][pl] = en[EN_PCUM + b * HARM_NPR + pl];
+    }
+  }
+  out->divbmax = tot[8];
+  out->divbavg = tot[9] / ((double)h->d.n1 * h->d.n2 * h->d.n3);
   return HARM_OK;
 }
 

@@ -8,7 +8,24 @@ import ctypes
 
 import numpy as np
 
-from . import lib, check, HarmConfig, NewtonStats, ARR, G_NFIELD, NCONN, default_config
+from . import lib, check, HarmConfig, NewtonStats, HarmEner, ARR, G_NFIELD, NCONN, default_config
+
+
+# NUMDISSVERSIONS (global.nondepnmemonics.h:417): dissipation columns, -1.0
+# placeholders when dissipation is not tracked (DODISS 0, dump_ener.c:405)
+NUMDISSVERSIONS = 18
+
+
+def ener_line(e):
+    """The energy-file line of record `e` (Grid.ener) in the reference's
+    column order and format (dump_ener.c:391-407, 530): t realnstep U[8]
+    pdot[6][8] pcum[6][8] fladd[8] sourceadd[8] divbmax divbavg diss[18]."""
+    vals = [e["t"]]
+    vals += list(e["U"]) + list(np.ravel(e["pdot"])) + list(np.ravel(e["pcum"]))
+    vals += list(e["fladd"]) + list(e["sourceadd"]) + [e["divbmax"], e["divbavg"]]
+    vals += [-1.0] * NUMDISSVERSIONS
+    out = "%21.15g %ld " % (vals[0], e["nstep"])
+    return out + "".join("%21.15g " % v for v in vals[1:]) + "\n"
 
 
 class Grid:
@@ -178,6 +195,25 @@ class Grid:
         s = ctypes.c_void_p()
         check(lib().harm_get_stream(self.h, ctypes.byref(s)), "harm_get_stream")
         return s.value
+
+    # ---- energy-file diagnostics (dump_ener.c) ----
+    def set_ener_tracking(self, on=True):
+        """Steps of this grid add to the energy-file accumulators (cleared now)."""
+        check(lib().harm_set_ener_tracking(self.h, int(bool(on))), "harm_set_ener_tracking")
+
+    def reset_ener(self):
+        check(lib().harm_reset_ener(self.h), "harm_reset_ener")
+
+    def ener(self, which="p"):
+        """One energy-file record as a dict of numpy arrays: t, nstep, U (8),
+        pdot and pcum (6 boundaries x 8), fladd, fladdterms (12 categories x
+        8), sourceadd, divbmax, divbavg."""
+        e = HarmEner()
+        check(lib().harm_get_ener(self.h, self._which(which), ctypes.byref(e)), "harm_get_ener")
+        return {"t": e.t, "nstep": e.nstep, "U": np.array(e.U[:]), "pdot": np.array([r[:] for r in e.pdot]),
+                "pcum": np.array([r[:] for r in e.pcum]), "fladd": np.array(e.fladd[:]),
+                "fladdterms": np.array([r[:] for r in e.fladdterms]),
+                "sourceadd": np.array(e.sourceadd[:]), "divbmax": e.divbmax, "divbavg": e.divbavg}
 
     def boundary_fluxes(self):
         out = (ctypes.c_double * 4)()

@@ -239,6 +239,45 @@ harm_status harm_step(harm_handle* h, int32_t nsteps);
  * mdot_out, edot_out}. */
 harm_status harm_get_boundary_fluxes(harm_handle* h, double out[4]);
 
+/* Fix-up categories of harm_ener.fladdterms (NUMFAILFLOORFLAGS and the
+ * COUNT* values, global.nondepnmemonics.h:801-814): failure averaging after
+ * a convergence failure, floors, the gamma limit, the inflow check (ghost
+ * zones only, so always 0 here), failure averaging after rho < 0, u < 0 and
+ * both; the other slots stay 0. */
+enum { HARM_FL_CONV = 0, HARM_FL_FLOOR = 1, HARM_FL_GAMMA = 2, HARM_FL_INFLOW = 3, HARM_FL_RHONEG = 4,
+       HARM_FL_UNEG = 5, HARM_FL_RHOUNEG = 6, HARM_NUMFAILFLOORFLAGS = 12 };
+
+/* One record of the energy file (dump_ener.c:391-407, the "ener.out" line).
+ * Boundaries in the reference's order X1UP, X1DN, X2UP, X2DN, X3UP, X3DN
+ * (global.nondepnmemonics.h:699-704); variables in
+ * primitive order; U's in the diagnostic form (with gdet, T^t_t + rho u^t in
+ * UU) times dV = dx1 dx2 dx3. */
+typedef struct harm_ener {
+  double t;
+  int64_t nstep;
+  double U[HARM_NPR];           /* sum of U(p) dV over active zones (constotal, dump_ener.c:882) */
+  double pdot[6][HARM_NPR];     /* boundary face sums of F * surface, last substep (diag.c:1020) */
+  double pcum[6][HARM_NPR];     /* pdot * the substep's dt share, summed (diag.c:1209) */
+  double fladd[HARM_NPR];       /* final-substep changes from floors, gamma limits and
+                                   failure averaging (diag_fixup, fixup.c:220): the sum of
+                                   fladdterms over its categories */
+  double fladdterms[HARM_NUMFAILFLOORFLAGS][HARM_NPR]; /* by category (HARM_FL_*) */
+  double sourceadd[HARM_NPR];   /* sampled geometric source dt dV dU (diag_source_all,
+                                   diag.c:1958; advance.c:251-262 sampling) */
+  double divbmax, divbavg;      /* |div B| at the corners (divbmaxavg, dump_ener.c:680) */
+} harm_ener;
+
+/* Replaces the accumulation side of dump_ener()/diag(): with on != 0 every
+ * later harm_step / harm_advance of this handle adds to pdot, pcum, fladd
+ * and sourceadd (batch steps do not).  Switching clears the accumulators. */
+harm_status harm_set_ener_tracking(harm_handle* h, int32_t on);
+/* Clears pdot, pcum, fladd, sourceadd and the source-sample clock. */
+harm_status harm_reset_ener(harm_handle* h);
+/* Replaces the record-forming side of dump_ener(): totals and div B from
+ * array `which` (HARM_ARR_*, normally HARM_ARR_P), with the accumulators and
+ * the step clock. */
+harm_status harm_get_ener(harm_handle* h, int32_t which, harm_ener* out);
+
 /* ------------------------------------------------------------------------ */
 /* Device memory, streams and timing (so callers need no other HIP runtime)  */
 /* ------------------------------------------------------------------------ */

@@ -74,9 +74,10 @@ static double gamma_of(const double* pr, const o_geom* G)
 }
 
 /* limit_gamma (fixup.c:2723), DO_CONSERVE_D=0 */
-static void limit_gamma(double gammamax, double* pr, const o_geom* G)
+/* returns 1 when the velocity was rescaled (limit_gamma's -1, fixup.c:2891) */
+static int limit_gamma(double gammamax, double* pr, const o_geom* G)
 {
-  if (gammamax <= 1.0) return;
+  if (gammamax <= 1.0) return 0;
   const double gamma = gamma_of(pr, G);
   if ((gamma > gammamax && (gamma != 1.0))) {
     const double pref = (gammamax * gammamax - 1.) / (gamma * gamma - 1.);
@@ -84,6 +85,27 @@ static void limit_gamma(double gammamax, double* pr, const o_geom* G)
     pr[O_U1] *= f;
     pr[O_U2] *= f;
     pr[O_U3] *= f;
+    return 1;
+  }
+  return 0;
+}
+
+/* diag_fixup (fixup.c:220-405) on the final substep, the original HARM
+ * branch: fl[pl] += dVF * (Uf[pl] - Ui[pl]) with Ui = primtoU(UDIAG, pr0),
+ * Uf = primtoU(UDIAG, pr); UDIAG equals the evolved form here (WITHGDET,
+ * UtoU_evolve2diag phys.tools.c:740), dVF = dx1 dx2 dx3 (coord.c:2306). */
+static void diag_fixup_acc(const o_grid* g, const o_geom* G, const double* pr0, const double* pr, double* fl)
+{
+  o_state q;
+  double Ui[NPR], Uf[NPR];
+  o_get_state(pr0, G, g->gam, &q);
+  o_primtoflux(pr0, &q, 0, G, Ui);
+  o_get_state(pr, G, g->gam, &q);
+  o_primtoflux(pr, &q, 0, G, Uf);
+  const double dVF = g->dx[1] * g->dx[2] * g->dx[3];
+  for (int pl = 0; pl < NPR; pl++) {
+    const double deltaUavg = Uf[pl] - Ui[pl];
+    fl[pl] += dVF * deltaUavg;
   }
 }
 
@@ -812,11 +834,16 @@ void o_update(const o_grid* g, const double* pi, const double* pb, const double*
 /* Inversion, floors, failure averaging                                    */
 /* ---------------------------------------------------------------------- */
 
-/* fixup1zone (fixup.c:561), FIXUPTYPE=1, rescaletype 0 floors */
-static void fixup1zone(const o_grid* g, const o_geom* G, double r, double* pr, int32_t* pflag, int* nstroke)
+/* fixup1zone (fixup.c:561), FIXUPTYPE=1, rescaletype 0 floors; with fl (the
+ * fladdterms block), the floor step (fixup.c:766, COUNTFLOORACT) and the
+ * gamma limit (fixup.c:784, COUNTLIMITGAMMAACT) are each accounted from pr0,
+ * the primitives on entry (fixup.c:593). */
+static void fixup1zone(const o_grid* g, const o_geom* G, double r, double* pr, int32_t* pflag, int* nstroke,
+                       double* fl)
 {
   o_params P = {g->gam, g->gammafail, g->gammamax};
-  double prfloor[2], prnew[NPR];
+  double prfloor[2], prnew[NPR], pr0[NPR];
+  for (int pl = 0; pl < NPR; pl++) pr0[pl] = pr[pl];
   prfloor[0] = g->rhomin * pow(r, -1.5);
   prfloor[1] = g->uumin * pow(r, -2.5);
   for (int pl = 0; pl < 2; pl++)
@@ -838,23 +865,27 @@ static void fixup1zone(const o_grid* g, const o_geom* G, double r, double* pr, i
     o_get_state(dpr, G, g->gam, &dq);
     o_primtoflux(dpr, &dq, 0, &G1, dU);
     for (int pl = 0; pl < NPR; pl++) U[pl] += dU[pl];
-    const int fl = o_utoprimgen_unothing(U, G, &P, pr, nstroke);
-    *pflag = fl;
-    if (fl > 0)
+    const int flag = o_utoprimgen_unothing(U, G, &P, pr, nstroke);
+    *pflag = flag;
+    if (flag > 0)
       for (int pl = 0; pl < NPR; pl++) pr[pl] = prnew[pl];
+    if (fl) diag_fixup_acc(g, G, pr0, pr, fl + O_FL_FLOOR * NPR);
   }
-  limit_gamma(g->gammamax, pr, G);
+  if (limit_gamma(g->gammamax, pr, G) && fl) diag_fixup_acc(g, G, pr0, pr, fl + O_FL_GAMMA * NPR);
 }
 
-void o_invert(const o_grid* g, const double* pb, const double* U, double* pf, int32_t* pflag, int finalstep,
-              int64_t* nstroke)
+/* fl (final substep): the fladdterms block, or NULL */
+static void invert(const o_grid* g, const double* pb, const double* U, double* pf, int32_t* pflag, int finalstep,
+                   int64_t* nstroke, double* fl)
 {
   dims_t d = D(g);
   const long ntot = (long)d.ns1 * d.ns2 * d.ns3 * NPR;
   memcpy(pf, pb, sizeof(double) * ntot);
   int64_t ns = 0;
   o_params P = {g->gam, g->gammafail, g->gammamax};
-#pragma omp parallel for collapse(3) reduction(+ : ns) schedule(dynamic, 64) num_threads(g->nthreads > 0 ? g->nthreads : 1)
+  double fls[3 * NPR] = {0};  /* categories 0..2: only FLOOR and GAMMA are reached here */
+  const int acc = fl != NULL;
+#pragma omp parallel for collapse(3) reduction(+ : ns, fls[:3 * NPR]) schedule(dynamic, 64) num_threads(g->nthreads > 0 ? g->nthreads : 1)
   for (int k = 0; k < d.n3; k++)
     for (int j = 0; j < d.n2; j++)
       for (int i = 0; i < d.n1; i++) {
@@ -864,16 +895,27 @@ void o_invert(const o_grid* g, const double* pb, const double* U, double* pf, in
         geom_at(g, &d, 0, i, j, &G);
         int nst = 0;
         pflag[f] = o_utoprimgen(U + o, &G, &P, pf + o, &nst);
-        if (finalstep) fixup1zone(g, &G, g->rcent[i + d.ng1], pf + o, &pflag[f], &nst);
+        if (finalstep) fixup1zone(g, &G, g->rcent[i + d.ng1], pf + o, &pflag[f], &nst, acc ? fls : NULL);
         ns += nst;
       }
   if (nstroke) *nstroke = ns;
+  if (acc)
+    for (int c = 0; c < 3 * NPR; c++) fl[c] += fls[c];
+}
+
+void o_invert(const o_grid* g, const double* pb, const double* U, double* pf, int32_t* pflag, int finalstep,
+              int64_t* nstroke)
+{
+  invert(g, pb, U, pf, pflag, finalstep, nstroke, NULL);
 }
 
 /* fixup_utoprim (fixup.c:1061) with the simple_average cascade (:2087); every
  * failure averages rho..U3 (HANDLE*NEG=0); averages read a copy of pf and the
  * original flags; ghost flags count as good. */
-void o_post_fixup(const o_grid* g, double* pf, const int32_t* pflag)
+/* fl (final substep, the fladdterms block): each averaged zone's change from
+ * its value before the averaging, in the category of its flag
+ * (fixuputoprim_accounting, fixup.c:1370, 1595-1720), or NULL */
+static void post_fixup(const o_grid* g, double* pf, const int32_t* pflag, double* fl)
 {
   dims_t d = D(g);
   const long ntot = (long)d.ns1 * d.ns2 * d.ns3 * NPR;
@@ -911,11 +953,18 @@ void o_post_fixup(const o_grid* g, double* pf, const int32_t* pflag)
         o_geom G;
         geom_at(g, &d, 0, i, j, &G);
         limit_gamma(g->gammamax, p, &G);
+        if (fl) {
+          const int f = FL(i, j, k);
+          const int cat = f == 9 ? O_FL_RHONEG : f == 10 ? O_FL_UNEG : f == 11 ? O_FL_RHOUNEG : O_FL_CONV;
+          diag_fixup_acc(g, &G, pa + ix(&d, i, j, k), p, fl + cat * NPR);
+        }
       }
 #undef GOOD
 #undef FL
   free(pa);
 }
+
+void o_post_fixup(const o_grid* g, double* pf, const int32_t* pflag) { post_fixup(g, pf, pflag, NULL); }
 
 /* ---------------------------------------------------------------------- */
 /* Step driver                                                             */
@@ -952,6 +1001,137 @@ static void rk_flux_weights(int timeorder, double* w)
   for (int r = 0; r < n; r++) w[r] = C[n - 1][r];
 }
 
+/* ---------------------------------------------------------------------- */
+/* Energy-file diagnostics                                                 */
+/* ---------------------------------------------------------------------- */
+
+/* diag_flux_pureflux (diag.c:1020-1210): per boundary X1UP, X1DN, X2UP,
+ * X2DN, X3UP, X3DN (global.nondepnmemonics.h:699-704) the sum over its faces of F * surface (non-finite terms
+ * dropped, diag.c:1196-1201), and pcum += pdot * Dt. */
+static void ener_pdot(const o_grid* g, const dims_t* d, const double* F1, const double* F2, const double* F3,
+                      double Dt, double* en)
+{
+  const int nn[4] = {0, d->n1, d->n2, d->n3};
+  for (int bd = 0; bd < 6; bd++) {
+    const int dir = bd / 2 + 1, up = !(bd & 1);
+    double* pdot = en + O_EN_PDOT + bd * NPR;
+    for (int pl = 0; pl < NPR; pl++) pdot[pl] = 0.0;
+    if (nn[dir] > 1) {
+      const double* F = dir == 1 ? F1 : dir == 2 ? F2 : F3;
+      const double surface = dir == 1 ? g->dx[2] * g->dx[3] : dir == 2 ? g->dx[1] * g->dx[3] : g->dx[1] * g->dx[2];
+      const int is = dir == 1 ? (up ? d->n1 : 0) : 0, ie = dir == 1 ? is : d->n1 - 1;
+      const int js = dir == 2 ? (up ? d->n2 : 0) : 0, je = dir == 2 ? js : d->n2 - 1;
+      const int ks = dir == 3 ? (up ? d->n3 : 0) : 0, ke = dir == 3 ? ks : d->n3 - 1;
+      for (int k = ks; k <= ke; k++)
+        for (int j = js; j <= je; j++)
+          for (int i = is; i <= ie; i++)
+            for (int pl = 0; pl < NPR; pl++) {
+              double Ftemp = F[ix(d, i, j, k) + pl] * surface;
+              if (!isfinite(Ftemp)) Ftemp = 0.0;
+              pdot[pl] += Ftemp;
+            }
+    }
+    for (int pl = 0; pl < NPR; pl++) en[O_EN_PCUM + bd * NPR + pl] += pdot[pl] * Dt;
+  }
+}
+
+/* dt4diag (advance.c:251-262, DIAGSOURCECOMPSTEP = 4) for this substep */
+static double ener_dt4(double* en, double dt, int last)
+{
+  const long nstep = (long)en[O_EN_NSTEP];
+  if (last && nstep % 4 == 0) {
+    en[O_EN_DT4] = en[O_EN_WILLBE];
+    en[O_EN_WILLBE] = 0.0;
+  } else {
+    en[O_EN_WILLBE] += dt;
+    en[O_EN_DT4] = -1.0;
+  }
+  return en[O_EN_DT4];
+}
+
+/* diag_source_all (diag.c:1958) over the active zones: sourceadd +=
+ * Dt * dVF * dUgeom(pb) when Dt > 0 */
+static void ener_source(const o_grid* g, const dims_t* d, const double* pb, double Dt, double* en)
+{
+  if (!(Dt > 0.0)) return;
+  const double dVF = g->dx[1] * g->dx[2] * g->dx[3];
+  for (int k = 0; k < d->n3; k++)
+    for (int j = 0; j < d->n2; j++)
+      for (int i = 0; i < d->n1; i++) {
+        o_geom G;
+        geom_at(g, d, 0, i, j, &G);
+        o_state q;
+        double dUg[NPR];
+        o_get_state(pb + ix(d, i, j, k), &G, g->gam, &q);
+        o_source(pb + ix(d, i, j, k), &G, &q, g->conn + ig2(d, i, j) * 40, dUg);
+        for (int pl = 0; pl < NPR; pl++) en[O_EN_SRCADD + pl] += Dt * dVF * dUg[pl];
+      }
+}
+
+/* FLUXCTTOTH corner divergence (SETFDIVBFLUXCTTOTHPRIM,
+ * global.fieldmacros.h:130-244, 383): Fg = B * gdet(CENT), averaged over
+ * the transverse pairs, differenced along each direction and normalised by
+ * the same averages of |Fg|.  im1mac and friends keep the index in a
+ * direction with one zone. */
+static double divb_at(const o_grid* g, const dims_t* d, const double* p, int i, int j, int k)
+{
+#define GD(ii, jj) (g->geom[0][ig2(d, ii, jj) * 27 + 21])
+#define FG(ii, jj, kk, pl) (p[ix(d, ii, jj, kk) + (pl)] * GD(ii, jj))
+#define AFG(ii, jj, kk, pl) fabs(p[ix(d, ii, jj, kk) + (pl)] * GD(ii, jj))
+#define IM(x) (d->n1 > 1 ? (x) - 1 : (x))
+#define JM(x) (d->n2 > 1 ? (x) - 1 : (x))
+#define KM(x) (d->n3 > 1 ? (x) - 1 : (x))
+#define AV1(F, ii, jj, kk, pl) (0.5 * (F(ii, jj, kk, pl) + F(IM(ii), jj, kk, pl)))
+#define AV2(F, ii, jj, kk, pl) (0.5 * (F(ii, jj, kk, pl) + F(ii, JM(jj), kk, pl)))
+#define FOR1(F, ii, jj, kk) (0.5 * (AV2(F, ii, jj, kk, O_B1) + AV2(F, ii, jj, KM(kk), O_B1)))
+#define FOR2(F, ii, jj, kk) (0.5 * (AV1(F, ii, jj, kk, O_B2) + AV1(F, ii, jj, KM(kk), O_B2)))
+#define FOR3(F, ii, jj, kk) (0.5 * (AV2(F, ii, jj, kk, O_B3) + AV2(F, IM(ii), jj, kk, O_B3)))
+  const double diffx = FOR1(FG, i, j, k) - FOR1(FG, IM(i), j, k);
+  const double diffy = FOR2(FG, i, j, k) - FOR2(FG, i, JM(j), k);
+  const double diffz = FOR3(FG, i, j, k) - FOR3(FG, i, j, KM(k));
+  const double normx = FOR1(AFG, i, j, k) + FOR1(AFG, IM(i), j, k);
+  const double normy = FOR2(AFG, i, j, k) + FOR2(AFG, i, JM(j), k);
+  const double normz = FOR3(AFG, i, j, k) + FOR3(AFG, i, j, KM(k));
+  const double norm = 1.0 / (fabs(normx / g->dx[1] + normy / g->dx[2] + normz / g->dx[3]) + SMALL);
+  return fabs((diffx / g->dx[1] + diffy / g->dx[2] + diffz / g->dx[3]) * norm);
+#undef GD
+#undef FG
+#undef AFG
+#undef IM
+#undef JM
+#undef KM
+#undef AV1
+#undef AV2
+#undef FOR1
+#undef FOR2
+#undef FOR3
+}
+
+void o_ener_totals(const o_grid* g, const double* p, double* out)
+{
+  dims_t d = D(g);
+  const double dVF = g->dx[1] * g->dx[2] * g->dx[3];
+  double divbmax = 0.0, divbavg = 0.0;
+  for (int pl = 0; pl < NPR; pl++) out[pl] = 0.0;
+  /* constotal (dump_ener.c:882) and divbmaxavg (dump_ener.c:680) */
+  for (int k = 0; k < d.n3; k++)
+    for (int j = 0; j < d.n2; j++)
+      for (int i = 0; i < d.n1; i++) {
+        o_geom G;
+        geom_at(g, &d, 0, i, j, &G);
+        o_state q;
+        double U[NPR];
+        o_get_state(p + ix(&d, i, j, k), &G, g->gam, &q);
+        o_primtoflux(p + ix(&d, i, j, k), &q, 0, &G, U);
+        for (int pl = 0; pl < NPR; pl++) out[pl] += U[pl] * dVF;
+        const double divb = divb_at(g, &d, p, i, j, k);
+        if (divb > divbmax) divbmax = divb;
+        divbavg += divb;
+      }
+  out[8] = divbmax;
+  out[9] = divbavg;
+}
+
 void o_step(const o_grid* g, double* p, double* pk0, double* pk1, double* ui, double* uf, double* ucum, double* F1,
             double* F2, double* F3, int32_t* pflag, double* dt, double* fluxacc)
 {
@@ -964,10 +1144,13 @@ void o_step(const o_grid* g, double* p, double* pk0, double* pk1, double* ui, do
   double* pff[3] = {pk0, pk1, p};
   if (nto == 2) { pbb[1] = pk0; pff[1] = p; }
   double lastndt = BIG;
+  double* en = g->ener;
   for (int to = 0; to < nto; to++) {
     const int finalstep = (to == nto - 1);
     double ndt[4], accdt;
+    if (en) ener_source(g, &d, pbb[to], ener_dt4(en, *dt, finalstep), en);
     o_fluxcalc(g, pbb[to], F1, F2, F3, ndt, 1);
+    if (en) ener_pdot(g, &d, F1, F2, F3, (*dt) * wflux[to], en);
     if (fluxacc && d.n1 > 1) {
       /* diag_flux_pureflux: F1[RHO], F1[UU] through the radial boundaries */
       double s[4] = {0, 0, 0, 0};
@@ -982,14 +1165,16 @@ void o_step(const o_grid* g, double* p, double* pk0, double* pk1, double* ui, do
     }
     o_update(g, pii[to], pbb[to], F1, F2, F3, ui, uf, ucum, CUf[to], Cunew[to], *dt, to, &accdt);
     int64_t ns;
-    o_invert(g, pbb[to], finalstep ? ucum : uf, pff[to], pflag, finalstep, &ns);
+    double* fl = (en && finalstep) ? en + O_EN_FLTERMS : NULL;
+    invert(g, pbb[to], finalstep ? ucum : uf, pff[to], pflag, finalstep, &ns, fl);
     const double wavedt = 1. / (1. / ndt[1] + 1. / ndt[2] + 1. / ndt[3]);
     const double nd = 1.0 * fmin(wavedt, accdt);
     if (nd < lastndt) lastndt = nd;
     o_bound(g, pff[to]);
-    o_post_fixup(g, pff[to], pflag);
+    post_fixup(g, pff[to], pflag, fl);
     o_bound(g, pff[to]);
   }
+  if (en) en[O_EN_NSTEP] += 1.0;
   double fullndt = lastndt;
   if (fullndt > g->safe * (*dt)) fullndt = g->safe * (*dt);
   *dt = fullndt;

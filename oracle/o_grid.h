@@ -28,7 +28,19 @@ typedef struct o_grid {
   const double* rcent;      /* r at zone centre per i (ns1) */
   int problem;              /* 0 torus, 1 Bondi: the boundary-condition set */
   double R0;                /* r = R0 + exp(X1) */
+  double* ener;             /* energy-file accumulators (O_EN_*), or NULL: o_step adds to them */
 } o_grid;
+
+/* Energy-file accumulators (dump_ener.c:391-407): boundary flux sums of the
+ * last substep and their time integral, six boundaries (X1UP, X1DN, X2UP,
+ * X2DN, X3UP, X3DN) by NPR; fix-up changes by category (fladdterms,
+ * NUMFAILFLOORFLAGS = 12 by NPR, global.nondepnmemonics.h:801-814); the
+ * sampled source; the source sample's weight, the time gathered towards the
+ * next sample, and the step count of the next step (nstep, advance.c:254). */
+enum { O_EN_PDOT = 0, O_EN_PCUM = 48, O_EN_FLTERMS = 96, O_EN_SRCADD = 192, O_EN_DT4 = 200, O_EN_WILLBE = 201,
+       O_EN_NSTEP = 202, O_EN_N = 256 };
+/* COUNT* categories (global.nondepnmemonics.h:803-809) */
+enum { O_FL_CONV = 0, O_FL_FLOOR = 1, O_FL_GAMMA = 2, O_FL_RHONEG = 4, O_FL_UNEG = 5, O_FL_RHOUNEG = 6 };
 
 /* storage helpers */
 void o_dims(const o_grid* g, int* ng1, int* ng2, int* ng3, int* ns1, int* ns2, int* ns3);
@@ -51,6 +63,9 @@ void o_post_fixup(const o_grid* g, double* pf, const int32_t* pflag);
  * *dt in/out; fluxacc[4] accumulates boundary fluxes (may be NULL). */
 void o_step(const o_grid* g, double* p, double* pk0, double* pk1, double* ui, double* uf, double* ucum, double* F1,
             double* F2, double* F3, int32_t* pflag, double* dt, double* fluxacc);
+/* totals over active zones (constotal dump_ener.c:882, divbmaxavg :680):
+ * out[0..7] = sum U(p) dV, out[8] = max |div B|, out[9] = sum |div B| */
+void o_ener_totals(const o_grid* g, const double* p, double* out);
 /* one cell's (left, right) edge values from y5[0..4] = y[-2..2] (recon 0 donor,
  * 2 PARA, 3 WENO5), and one face's Riemann flux (flux 0 LAX-F, 1 HLL) from
  * two states at a 27-field geometry record; p_l, p_r are modified as

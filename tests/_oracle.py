@@ -130,7 +130,12 @@ class OGridStruct(ctypes.Structure):
         ("rhomin", ctypes.c_double), ("uumin", ctypes.c_double),
         ("geom", ctypes.c_void_p * 4), ("gdet_corn3", ctypes.c_void_p), ("conn", ctypes.c_void_p),
         ("rcent", ctypes.c_void_p), ("problem", ctypes.c_int), ("R0", ctypes.c_double),
+        ("ener", ctypes.c_void_p),
     ]
+
+
+# energy-file accumulator slots of o_grid.h
+EN_PDOT, EN_PCUM, EN_FLTERMS, EN_SRCADD, EN_DT4, EN_WILLBE, EN_NSTEP, EN_N = 0, 48, 96, 192, 200, 201, 202, 256
 
 
 def _grid_protos(L):
@@ -141,6 +146,8 @@ def _grid_protos(L):
     L.o_invert.argtypes = [P, _P, _P, _P, _P, _I, _P]
     L.o_post_fixup.argtypes = [P, _P, _P]
     L.o_step.argtypes = [P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]
+    L.o_ener_totals.argtypes = [P, _P, _P]
+    L.o_ener_totals.restype = None
     L.o_rk_tables.argtypes = [_I, _P, _P]
     L.o_rk_tables.restype = _I
     for n in ("o_bound", "o_fluxcalc", "o_update", "o_invert", "o_post_fixup", "o_step"):
@@ -185,6 +192,8 @@ class OGrid:
         g.conn = self.conn.ctypes.data
         g.rcent = self.rc.ctypes.data
         g.problem, g.R0 = cfg.problem, cfg.R0
+        g.ener = None
+        self.en = None
         self.g = g
         self.dx = [dx[a] for a in range(4)]
 
@@ -253,6 +262,24 @@ class OGrid:
                                              ("p", "pk0", "pk1", "ui", "uf", "ucum", "F1", "F2", "F3", "pflag")],
                      ctypes.byref(d), fa.ctypes.data if fa is not None else None)
         return d.value
+
+    def track_ener(self, nstep=0):
+        """Make step() add to the energy-file accumulators from now on (zeroed;
+        `nstep`: the step count of the next step, as the device clock)."""
+        self.en = np.zeros(EN_N)
+        self.en[EN_NSTEP] = nstep
+        self.g.ener = self.en.ctypes.data
+
+    def ener(self, p):
+        """dict like the device record: U, pdot, pcum, fladd, sourceadd, divbmax, divbavg"""
+        tot = np.zeros(10)
+        lib().o_ener_totals(ctypes.byref(self.g), p.ctypes.data, tot.ctypes.data)
+        en = self.en if self.en is not None else np.zeros(EN_N)
+        nz = self.cfg.n1 * self.cfg.n2 * self.cfg.n3
+        terms = en[EN_FLTERMS:EN_FLTERMS + 96].reshape(12, 8).copy()
+        return {"U": tot[:8].copy(), "pdot": en[EN_PDOT:EN_PDOT + 48].reshape(6, 8).copy(),
+                "pcum": en[EN_PCUM:EN_PCUM + 48].reshape(6, 8).copy(), "fladd": terms.sum(0), "fladdterms": terms,
+                "sourceadd": en[EN_SRCADD:EN_SRCADD + 8].copy(), "divbmax": tot[8], "divbavg": tot[9] / nz}
 
     def new_state(self, p):
         st = {k: self.zeros() for k in ("pk0", "pk1", "ui", "uf", "ucum", "F1", "F2", "F3")}
