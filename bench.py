@@ -539,6 +539,26 @@ def main():
     prof = g.profile_read()
     g.profile(False)
     prof_steps = args.prof_steps
+    # the same steps with the energy-file accumulators on (boundary sums every
+    # substep, fix-up accounting and the source sample), and the conservation
+    # identity of the density over them
+    g.set_ener_tracking(True)
+    g.step(1)  # capture the tracking step
+    g.sync()
+    e0 = g.ener()
+    t0 = time.perf_counter()
+    g.step(args.steps)
+    g.sync()
+    el_en = time.perf_counter() - t0
+    e1 = g.ener()
+    g.set_ener_tracking(False)
+    pc = e1["pcum"]
+    inflow = pc[1] - pc[0] + pc[3] - pc[2] + pc[5] - pc[4]
+    resid = e1["U"] - e0["U"] - (inflow - e0["pcum"][1] + e0["pcum"][0] - e0["pcum"][3] + e0["pcum"][2]
+                                 - e0["pcum"][5] + e0["pcum"][4]) - (e1["fladd"] - e0["fladd"])
+    ener = {"ms_per_step": el_en / args.steps * 1e3, "overhead_vs_plain": el_en / el - 1.0,
+            "density_identity_residual": abs(resid[0]) / abs(e1["U"][0]),
+            "U_rho": e1["U"][0], "fladd_rho": e1["fladd"][0], "divbmax": e1["divbmax"], "divbavg": e1["divbavg"]}
 
     zones = n1 * n2 * n3
     value = zones * args.steps / el
@@ -598,6 +618,7 @@ def main():
         "roofline_fp64": fp64,
         "step_roofline_hbm": step_roof,
         "kernels": kernels,
+        "ener_tracking": ener,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

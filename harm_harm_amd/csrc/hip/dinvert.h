@@ -13,23 +13,8 @@ using hd::Dims;
 using hd::Geom;
 using hd::State;
 
-// ---------------------------------------------------------------------------
-// inversion + fixup1zone (floors via ZAMO-frame mass/energy addition)
-// ---------------------------------------------------------------------------
-// One zone: conserved Uz (with gdet) in registers, the guess read from pg at
-// offset o, the result written to pf and pflag.  Returns the flag; the Newton
-// stroke count is added to ns.
-//
-// SEEDED: the Newton solve starts from W' = wp_seed (the learned guess); when
-// that solve fails, the zone is solved again from the primitives' guess as
-// the plain inversion does (Newton from the previous primitives is the
-// fallback and decides), and *fellback is set.
-// DIAG: on the final substep, the conserved change of each fix is added as
-// diag_fixup does (fixup.c:220, finalstep > 0): dV (U(after) - U(before)),
-// with U = primtoU with gdet and `before` the primitives ahead of the floors,
-// for the floor step (fixup.c:766) to dfl[0..7] and for the gamma limit
-// (fixup.c:784) to dfl[8..15]; changed[0], changed[1] say whether each was.
-// dfl[pl] += dV (U(pa)[pl] - U(pb)[pl]), U with gdet (diag_fixup, fixup.c:303-405)
+// dfl[pl] += dV (U(pa)[pl] - U(pb)[pl]), U with gdet: the conserved change of
+// one fix as the energy file accounts it (diag_fixup, fixup.c:303-405)
 __device__ __forceinline__ void fix_delta_to(const Params& P, const Geom& G, const double* pb, const double* pa,
                                              double* dfl)
 {
@@ -44,13 +29,31 @@ __device__ __forceinline__ void fix_delta_to(const Params& P, const Geom& G, con
   for (int pl = 0; pl < HARM_NPR; pl++) dfl[pl] += dV * (Ua[pl] - Ub[pl]);
 }
 
+// ---------------------------------------------------------------------------
+// inversion + fixup1zone (floors via ZAMO-frame mass/energy addition)
+// ---------------------------------------------------------------------------
+// One zone: conserved Uz (with gdet) in registers, the guess read from pg at
+// offset o, the result written to pf and pflag.  Returns the flag; the Newton
+// stroke count is added to ns.
+//
+// SEEDED: the Newton solve starts from W' = wp_seed (the learned guess); when
+// that solve fails, the zone is solved again from the primitives' guess as
+// the plain inversion does (Newton from the previous primitives is the
+// fallback and decides), and *fellback is set.
+// DIAG (final substep): the states each fix started from are kept for the
+// energy-file accounting (diag_fixup, fixup.c:220): fx_flag[o] gets bit 0
+// when the floors changed the zone (fixup.c:766) and bit 1 when the gamma
+// limit did (fixup.c:784); fx_p0 (SoA, stride ncell) then holds the
+// primitives ahead of both fixes and, when the floors changed them, fx_pfl
+// the primitives after the floors.  Only fixed zones store their states.
 template <bool SEEDED = false, bool DIAG = false>
 __device__ __forceinline__ int invert_zone(const Dims& d, const Params& P, double* Uz,
                                            const double* __restrict__ pg, double* __restrict__ pf,
                                            int32_t* __restrict__ pflag, const double* __restrict__ gcent,
                                            const double* __restrict__ floors, int finalstep, long o, long pp, int& ns,
-                                           double wp_seed = 0.0, int* fellback = nullptr, double* dfl = nullptr,
-                                           bool* changed = nullptr)
+                                           double wp_seed = 0.0, int* fellback = nullptr,
+                                           double* __restrict__ fx_p0 = nullptr, double* __restrict__ fx_pfl = nullptr,
+                                           uint8_t* __restrict__ fx_flag = nullptr)
 {
   const long nc = d.ncell;
   Geom G;
@@ -70,11 +73,6 @@ __device__ __forceinline__ int invert_zone(const Dims& d, const Params& P, doubl
   // registers during the Newton iterations (they come back from L1/L2).
   hd::load_geom(hd::fresh(gcent), pp, d.nplane, G);
   if (finalstep) {
-    double p0[HARM_NPR];
-    if (DIAG) {
-#pragma unroll
-      for (int pl = 0; pl < HARM_NPR; pl++) p0[pl] = pz[pl];
-    }
     double prfloor[2] = {floors[pp], floors[d.nplane + pp]};
     for (int pl = 0; pl < 2; pl++)
       if (prfloor[pl] < D_SMALL) prfloor[pl] = D_SMALL;
@@ -86,6 +84,10 @@ __device__ __forceinline__ int invert_zone(const Dims& d, const Params& P, doubl
       else prnew[pl] = pz[pl];
     }
     if (didchange) {
+      if (DIAG) {
+#pragma unroll
+        for (int pl = 0; pl < HARM_NPR; pl++) fx_p0[pl * nc + o] = pz[pl];
+      }
       Geom G1 = G;
       G1.gdet = 1.0;
       State q, dq;
@@ -104,10 +106,23 @@ __device__ __forceinline__ int invert_zone(const Dims& d, const Params& P, doubl
 #pragma unroll
         for (int pl = 0; pl < HARM_NPR; pl++) pz[pl] = prnew[pl];
       }
-      if (DIAG) { fix_delta_to(P, G, p0, pz, dfl); changed[0] = true; }
+    }
+    double pb4[HARM_NPR];  // DIAG: the primitives ahead of the gamma limit
+    if (DIAG) {
+#pragma unroll
+      for (int pl = 0; pl < HARM_NPR; pl++) pb4[pl] = pz[pl];
     }
     const bool lg = hd::limit_gamma(P.gammamax, pz, G);
-    if (DIAG && lg) { fix_delta_to(P, G, p0, pz, dfl + HARM_NPR); changed[1] = true; }
+    if (DIAG) {
+      if (lg) {
+        double* dst = didchange ? fx_pfl : fx_p0;  // with no floor step these are also the primitives ahead of both
+#pragma unroll
+        for (int pl = 0; pl < HARM_NPR; pl++) dst[pl * nc + o] = pb4[pl];
+      }
+      fx_flag[o] = (uint8_t)((didchange ? 1 : 0) | (lg ? 2 : 0));
+    }
+  } else if (DIAG) {
+    fx_flag[o] = 0;
   }
 #pragma unroll
   for (int pl = 0; pl < HARM_NPR; pl++) pf[pl * nc + o] = pz[pl];

@@ -10,14 +10,18 @@
 //   k_ener_totals    conserved totals sum U dV (constotal, dump_ener.c:882)
 //   k_ener_divb      max and sum of |div B| at the corners (divbmaxavg,
 //                    dump_ener.c:680, SETFDIVBFLUXCTTOTHPRIM)
-//   k_ener_reduce    fixed-order sums of per-block partials
+//   k_ener_reduce    fixed-order sums of per-block partials, one block per
+//                    channel
 //
-// The floor, gamma-limit and failure-averaging changes (fladd) are summed by
-// the inversion and failure-averaging kernels themselves (their DIAG
-// variants) into per-block partials that k_ener_reduce adds up.  Every sum
-// is reduced in a fixed order, so the same state gives the same bits.
+//   k_ener_fix       the final substep's floor and gamma-limit changes from
+//                    the states the inversion kept (diag_fixup, fixup.c:220)
+//
+// The failure-averaging changes are summed by the failure-averaging kernel
+// itself (its DIAG variant) into per-block partials.  Every sum is reduced
+// in a fixed order, so the same state gives the same bits.
 #include <hip/hip_runtime.h>
 #include "gkern.h"
+#include "dinvert.h"
 
 namespace hk {
 
@@ -154,6 +158,47 @@ __global__ __launch_bounds__(kBlock) void k_ener_totals(Dims d, Params P, const 
   block_sum8_part(v, on, part, blockIdx.x);
 }
 
+// dV (U(after) - U(before)) of each fix: floors (flag bit 0) from fx_p0 to
+// the after-floor state (fx_pfl when the gamma limit followed, else pf); the
+// gamma limit (bit 1) from fx_p0 to pf.  Per block into part (floors) and
+// part + nblk * 8 (gamma limits).
+__global__ __launch_bounds__(kBlock) void k_ener_fix(Dims d, Params P, const double* __restrict__ pf,
+                                                     const double* __restrict__ gcent,
+                                                     const double* __restrict__ fx_p0,
+                                                     const double* __restrict__ fx_pfl,
+                                                     const uint8_t* __restrict__ fx_flag, double* __restrict__ part)
+{
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  double vf[HARM_NPR] = {0, 0, 0, 0, 0, 0, 0, 0}, vg[HARM_NPR] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int fl = 0;
+  if (t < d.n1 * d.n2 * d.n3) {
+    int i, j, k;
+    zone_ijk(d, t, i, j, k);
+    const long o = hd::cidx(d, i, j, k), nc = d.ncell;
+    fl = fx_flag[o];
+    if (fl) {
+      Geom G;
+      hd::load_geom(gcent, hd::pidx(d, i, j), d.nplane, G);
+      double p0[HARM_NPR], pa[HARM_NPR];
+#pragma unroll
+      for (int pl = 0; pl < HARM_NPR; pl++) p0[pl] = fx_p0[pl * nc + o];
+      if (fl & 1) {
+        const double* src = (fl & 2) ? fx_pfl : pf;
+#pragma unroll
+        for (int pl = 0; pl < HARM_NPR; pl++) pa[pl] = src[pl * nc + o];
+        fix_delta_to(P, G, p0, pa, vf);
+      }
+      if (fl & 2) {
+#pragma unroll
+        for (int pl = 0; pl < HARM_NPR; pl++) pa[pl] = pf[pl * nc + o];
+        fix_delta_to(P, G, p0, pa, vg);
+      }
+    }
+  }
+  block_sum8_part(vf, (fl & 1) != 0, part, blockIdx.x);
+  block_sum8_part(vg, (fl & 2) != 0, part + (long)gridDim.x * 8, blockIdx.x);
+}
+
 // |div B| at the corner (i-1/2, j-1/2, k-1/2) of zone (i, j, k)
 // (DIVBFLUXCTTOTHPRIM and its normalisation, global.fieldmacros.h:223-244):
 // gdet B averaged onto the corner-centred faces, differenced, divided by the
@@ -229,22 +274,31 @@ __global__ __launch_bounds__(kBlock) void k_ener_divb(Dims d, Params P, const do
   }
 }
 
-// en[dst + c] += sum_b part[b][c]: 32 groups of 8 lanes each sum every
-// 32nd block in increasing order, then the 32 group sums in order.
-__global__ __launch_bounds__(256) void k_ener_reduce(const double* __restrict__ part, long n, double* __restrict__ en,
-                                                     int dst, int assign)
+// One block per channel k < nchan: en[dst.v[k] + c] += sum_b part_k[b][c],
+// part_k = part + k * n * 8.  128 groups of 8 lanes each sum every 128th
+// block in increasing order (loads unrolled so they overlap), then lane c
+// adds the 128 group sums in order: a fixed order, so fixed bits.
+struct Dst4 {
+  int v[4];
+};
+constexpr int kRedThreads = 1024;
+__global__ __launch_bounds__(kRedThreads) void k_ener_reduce(const double* __restrict__ part, long n,
+                                                             double* __restrict__ en, Dst4 dst, int assign)
 {
-  __shared__ double red[32][8];
+  __shared__ double red[kRedThreads / 8][8];
+  const double* pk = part + (long)blockIdx.x * n * 8;
   const int c = threadIdx.x & 7, g = threadIdx.x >> 3;
   double s = 0.0;
-  for (long b = g; b < n; b += 32) s += part[b * 8 + c];
+#pragma unroll 8
+  for (long b = g; b < n; b += kRedThreads / 8) s += pk[b * 8 + c];
   red[g][c] = s;
   __syncthreads();
   if (threadIdx.x < 8) {
     double x = red[0][c];
-    for (int q = 1; q < 32; q++) x += red[q][c];
-    if (assign) en[dst + c] = x;
-    else en[dst + c] += x;
+    for (int q = 1; q < kRedThreads / 8; q++) x += red[q][c];
+    const int d = dst.v[blockIdx.x];
+    if (assign) en[d + c] = x;
+    else en[d + c] += x;
   }
 }
 
@@ -284,9 +338,20 @@ void launch_ener_pdot(const hd::Dims& d, const Params& P, const double* F1, cons
   hipLaunchKernelGGL(k_ener_pdot_final, dim3(1), dim3(64), 0, s, part, en, ds, weight);
 }
 
-void launch_ener_reduce(const double* part, long nb, double* en, int dst, hipStream_t s)
+void launch_ener_reduce(const double* part, long nb, double* en, const int* dst, int nchan, hipStream_t s)
 {
-  hipLaunchKernelGGL(k_ener_reduce, dim3(1), dim3(256), 0, s, part, nb, en, dst, 0);
+  Dst4 d{{0, 0, 0, 0}};
+  for (int k = 0; k < nchan && k < 4; k++) d.v[k] = dst[k];
+  hipLaunchKernelGGL(k_ener_reduce, dim3(nchan), dim3(kRedThreads), 0, s, part, nb, en, d, 0);
+}
+
+void launch_ener_fix(const hd::Dims& d, const Params& P, const double* pf, const double* gcent, const double* fx_p0,
+                     const double* fx_pfl, const uint8_t* fx_flag, double* part, double* en, hipStream_t s)
+{
+  const unsigned nb = nblk((long)d.n1 * d.n2 * d.n3);
+  hipLaunchKernelGGL(k_ener_fix, dim3(nb), dim3(kBlock), 0, s, d, P, pf, gcent, fx_p0, fx_pfl, fx_flag, part);
+  const int dst[2] = {EN_FLTERMS + FL_FLOOR * HARM_NPR, EN_FLTERMS + FL_GAMMA * HARM_NPR};
+  launch_ener_reduce(part, nb, en, dst, 2, s);
 }
 
 void launch_ener_dt4(double* en, const double* ds, int last, hipStream_t s)
@@ -299,7 +364,8 @@ void launch_ener_source(const hd::Dims& d, const Params& P, const double* pb, co
 {
   const unsigned nb = nblk((long)d.n1 * d.n2 * d.n3);
   hipLaunchKernelGGL(k_ener_source, dim3(nb), dim3(kBlock), 0, s, d, P, pb, gcent, conn, en, part);
-  hipLaunchKernelGGL(k_ener_reduce, dim3(1), dim3(256), 0, s, part, (long)nb, en, (int)EN_SRCADD, 0);
+  hipLaunchKernelGGL(k_ener_reduce, dim3(1), dim3(kRedThreads), 0, s, part, (long)nb, en, Dst4{{EN_SRCADD, 0, 0, 0}},
+                     0);
 }
 
 void launch_ener_totals(const hd::Dims& d, const Params& P, const double* p, const double* gcent, double* part,
@@ -307,7 +373,7 @@ void launch_ener_totals(const hd::Dims& d, const Params& P, const double* p, con
 {
   const unsigned nb = nblk((long)d.n1 * d.n2 * d.n3);
   hipLaunchKernelGGL(k_ener_totals, dim3(nb), dim3(kBlock), 0, s, d, P, p, gcent, part);
-  hipLaunchKernelGGL(k_ener_reduce, dim3(1), dim3(256), 0, s, part, (long)nb, out, 0, 1);
+  hipLaunchKernelGGL(k_ener_reduce, dim3(1), dim3(kRedThreads), 0, s, part, (long)nb, out, Dst4{{0, 0, 0, 0}}, 1);
   hipLaunchKernelGGL(k_ener_divb, dim3(nb), dim3(kBlock), 0, s, d, P, p, gcent, part);
   hipLaunchKernelGGL(k_ener_divb_final, dim3(1), dim3(64), 0, s, part, (long)nb, out);
 }

@@ -209,13 +209,12 @@ __device__ __forceinline__ void zone_ijk(const Dims& d, int t, int& i, int& j, i
 
 // The step's inversion started from a learned W' (mlp.hip): harm_step's
 // k_invert with the network in front of the Newton solve.
-// With fl_part, the final substep's conserved changes from floors and gamma
-// limits are summed per block (block_sum8_part) into fl_part[0][blk] and
-// fl_part[1][blk] (each nblk x 8).
+// With fx_p0, the states the final substep's fixes started from are kept
+// as k_invert<true> keeps them (invert_zone).
 harm_status launch_invert_grid_mlp(const harm_mlp* m, const hd::Dims& d, const Params& P, const double* U,
                                    const double* pg, double* pf, int32_t* pflag, const double* gcent,
                                    const double* floors, int finalstep, unsigned long long* di, hipStream_t s,
-                                   double* fl_part = nullptr);
+                                   double* fx_p0 = nullptr, double* fx_pfl = nullptr, uint8_t* fx_flag = nullptr);
 
 // Energy-file diagnostics (ener.hip).  `part` holds per-block partial sums:
 // at least 4 * ener_part_blocks(d) * 8 doubles.
@@ -223,8 +222,13 @@ long ener_part_blocks(const hd::Dims& d);
 // boundary flux sums of F1..F3 into en[EN_PDOT..], and en[EN_PCUM..] += pdot * dt * weight
 void launch_ener_pdot(const hd::Dims& d, const Params& P, const double* F1, const double* F2, const double* F3,
                       double* part, double* en, const double* ds, double weight, hipStream_t s);
-// en[dst + c] += sum over nblk blocks of part[b][c] (fixed order), c < 8
-void launch_ener_reduce(const double* part, long nblk, double* en, int dst, hipStream_t s);
+// for each channel k < nchan (at most 4): en[dst[k] + c] += sum over nblk
+// blocks of part[k][b][c] (fixed order), c < 8
+void launch_ener_reduce(const double* part, long nblk, double* en, const int* dst, int nchan, hipStream_t s);
+// the final substep's floor and gamma-limit changes into their fladdterms
+// from the states the inversion kept (fx_*) and the primitives pf after it
+void launch_ener_fix(const hd::Dims& d, const Params& P, const double* pf, const double* gcent, const double* fx_p0,
+                     const double* fx_pfl, const uint8_t* fx_flag, double* part, double* en, hipStream_t s);
 // source sample weight at the end of substep `to` of nto (advance.c:251-262)
 void launch_ener_dt4(double* en, const double* ds, int last, hipStream_t s);
 // sourceadd += dt4diag * dV * dUgeom(pb) over active zones when dt4diag > 0

@@ -281,21 +281,21 @@ __global__ __launch_bounds__(kBlock) void k_update(Dims d, Params P, const doubl
   update_body<DM, MODE>(d, P, pi, pb, F1, F2, F3, ui, uf, ucum, gcent, conn, cuf0, cuf1, cuf2, cun0, cun1, cun2, timeorder, ds, xcd_tile(blockIdx.x, gridDim.x));
 }
 
-// DIAG: also the final substep's floor and gamma-limit changes, summed per
-// block into fl_part (floors) and fl_part + nblk * 8 (gamma limits).
+// DIAG: also keeps the states the final substep's fixes started from
+// (invert_zone; fx = {p0, after-floor, flags}).
 template <bool DIAG = false>
 __device__ __forceinline__ void invert_body(Dims d, Params P, const double* __restrict__ U,
                                                    const double* __restrict__ pg, double* __restrict__ pf,
                                                    int32_t* __restrict__ pflag,
                                                    const double* __restrict__ gcent, const double* __restrict__ floors,
                                                    int finalstep, unsigned long long* __restrict__ di, const int blk,
-                                                   double* __restrict__ fl_part = nullptr)
+                                                   double* __restrict__ fx_p0 = nullptr,
+                                                   double* __restrict__ fx_pfl = nullptr,
+                                                   uint8_t* __restrict__ fx_flag = nullptr)
 {
   const int t = blk * kBlock + threadIdx.x;
   const int nact = d.n1 * d.n2 * d.n3;
   int ns = 0, nf = 0;
-  double dfl[2 * HARM_NPR] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-  bool changed[2] = {false, false};
   if (t < nact) {
     int i, j, k;
     zone_ijk(d, t, i, j, k);
@@ -305,14 +305,10 @@ __device__ __forceinline__ void invert_body(Dims d, Params P, const double* __re
     double Uz[HARM_NPR];
 #pragma unroll
     for (int pl = 0; pl < HARM_NPR; pl++) Uz[pl] = U[pl * nc + o];
-    nf = invert_zone<false, DIAG>(d, P, Uz, pg, pf, pflag, gcent, floors, finalstep, o, pp, ns, 0.0, nullptr, dfl,
-                                  changed) > 0;
+    nf = invert_zone<false, DIAG>(d, P, Uz, pg, pf, pflag, gcent, floors, finalstep, o, pp, ns, 0.0, nullptr, fx_p0,
+                                  fx_pfl, fx_flag) > 0;
   }
   newton_counts_to(ns, nf, di);
-  if (DIAG) {
-    block_sum8_part(dfl, changed[0], fl_part, blk);
-    block_sum8_part(dfl + HARM_NPR, changed[1], fl_part + (long)gridDim.x * 8, blk);
-  }
 }
 
 // Four waves per SIMD (128 VGPRs, a few bytes of spill outside the Newton
@@ -323,9 +319,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
                                                    int32_t* __restrict__ pflag,
                                                    const double* __restrict__ gcent, const double* __restrict__ floors,
                                                    int finalstep, unsigned long long* __restrict__ di,
-                                                   double* __restrict__ fl_part)
+                                                   double* __restrict__ fx_p0, double* __restrict__ fx_pfl,
+                                                   uint8_t* __restrict__ fx_flag)
 {
-  invert_body<DIAG>(d, P, U, pg, pf, pflag, gcent, floors, finalstep, di, xcd_tile(blockIdx.x, gridDim.x), fl_part);
+  invert_body<DIAG>(d, P, U, pg, pf, pflag, gcent, floors, finalstep, di, xcd_tile(blockIdx.x, gridDim.x), fx_p0,
+                    fx_pfl, fx_flag);
 }
 
 // ---------------------------------------------------------------------------
@@ -397,6 +395,10 @@ __global__ __launch_bounds__(kBlock) void k_post_fixup(Dims d, Params P, const d
   double dfl[HARM_NPR] = {0, 0, 0, 0, 0, 0, 0, 0};
   int cat = 0;
   const bool ch = post_fixup_body<DIAG>(d, P, pa, pf, pflag, gcent, blockIdx.x, dfl, &cat);
+  if (DIAG && !__syncthreads_or(ch)) {  // no failed zone in the block
+    if (threadIdx.x < 32) fl_part[(long)(threadIdx.x >> 3) * gridDim.x * 8 + (long)blockIdx.x * 8 + (threadIdx.x & 7)] = 0.0;
+    return;
+  }
   if (DIAG)  // one partial per failure category: fl_part + c * nblk * 8
     for (int c = 0; c < 4; c++) {
       const bool mine = ch && cat == c;
@@ -847,6 +849,9 @@ struct harm_handle {
   double* enpart = nullptr;  // their per-block partial sums
   double* entot = nullptr;   // on-demand totals: U[8], divb max, divb sum
   bool ener_on = false;      // steps add to the energy-file accumulators
+  double* fx_p0 = nullptr;   // with ener_on: states the final substep's fixes started from
+  double* fx_pfl = nullptr;
+  uint8_t* fx_flag = nullptr;
   const harm_mlp* guess = nullptr;  // learned inversion guess (harm_set_inversion_guess), or none
   int tile_td[4] = {0, 0, 0, 0};  // sweep depth of the face tiles of dirs 2,3 (0: cost model)
   hipGraph_t sgraph = nullptr;      // one full step, captured on first use when not profiling
@@ -1140,19 +1145,20 @@ harm_status enq_invert(harm_handle* h, int which_pb, int which_pf, int finalstep
   if (h->guess) {
     HH_TRY(launch_invert_grid_mlp(h->guess, d, h->P, U, h->arr[which_pb], h->arr[which_pf], h->pflag,
                                   h->geom[HARM_LOC_CENT], h->floors, finalstep, h->di, h->stream,
-                                  diag ? h->enpart : nullptr));
+                                  diag ? h->fx_p0 : nullptr, h->fx_pfl, h->fx_flag));
   } else if (diag) {
     hipLaunchKernelGGL(k_invert<true>, dim3(nb), dim3(kBlock), 0, h->stream, d, h->P, U, h->arr[which_pb],
-                       h->arr[which_pf], h->pflag, h->geom[HARM_LOC_CENT], h->floors, finalstep, h->di, h->enpart);
+                       h->arr[which_pf], h->pflag, h->geom[HARM_LOC_CENT], h->floors, finalstep, h->di, h->fx_p0,
+                       h->fx_pfl, h->fx_flag);
   } else {
     hipLaunchKernelGGL(k_invert<false>, dim3(nb), dim3(kBlock), 0, h->stream, d, h->P, U, h->arr[which_pb],
-                       h->arr[which_pf], h->pflag, h->geom[HARM_LOC_CENT], h->floors, finalstep, h->di, nullptr);
+                       h->arr[which_pf], h->pflag, h->geom[HARM_LOC_CENT], h->floors, finalstep, h->di, nullptr,
+                       nullptr, nullptr);
   }
   prof_end(h, HARM_K_INVERT);
-  if (diag) {
-    launch_ener_reduce(h->enpart, nb, h->en, EN_FLTERMS + FL_FLOOR * HARM_NPR, h->stream);
-    launch_ener_reduce(h->enpart + (long)nb * 8, nb, h->en, EN_FLTERMS + FL_GAMMA * HARM_NPR, h->stream);
-  }
+  if (diag)
+    launch_ener_fix(d, h->P, h->arr[which_pf], h->geom[HARM_LOC_CENT], h->fx_p0, h->fx_pfl, h->fx_flag, h->enpart,
+                    h->en, h->stream);
   return hh::check_launch("k_invert");
 }
 
@@ -1195,9 +1201,11 @@ harm_status enq_post_fixup(harm_handle* h, int which, bool ener = false)
     hipLaunchKernelGGL(k_post_fixup<false>, dim3(nb), dim3(kBlock), 0, h->stream, d, h->P, h->arr[which],
                        h->arr[which], h->pflag, h->geom[HARM_LOC_CENT], nullptr);
   prof_end(h, HARM_K_FIXUP);
-  if (ener)
-    for (int c = 0; c < 4; c++)
-      launch_ener_reduce(h->enpart + (long)c * nb * 8, nb, h->en, EN_FLTERMS + kFailCats[c] * HARM_NPR, h->stream);
+  if (ener) {
+    int dst[4];
+    for (int c = 0; c < 4; c++) dst[c] = EN_FLTERMS + kFailCats[c] * HARM_NPR;
+    launch_ener_reduce(h->enpart, nb, h->en, dst, 4, h->stream);
+  }
   return hh::check_launch("k_post_fixup");
 }
 
@@ -1662,7 +1670,7 @@ extern "C" harm_status harm_create(const harm_config* cfg, harm_handle** out)
   TRYC(alloc((void**)&h->facc, sizeof(double) * 4 * kFaccBlocks));
   TRYC(alloc((void**)&h->en, sizeof(double) * EN_N));
   TRYC(hh::check_hip(hipMemset(h->en, 0, sizeof(double) * EN_N), "hipMemset"));
-  TRYC(alloc((void**)&h->enpart, sizeof(double) * 4 * 8 * ener_part_blocks(d)));
+  TRYC(alloc((void**)&h->enpart, sizeof(double) * 4 * 8 * ener_part_blocks(d)));  // 4 channels
   TRYC(alloc((void**)&h->entot, sizeof(double) * 10));
   TRYC(hh::check_hip(hipMemset(h->ds, 0, sizeof(double) * DS_N), "hipMemset"));
   TRYC(alloc((void**)&h->di, sizeof(unsigned long long) * DI_N));
@@ -1691,7 +1699,7 @@ extern "C" harm_status harm_destroy(harm_handle* h)
   }
   for (hipEvent_t e : h->evpool) (void)hipEventDestroy(e);
   void* bufs[] = {h->pflag, h->gcorn3, h->conn, h->floors, h->rcent, h->emf, h->facc, h->en, h->enpart, h->entot,
-                  h->ds, h->di};
+                  h->fx_p0, h->fx_pfl, h->fx_flag, h->ds, h->di};
   for (void* b : bufs) (void)hipFree(b);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
@@ -2072,6 +2080,12 @@ extern "C" harm_status harm_set_ener_tracking(harm_handle* h, int32_t on)
 {
   HH_TRY(check_handle(h));
   HH_HIP(hipStreamSynchronize(h->stream));
+  if (on && !h->fx_p0) {  // the fix-up states, allocated on first use
+    const size_t n = (size_t)HARM_NPR * h->d.ncell;
+    HH_TRY(alloc((void**)&h->fx_p0, sizeof(double) * n));
+    HH_TRY(alloc((void**)&h->fx_pfl, sizeof(double) * n));
+    HH_TRY(alloc((void**)&h->fx_flag, (size_t)h->d.ncell));
+  }
   if (h->ener_on != (on != 0)) drop_step_graph(h);  // the captured step holds the other kernel set
   h->ener_on = on != 0;
   return harm_reset_ener(h);

@@ -507,7 +507,7 @@ harm_status pack(harm_mlp* m, hipStream_t s)
 // go through the network on the matrix cores, each zone's Newton solve then
 // starts from the predicted W' (falling back to the previous primitives when
 // it fails), and the final substep's floors follow, as in k_invert (DIAG:
-// with their conserved changes summed per block into fl_part).
+// keeping the states the fixes started from, as k_invert<true>).
 template <bool DIAG>
 __global__ __launch_bounds__(kBlock) void k_invert_grid_mlp(const uint16_t* __restrict__ wbf,
                                                             const float* __restrict__ w32,
@@ -518,7 +518,8 @@ __global__ __launch_bounds__(kBlock) void k_invert_grid_mlp(const uint16_t* __re
                                                             const double* __restrict__ gcent,
                                                             const double* __restrict__ floors, int finalstep,
                                                             unsigned long long* __restrict__ di,
-                                                            double* __restrict__ fl_part)
+                                                            double* __restrict__ fx_p0, double* __restrict__ fx_pfl,
+                                                            uint8_t* __restrict__ fx_flag)
 {
   __shared__ Fwd S;
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
@@ -557,22 +558,16 @@ __global__ __launch_bounds__(kBlock) void k_invert_grid_mlp(const uint16_t* __re
   __syncthreads();
   wave_forward(S.bufA[w], S.bufB[w], S.sW, S.sBias, S.y[w]);
   int ns = 0, nf = 0, fb = 0;
-  double dfl[2 * HARM_NPR] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-  bool changed[2] = {false, false};
   if (act) {
     const float y0 = S.y[w][l * 2] * norm[2 * hm::kIn + hm::kOut] + norm[2 * hm::kIn];
     const double seed = D * exp((double)fminf(fmaxf(y0, -60.0f), 60.0f));
     double Uz[HARM_NPR];
 #pragma unroll
     for (int pl = 0; pl < HARM_NPR; pl++) Uz[pl] = U[pl * d.ncell + o];
-    nf = hk::invert_zone<true, DIAG>(d, P, Uz, pg, pf, pflag, gcent, floors, finalstep, o, pp, ns, seed, &fb, dfl,
-                                     changed) > 0;
+    nf = hk::invert_zone<true, DIAG>(d, P, Uz, pg, pf, pflag, gcent, floors, finalstep, o, pp, ns, seed, &fb,
+                                     fx_p0, fx_pfl, fx_flag) > 0;
   }
   hk::newton_counts_to(ns, nf, di);
-  if (DIAG) {
-    hk::block_sum8_part(dfl, changed[0], fl_part, lb);
-    hk::block_sum8_part(dfl + HARM_NPR, changed[1], fl_part + (long)gridDim.x * 8, lb);
-  }
   __shared__ unsigned int nfb;
   if (threadIdx.x == 0) nfb = 0;
   __syncthreads();
@@ -762,14 +757,14 @@ extern "C" harm_status harm_invert_batch_mlp(harm_mlp* m, const double* U, const
 harm_status hk::launch_invert_grid_mlp(const harm_mlp* m, const hd::Dims& d, const hk::Params& P, const double* U,
                                        const double* pg, double* pf, int32_t* pflag, const double* gcent,
                                        const double* floors, int finalstep, unsigned long long* di, hipStream_t s,
-                                       double* fl_part)
+                                       double* fx_p0, double* fx_pfl, uint8_t* fx_flag)
 {
   const long nb = ((long)d.n1 * d.n2 * d.n3 + kBlock - 1) / kBlock;
-  if (fl_part)
+  if (fx_p0)
     hipLaunchKernelGGL(k_invert_grid_mlp<true>, dim3((unsigned)nb), dim3(kBlock), 0, s, m->wbf, m->w32, m->norm, d, P,
-                       U, pg, pf, pflag, gcent, floors, finalstep, di, fl_part);
+                       U, pg, pf, pflag, gcent, floors, finalstep, di, fx_p0, fx_pfl, fx_flag);
   else
     hipLaunchKernelGGL(k_invert_grid_mlp<false>, dim3((unsigned)nb), dim3(kBlock), 0, s, m->wbf, m->w32, m->norm, d,
-                       P, U, pg, pf, pflag, gcent, floors, finalstep, di, nullptr);
+                       P, U, pg, pf, pflag, gcent, floors, finalstep, di, nullptr, nullptr, nullptr);
   return hh::check_launch("k_invert_grid_mlp");
 }
