@@ -118,6 +118,10 @@ _OPTIONAL_PROTOS = [
     ("harm_set_ener_tracking", _I, [_P, _I]),
     ("harm_reset_ener", _I, [_P]),
     ("harm_get_ener", _I, [_P, _I, _P]),
+    ("harm_dump_columns", _I, [_P, _I, _P]),
+    ("harm_gdump_columns", _I, [_P, _P]),
+    ("harm_write_dump", _I, [_P, _I, ctypes.c_char_p]),
+    ("harm_write_gdump", _I, [_P, ctypes.c_char_p]),
     ("harm_get_stream", _I, [_P, ctypes.POINTER(_P)]),
     ("harm_profile", _I, [_P, _I]),
     ("harm_profile_read", _I, [_P, _P, _P]),

@@ -12,6 +12,7 @@
 //                    dump_ener.c:680, SETFDIVBFLUXCTTOTHPRIM)
 //   k_ener_reduce    fixed-order sums of per-block partials, one block per
 //                    channel
+//   k_dump_cols      the physics columns of HARM's main dump (dump.c:439)
 //
 //   k_ener_fix       the final substep's floor and gamma-limit changes from
 //                    the states the inversion kept (diag_fixup, fixup.c:220)
@@ -323,7 +324,100 @@ __global__ void k_ener_divb_final(const double* __restrict__ part, long n, doubl
   }
 }
 
+// The physics columns of one zone of HARM's main dump (dump_content,
+// dump.c:439-600, CALCFARADAYANDCURRENTS): out[c * nz + t] with t = i + n1 (j
+// + n2 k), the dump's row order, and c in DC_* order.  U is the conserved
+// state the last inversion consumed (udump = unewglobal, initbase.c:885).
+__global__ __launch_bounds__(kBlock) void k_dump_cols(Dims d, Params P, const double* __restrict__ p,
+                                                      const double* __restrict__ ucum,
+                                                      const double* __restrict__ gcent, double* __restrict__ out)
+{
+  const long nz = (long)d.n1 * d.n2 * d.n3;
+  const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= nz) return;
+  const int i = (int)(t % d.n1), j = (int)((t / d.n1) % d.n2), k = (int)(t / ((long)d.n1 * d.n2));
+  const long o = hd::cidx(d, i, j, k), nc = d.ncell;
+  Geom G;
+  hd::load_geom(gcent, hd::pidx(d, i, j), d.nplane, G);
+  double pz[HARM_NPR];
+#pragma unroll
+  for (int pl = 0; pl < HARM_NPR; pl++) pz[pl] = p[pl * nc + o];
+  auto put = [&](int c, double v) { out[(long)c * nz + t] = v; };
+#pragma unroll
+  for (int pl = 0; pl < HARM_NPR; pl++) put(DC_P + pl, pz[pl]);
+  // ideal-gas EOS columns (idealgaseos.c:19, 48, 71)
+  const double rho0 = pz[HARM_RHO], u = pz[HARM_UU];
+  const double pressure = (P.gam - 1.) * u;
+  put(DC_PRESS, pressure);
+  put(DC_CS2, P.gam * pressure / (rho0 + u + pressure));
+  const double indexn = 1.0 / (P.gam - 1.);
+  double Sden;
+  if (rho0 < D_SMALL && pressure < D_SMALL) Sden = 0.0;
+  else if (rho0 < D_SMALL) Sden = 0.0;
+  else if (pressure < D_SMALL) Sden = -kBIG;
+  else Sden = rho0 * log(pow(pressure, indexn) / pow(rho0, indexn + 1.0));
+  put(DC_SDEN, Sden);
+#pragma unroll
+  for (int pl = 0; pl < HARM_NPR; pl++) put(DC_U + pl, ucum[pl * nc + o]);
+  put(DC_DIVB, corner_divb(d, P, p, gcent, i, j, k));
+  State q;
+  hd::get_state(pz, G, P.gam, q);
+  double bcon[4], bcov[4];
+  bcon[0] = pz[HARM_B1] * q.ucov[1] + pz[HARM_B2] * q.ucov[2] + pz[HARM_B3] * q.ucov[3];
+#pragma unroll
+  for (int m = 1; m < 4; m++) bcon[m] = (pz[HARM_B1 - 1 + m] + bcon[0] * q.ucon[m]) / q.ucon[0];
+  hd::lower(bcon, G, bcov);
+#pragma unroll
+  for (int m = 0; m < 4; m++) {
+    put(DC_UCON + m, q.ucon[m]);
+    put(DC_UCOV + m, q.ucov[m]);
+    put(DC_BCON + m, bcon[m]);
+    put(DC_BCOV + m, bcov[m]);
+  }
+#pragma unroll
+  for (int dir = 1; dir <= 3; dir++) {
+    double vmax, vmin;
+    hd::vchar(pz, q, dir, G, P.gam, vmax, vmin);
+    put(DC_VCHAR + 2 * (dir - 1), vmin);
+    put(DC_VCHAR + 2 * (dir - 1) + 1, vmax);
+  }
+  put(DC_GDET, G.gdet);
+  // Faraday F^{mu nu} at the centre (current_precalc CURTYPEFARADAY,
+  // phys.c:701-708) and F_{mu nu} (lowerf, phys.tools.c:1449)
+  const double ig = -1.0 / G.gdet;
+  double f[6];
+  f[0] = ig * (bcov[3] * q.ucov[2] - bcov[2] * q.ucov[3]);
+  f[1] = ig * (-bcov[3] * q.ucov[1] + bcov[1] * q.ucov[3]);
+  f[2] = ig * (bcov[2] * q.ucov[1] - bcov[1] * q.ucov[2]);
+  f[3] = ig * (bcov[3] * q.ucov[0] - bcov[0] * q.ucov[3]);
+  f[4] = ig * (-bcov[2] * q.ucov[0] + bcov[0] * q.ucov[2]);
+  f[5] = ig * (bcov[1] * q.ucov[0] - bcov[0] * q.ucov[1]);
+  double F[4][4];
+  F[0][0] = F[1][1] = F[2][2] = F[3][3] = 0.0;
+  F[0][1] = f[0]; F[0][2] = f[1]; F[0][3] = f[2]; F[1][2] = f[3]; F[1][3] = f[4]; F[2][3] = f[5];
+  F[1][0] = -f[0]; F[2][0] = -f[1]; F[3][0] = -f[2]; F[2][1] = -f[3]; F[3][1] = -f[4]; F[3][2] = -f[5];
+  const int lo[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+#pragma unroll
+  for (int c = 0; c < 6; c++) {
+    const int a = lo[c][0], b = lo[c][1];
+    double s = 0.0;
+#pragma unroll
+    for (int jp = 0; jp < 4; jp++)
+#pragma unroll
+      for (int kp = 0; kp < 4; kp++) s += F[jp][kp] * hd::gco(G, a, jp) * hd::gco(G, b, kp);
+    put(DC_FCON + c, f[c]);
+    put(DC_FCOV + c, s);
+  }
+}
+
 }  // namespace
+
+void launch_dump_cols(const hd::Dims& d, const Params& P, const double* p, const double* ucum, const double* gcent,
+                      double* out, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_dump_cols, dim3(nblk((long)d.n1 * d.n2 * d.n3)), dim3(kBlock), 0, s, d, P, p, ucum, gcent,
+                     out);
+}
 
 long ener_part_blocks(const hd::Dims& d)
 {

@@ -229,6 +229,15 @@ void launch_ener_reduce(const double* part, long nblk, double* en, const int* ds
 // from the states the inversion kept (fx_*) and the primitives pf after it
 void launch_ener_fix(const hd::Dims& d, const Params& P, const double* pf, const double* gcent, const double* fx_p0,
                      const double* fx_pfl, const uint8_t* fx_flag, double* part, double* en, hipStream_t s);
+// The main dump's physics columns (dump_content, dump.c:439-600), device
+// SoA out[c][nz] in the dump's row order (i fastest, then j, then k): the
+// primitives, pressure, cs2, entropy density, U (from ucum), |div B|, u^mu,
+// u_mu, b^mu, b_mu, (vmin, vmax) per direction, gdet, F^{mu nu}, F_{mu nu}.
+// The coordinates and the current columns are the host's.
+enum { DC_P = 0, DC_PRESS = 8, DC_CS2 = 9, DC_SDEN = 10, DC_U = 11, DC_DIVB = 19, DC_UCON = 20, DC_UCOV = 24,
+       DC_BCON = 28, DC_BCOV = 32, DC_VCHAR = 36, DC_GDET = 42, DC_FCON = 43, DC_FCOV = 49, DC_N = 55 };
+void launch_dump_cols(const hd::Dims& d, const Params& P, const double* p, const double* ucum, const double* gcent,
+                      double* out, hipStream_t s);
 // source sample weight at the end of substep `to` of nto (advance.c:251-262)
 void launch_ener_dt4(double* en, const double* ds, int last, hipStream_t s);
 // sourceadd += dt4diag * dV * dUgeom(pb) over active zones when dt4diag > 0

@@ -17,6 +17,7 @@
 // so a whole step enqueues without a host round trip.
 #include <hip/hip_runtime.h>
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #include <vector>
@@ -2128,6 +2129,143 @@ extern "C" harm_status harm_get_ener(harm_handle* h, int32_t which, harm_ener* o
   out->divbmax = tot[8];
   out->divbavg = tot[9] / ((double)h->d.n1 * h->d.n2 * h->d.n3);
   return HARM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// HARM-format dumps (dump.c)
+// ---------------------------------------------------------------------------
+namespace {
+// coordinates of a zone centre, columns ti tj tk X1..X3 r th ph (bl_coord_ijk_2)
+void dump_coords(harm_handle* h, int i, int j, int k, double* row)
+{
+  double X[4], V[4];
+  harm_get_coords(h, i, j, k, X, V);
+  row[0] = i; row[1] = j; row[2] = k;
+  for (int m = 1; m <= 3; m++) { row[2 + m] = X[m]; row[5 + m] = V[m]; }
+}
+
+// the 2-D host geometry record of active zone (i, j)
+const double* host_rec(const harm_handle* h, int loc, int i, int j)
+{
+  const Dims& d = h->d;
+  return &h->hgeom[loc][((size_t)(j + d.ng2) * d.ns1 + (i + d.ng1)) * HARM_G_NFIELD];
+}
+
+// dump_header_general (dump.c:321-431), text form
+void dump_header(harm_handle* h, FILE* f, int whichdump, int ncol)
+{
+  double v[DS_N];
+  (void)hipMemcpy(v, h->ds, sizeof(v), hipMemcpyDeviceToHost);
+  const harm_config& c = h->cfg;
+  fprintf(f,
+          "%21.15g %d %d %d %21.15g %21.15g %21.15g %21.15g %21.15g %21.15g %ld %21.15g %21.15g %21.15g %21.15g "
+          "%21.15g %21.15g %21.15g %d %21.15g %21.15g %d %d %d %d %d %d %d %d %d\n",
+          v[DS_T], c.n1, c.n2, c.n3, h->P.startx[1], h->P.startx[2], h->P.startx[3], h->P.dx[1], h->P.dx[2],
+          h->P.dx[3], (long)v[DS_NSTEP], c.gam, c.a, c.R0, c.Rin, c.Rout, c.hslope, v[DS_DT], 0 /* LOGRSINTH */,
+          1.0 /* MBH */, 0.0 /* QBH */, 0, c.n1 - 1, 0, c.n2 - 1, 0, c.n3 - 1, whichdump, 0, ncol);
+}
+
+harm_status write_rows(FILE* f, const double* rows, long nz, int ncol)
+{
+  for (long z = 0; z < nz; z++) {
+    const double* r = rows + z * ncol;
+    for (int c = 0; c < ncol; c++) fprintf(f, "%21.15g ", r[c]);
+    fprintf(f, "\n");
+  }
+  return ferror(f) ? hh::set_error(HARM_EINVAL, "dump: write failed") : HARM_OK;
+}
+}  // namespace
+
+extern "C" harm_status harm_dump_columns(harm_handle* h, int32_t which, double* rows)
+{
+  HH_TRY(check_handle(h));
+  if (!rows) return hh::set_error(HARM_EINVAL, "harm_dump_columns: null argument");
+  if (which < 0 || which >= HARM_NARR) return hh::set_error(HARM_EINVAL, "harm_dump_columns: bad array %d", which);
+  const Dims& d = h->d;
+  const long nz = (long)d.n1 * d.n2 * d.n3;
+  double* dcol = nullptr;
+  HH_TRY(alloc((void**)&dcol, sizeof(double) * DC_N * nz));
+  launch_dump_cols(d, h->P, h->arr[which], h->arr[HARM_ARR_UCUM], h->geom[HARM_LOC_CENT], dcol, h->stream);
+  std::vector<double> col((size_t)DC_N * nz);
+  harm_status st = hh::check_launch("k_dump_cols");
+  if (st == HARM_OK) st = hh::check_hip(hipStreamSynchronize(h->stream), "harm_dump_columns");
+  if (st == HARM_OK)
+    st = hh::check_hip(hipMemcpy(col.data(), dcol, sizeof(double) * col.size(), hipMemcpyDeviceToHost), "copy");
+  (void)hipFree(dcol);
+  if (st != HARM_OK) return st;
+  // dump_content's column order: coordinates, then the device columns with
+  // the four-current j^mu, j_mu (8 columns, 0 here) between gdet and F
+  for (long t = 0; t < nz; t++) {
+    const int i = (int)(t % d.n1), j = (int)((t / d.n1) % d.n2), k = (int)(t / ((long)d.n1 * d.n2));
+    double* r = rows + t * HARM_DUMP_NCOL;
+    dump_coords(h, i, j, k, r);
+    int c = 9;
+    for (int q = 0; q <= DC_GDET; q++) r[c++] = col[(size_t)q * nz + t];
+    for (int q = 0; q < 8; q++) r[c++] = 0.0;
+    for (int q = DC_FCON; q < DC_N; q++) r[c++] = col[(size_t)q * nz + t];
+  }
+  return HARM_OK;
+}
+
+extern "C" harm_status harm_gdump_columns(harm_handle* h, double* rows)
+{
+  HH_TRY(check_handle(h));
+  if (!rows) return hh::set_error(HARM_EINVAL, "harm_gdump_columns: null argument");
+  const Dims& d = h->d;
+  const long nz = (long)d.n1 * d.n2 * d.n3;
+  for (long t = 0; t < nz; t++) {
+    const int i = (int)(t % d.n1), j = (int)((t / d.n1) % d.n2), k = (int)(t / ((long)d.n1 * d.n2));
+    double* r = rows + t * HARM_GDUMP_NCOL;
+    dump_coords(h, i, j, k, r);
+    int c = 9;
+    const double* cn = &h->hconn[((size_t)(j + d.ng2) * d.ns1 + (i + d.ng1)) * HARM_NCONN];
+    for (int a = 0; a < 4; a++)
+      for (int b = 0; b < 4; b++)
+        for (int e = 0; e < 4; e++) r[c++] = cn[a * 10 + hg_sym(b, e)];
+    const double* g = host_rec(h, HARM_LOC_CENT, i, j);
+    for (int a = 0; a < 4; a++)
+      for (int b = 0; b < 4; b++) r[c++] = g[HARM_G_GCON + hg_sym(a, b)];
+    for (int a = 0; a < 4; a++)
+      for (int b = 0; b < 4; b++) r[c++] = g[HARM_G_GCOV + hg_sym(a, b)];
+    r[c++] = g[HARM_G_GDET];
+    for (int a = 0; a < 4; a++) r[c++] = 0.0;  // conn2: no second connection with gdet in the equations
+    double X[4], V[4], J[4][4];
+    harm_get_coords(h, i, j, k, X, V);
+    hg_mks_dxdxp(&h->cfg, X, J);
+    for (int a = 0; a < 4; a++)
+      for (int b = 0; b < 4; b++) r[c++] = J[a][b];
+  }
+  return HARM_OK;
+}
+
+extern "C" harm_status harm_write_dump(harm_handle* h, int32_t which, const char* path)
+{
+  HH_TRY(check_handle(h));
+  if (!path) return hh::set_error(HARM_EINVAL, "harm_write_dump: null path");
+  const long nz = (long)h->d.n1 * h->d.n2 * h->d.n3;
+  std::vector<double> rows((size_t)nz * HARM_DUMP_NCOL);
+  HH_TRY(harm_dump_columns(h, which, rows.data()));
+  FILE* f = fopen(path, "w");
+  if (!f) return hh::set_error(HARM_EINVAL, "harm_write_dump: cannot open %s", path);
+  dump_header(h, f, 3 /* MAINDUMPTYPE */, HARM_DUMP_NCOL);
+  const harm_status st = write_rows(f, rows.data(), nz, HARM_DUMP_NCOL);
+  fclose(f);
+  return st;
+}
+
+extern "C" harm_status harm_write_gdump(harm_handle* h, const char* path)
+{
+  HH_TRY(check_handle(h));
+  if (!path) return hh::set_error(HARM_EINVAL, "harm_write_gdump: null path");
+  const long nz = (long)h->d.n1 * h->d.n2 * h->d.n3;
+  std::vector<double> rows((size_t)nz * HARM_GDUMP_NCOL);
+  HH_TRY(harm_gdump_columns(h, rows.data()));
+  FILE* f = fopen(path, "w");
+  if (!f) return hh::set_error(HARM_EINVAL, "harm_write_gdump: cannot open %s", path);
+  dump_header(h, f, 4 /* GRIDDUMPTYPE */, HARM_GDUMP_NCOL);
+  const harm_status st = write_rows(f, rows.data(), nz, HARM_GDUMP_NCOL);
+  fclose(f);
+  return st;
 }
 
 extern "C" harm_status harm_host_tables(const harm_config* cfg, double* geom_aos, double* conn_aos,

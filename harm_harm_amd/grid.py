@@ -215,6 +215,29 @@ class Grid:
                 "fladdterms": np.array([r[:] for r in e.fladdterms]),
                 "sourceadd": np.array(e.sourceadd[:]), "divbmax": e.divbmax, "divbavg": e.divbavg}
 
+    # ---- HARM-format dumps (dump.c) ----
+    DUMP_NCOL, GDUMP_NCOL = 72, 126
+
+    def dump_columns(self, which="p"):
+        """the main dump's rows (dump_content): (n1*n2*n3, 72), i fastest"""
+        c = self.cfg
+        out = np.empty((c.n1 * c.n2 * c.n3, self.DUMP_NCOL))
+        check(lib().harm_dump_columns(self.h, self._which(which), out.ctypes.data), "harm_dump_columns")
+        return out
+
+    def gdump_columns(self):
+        """the grid dump's rows (gdump_content): (n1*n2*n3, 126)"""
+        c = self.cfg
+        out = np.empty((c.n1 * c.n2 * c.n3, self.GDUMP_NCOL))
+        check(lib().harm_gdump_columns(self.h, out.ctypes.data), "harm_gdump_columns")
+        return out
+
+    def write_dump(self, path, which="p"):
+        check(lib().harm_write_dump(self.h, self._which(which), str(path).encode()), "harm_write_dump")
+
+    def write_gdump(self, path):
+        check(lib().harm_write_gdump(self.h, str(path).encode()), "harm_write_gdump")
+
     def boundary_fluxes(self):
         out = (ctypes.c_double * 4)()
         check(lib().harm_get_boundary_fluxes(self.h, out), "harm_get_boundary_fluxes")

@@ -271,6 +271,27 @@ typedef struct harm_ener {
  * later harm_step / harm_advance of this handle adds to pdot, pcum, fladd
  * and sourceadd (batch steps do not).  Switching clears the accumulators. */
 harm_status harm_set_ener_tracking(harm_handle* h, int32_t on);
+/* ---- HARM-format dumps (dump.c:256-600, 983-1060; docs/datadesc.txt) ----
+ * Main dump: HARM_DUMP_NCOL columns per active zone, rows with i fastest,
+ * then j, then k: ti tj tk, X1..X3 (MKS), r th ph, the 8 primitives, p,
+ * cs2, Sden, U[8] (the conserved state of the last inversion), |div B|,
+ * u^mu, u_mu, b^mu, b_mu, v1m v1p v2m v2p v3m v3p, gdet, j^mu, j_mu,
+ * F^{mu nu} (01,02,03,12,13,23), F_{mu nu}.  The current columns j are
+ * written as 0: the time history they need is not kept.
+ * Grid dump: HARM_GDUMP_NCOL columns: ti tj tk, X, V, Gamma^a_{bc} (a
+ * slowest), g^{ab}, g_{ab}, gdet, conn2 (0 with gdet in the equations),
+ * dV^a/dX^b. */
+enum { HARM_DUMP_NCOL = 72, HARM_GDUMP_NCOL = 126 };
+/* Replaces dump_content() over the grid: rows[z * HARM_DUMP_NCOL + c] from
+ * primitives `which` (HARM_ARR_*) on the host. */
+harm_status harm_dump_columns(harm_handle* h, int32_t which, double* rows);
+/* Replaces gdump_content() over the grid: rows[z * HARM_GDUMP_NCOL + c]. */
+harm_status harm_gdump_columns(harm_handle* h, double* rows);
+/* Replace dump() and gdump() (text output, dump_header_general's header line,
+ * "%21.15g " per value, one row per zone). */
+harm_status harm_write_dump(harm_handle* h, int32_t which, const char* path);
+harm_status harm_write_gdump(harm_handle* h, const char* path);
+
 /* Clears pdot, pcum, fladd, sourceadd and the source-sample clock. */
 harm_status harm_reset_ener(harm_handle* h);
 /* Replaces the record-forming side of dump_ener(): totals and div B from

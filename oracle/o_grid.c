@@ -1132,6 +1132,74 @@ void o_ener_totals(const o_grid* g, const double* p, double* out)
   out[9] = divbavg;
 }
 
+/* dump_content (dump.c:439-600, CALCFARADAYANDCURRENTS=1) for active zone
+ * (i,j,k): the columns after the coordinates and without the four-current:
+ * p[8], pressure, cs2, Sden (idealgaseos.c:19,48,71), U[8] from ucum, divb,
+ * u^mu, u_mu, b^mu, b_mu (bcon_calc), vmin/vmax per direction (vchar),
+ * gdet, F^{mu nu} (current_precalc CURTYPEFARADAY, phys.c:701-708) and
+ * F_{mu nu} (lowerf, phys.tools.c:1449). */
+void o_dump_row(const o_grid* g, const double* p, const double* ucum, int i, int j, int k, double* out)
+{
+  dims_t d = D(g);
+  const double* pr = p + ix(&d, i, j, k);
+  o_geom G;
+  geom_at(g, &d, 0, i, j, &G);
+  int c = 0;
+  for (int pl = 0; pl < NPR; pl++) out[c++] = pr[pl];
+  const double rho0 = pr[O_RHO], u = pr[O_UU];
+  const double pressure = (g->gam - 1.) * u;
+  out[c++] = pressure;
+  out[c++] = g->gam * pressure / (rho0 + u + pressure);
+  const double indexn = 1.0 / (g->gam - 1.);
+  double entropy;
+  if (rho0 < SMALL && pressure < SMALL) entropy = 0.0;
+  else if (rho0 < SMALL) entropy = 0.0;
+  else if (pressure < SMALL) entropy = -1.e+100;
+  else entropy = rho0 * log(pow(pressure, indexn) / pow(rho0, indexn + 1.0));
+  out[c++] = entropy;
+  for (int pl = 0; pl < NPR; pl++) out[c++] = ucum[ix(&d, i, j, k) + pl];
+  out[c++] = divb_at(g, &d, p, i, j, k);
+  o_state q;
+  o_get_state(pr, &G, g->gam, &q);
+  double bcon[4], bcov[4];
+  bcon[0] = pr[O_B1] * q.ucov[1] + pr[O_B2] * q.ucov[2] + pr[O_B3] * q.ucov[3];
+  for (int m = 1; m < 4; m++) bcon[m] = (pr[O_B1 - 1 + m] + bcon[0] * q.ucon[m]) / q.ucon[0];
+  for (int a = 0; a < 4; a++) {
+    bcov[a] = 0.0;
+    for (int b = 0; b < 4; b++) bcov[a] += G.gcov[a][b] * bcon[b];
+  }
+  for (int m = 0; m < 4; m++) out[c++] = q.ucon[m];
+  for (int m = 0; m < 4; m++) out[c++] = q.ucov[m];
+  for (int m = 0; m < 4; m++) out[c++] = bcon[m];
+  for (int m = 0; m < 4; m++) out[c++] = bcov[m];
+  for (int dir = 1; dir <= 3; dir++) {
+    double vmax, vmin;
+    o_vchar(pr, &q, dir, &G, g->gam, &vmax, &vmin);
+    out[c++] = vmin;
+    out[c++] = vmax;
+  }
+  out[c++] = G.gdet;
+  double f[6];
+  f[0] = -1.0 / G.gdet * (bcov[3] * q.ucov[2] - bcov[2] * q.ucov[3]);
+  f[1] = -1.0 / G.gdet * (-bcov[3] * q.ucov[1] + bcov[1] * q.ucov[3]);
+  f[2] = -1.0 / G.gdet * (bcov[2] * q.ucov[1] - bcov[1] * q.ucov[2]);
+  f[3] = -1.0 / G.gdet * (bcov[3] * q.ucov[0] - bcov[0] * q.ucov[3]);
+  f[4] = -1.0 / G.gdet * (-bcov[2] * q.ucov[0] + bcov[0] * q.ucov[2]);
+  f[5] = -1.0 / G.gdet * (bcov[1] * q.ucov[0] - bcov[0] * q.ucov[1]);
+  double F[4][4] = {{0}};
+  F[0][1] = f[0]; F[0][2] = f[1]; F[0][3] = f[2]; F[1][2] = f[3]; F[1][3] = f[4]; F[2][3] = f[5];
+  F[1][0] = -f[0]; F[2][0] = -f[1]; F[3][0] = -f[2]; F[2][1] = -f[3]; F[3][1] = -f[4]; F[3][2] = -f[5];
+  for (int m = 0; m < 6; m++) out[c++] = f[m];
+  static const int lo[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+  for (int m = 0; m < 6; m++) {
+    const int a = lo[m][0], b = lo[m][1];
+    double sum = 0.0;
+    for (int jp = 0; jp < 4; jp++)
+      for (int kp = 0; kp < 4; kp++) sum += F[jp][kp] * G.gcov[a][jp] * G.gcov[b][kp];
+    out[c++] = sum;
+  }
+}
+
 void o_step(const o_grid* g, double* p, double* pk0, double* pk1, double* ui, double* uf, double* ucum, double* F1,
             double* F2, double* F3, int32_t* pflag, double* dt, double* fluxacc)
 {

@@ -66,6 +66,9 @@ void o_step(const o_grid* g, double* p, double* pk0, double* pk1, double* ui, do
 /* totals over active zones (constotal dump_ener.c:882, divbmaxavg :680):
  * out[0..7] = sum U(p) dV, out[8] = max |div B|, out[9] = sum |div B| */
 void o_ener_totals(const o_grid* g, const double* p, double* out);
+/* dump_content's physics columns for active zone (i,j,k): 55 values, the
+ * dump's columns 10-52 and 61-72 (dump.c:439-600) */
+void o_dump_row(const o_grid* g, const double* p, const double* ucum, int i, int j, int k, double* out);
 /* one cell's (left, right) edge values from y5[0..4] = y[-2..2] (recon 0 donor,
  * 2 PARA, 3 WENO5), and one face's Riemann flux (flux 0 LAX-F, 1 HLL) from
  * two states at a 27-field geometry record; p_l, p_r are modified as

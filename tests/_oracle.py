@@ -148,6 +148,8 @@ def _grid_protos(L):
     L.o_step.argtypes = [P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]
     L.o_ener_totals.argtypes = [P, _P, _P]
     L.o_ener_totals.restype = None
+    L.o_dump_row.argtypes = [P, _P, _P, _I, _I, _I, _P]
+    L.o_dump_row.restype = None
     L.o_rk_tables.argtypes = [_I, _P, _P]
     L.o_rk_tables.restype = _I
     for n in ("o_bound", "o_fluxcalc", "o_update", "o_invert", "o_post_fixup", "o_step"):
@@ -280,6 +282,21 @@ class OGrid:
         return {"U": tot[:8].copy(), "pdot": en[EN_PDOT:EN_PDOT + 48].reshape(6, 8).copy(),
                 "pcum": en[EN_PCUM:EN_PCUM + 48].reshape(6, 8).copy(), "fladd": terms.sum(0), "fladdterms": terms,
                 "sourceadd": en[EN_SRCADD:EN_SRCADD + 8].copy(), "divbmax": tot[8], "divbavg": tot[9] / nz}
+
+    def dump_rows(self, p, ucum):
+        """dump_content's 55 physics columns (o_dump_row) of every active zone,
+        rows in the dump's order (i fastest, then j, then k)"""
+        c = self.cfg
+        out = np.empty((c.n1 * c.n2 * c.n3, 55))
+        row = np.empty(55)
+        t = 0
+        for k in range(c.n3):
+            for j in range(c.n2):
+                for i in range(c.n1):
+                    lib().o_dump_row(ctypes.byref(self.g), p.ctypes.data, ucum.ctypes.data, i, j, k, row.ctypes.data)
+                    out[t] = row
+                    t += 1
+        return out
 
     def new_state(self, p):
         st = {k: self.zeros() for k in ("pk0", "pk1", "ui", "uf", "ucum", "F1", "F2", "F3")}
