@@ -233,8 +233,8 @@ __device__ __forceinline__ void vchar(const double* pr, const State& q, int dir,
   double WW = rho + u + P, cs2;
   if (WW > 0.0) {
     const double pressure = (gam - 1.0) * u;
-    const double h = rho + u + pressure;
-    cs2 = gam * pressure / h;
+    const double h = rho + u + pressure;  // > 0 here: h = WW when u >= 0, else h = rho > -P > 0
+    cs2 = FAST ? gam * pressure * recip_nr(h) : gam * pressure / h;
   } else {
     WW = D_SMALL;
     cs2 = D_SMALL;

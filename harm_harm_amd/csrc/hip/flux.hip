@@ -58,7 +58,8 @@ __device__ __forceinline__ double face_riemann(double* p_l, double* p_r, const G
     for (int pl = 0; pl < HARM_NPR; pl++) Fo[pl] = (0.5 * ((F_l[pl] + F_r[pl]) - ct * (U_r[pl] - U_l[pl])));
   }
   if (do_ct) {
-    const double ign = hd::igdetnosing(G.gdet);
+    // igdetnosing with the reciprocal: |gdet| + SMALL is positive and normal
+    const double ign = (G.gdet >= 0.0 ? 1.0 : -1.0) * hd::recip_nr(fabs(G.gdet) + D_SMALL);
     Fo[HARM_B1] *= ign;
     Fo[HARM_B2] *= ign;
     Fo[HARM_B3] *= ign;
