@@ -179,3 +179,40 @@ def test_seeded_step_matches_plain_step():
           "failures", nf_plain, nf_seed, "fallbacks", fb, "relmax", err)
     assert nf_seed <= nf_plain
     assert err <= 1e-12
+
+
+def test_seeded_step_energy_accounting():
+    """The learned-guess inversion's accounting variant keeps the same fix-up
+    states as the plain one: with tracking on, one step seeded by an
+    (untrained) network and one plain step from the same state give the same
+    energy-file record (floor and gamma-limit sums to 1e-10 of their scale;
+    the seeded and plain Newton roots agree within the inverter's
+    tolerance, so the sums are not bit-identical)."""
+    from harm_harm_amd.grid import Grid
+    cfg = H.default_config(24, 16, 8, recon="weno5", flux="hll", timeorder=3)
+    g = Grid(cfg)
+    g.init_torus(seed=2)
+    ndt = g.fluxcalc("p")
+    g.set_time(0.0, 0.5 / sum(1.0 / x for x in ndt))
+    g.step(2)
+    p0, (t0, dt0), n0 = g.get("p"), g.time(), g.nstep()
+    recs = []
+    for guess in (None, MLP(seed=3)):
+        g.set("p", p0)
+        g.set_time(t0, dt0, n0)
+        g.set_inversion_guess(guess)
+        g.set_ener_tracking(True)
+        g.step(1)
+        recs.append(g.ener())
+        g.set_ener_tracking(False)
+    g.set_inversion_guess(None)
+    a, b = recs
+    for k in ("U", "pdot", "pcum", "sourceadd", "fladd"):
+        s = np.abs(a[k]).max()
+        err = np.abs(a[k] - b[k]).max() / (s if s > 0 else 1.0)
+        print(k, "scale", s, "relative difference", err)
+        assert err <= 1e-10, k
+    for c in (1, 2):  # floors, gamma limit
+        s = np.abs(a["fladdterms"][c]).max()
+        assert np.abs(a["fladdterms"][c] - b["fladdterms"][c]).max() <= 1e-10 * (s if s > 0 else 1.0)
+    assert np.any(a["fladdterms"][1] != 0.0)
