@@ -2081,11 +2081,11 @@ extern "C" harm_status harm_set_ener_tracking(harm_handle* h, int32_t on)
 {
   HH_TRY(check_handle(h));
   HH_HIP(hipStreamSynchronize(h->stream));
-  if (on && !h->fx_p0) {  // the fix-up states, allocated on first use
+  if (on) {  // the fix-up states, allocated on first use (each one that is missing)
     const size_t n = (size_t)HARM_NPR * h->d.ncell;
-    HH_TRY(alloc((void**)&h->fx_p0, sizeof(double) * n));
-    HH_TRY(alloc((void**)&h->fx_pfl, sizeof(double) * n));
-    HH_TRY(alloc((void**)&h->fx_flag, (size_t)h->d.ncell));
+    if (!h->fx_p0) HH_TRY(alloc((void**)&h->fx_p0, sizeof(double) * n));
+    if (!h->fx_pfl) HH_TRY(alloc((void**)&h->fx_pfl, sizeof(double) * n));
+    if (!h->fx_flag) HH_TRY(alloc((void**)&h->fx_flag, (size_t)h->d.ncell));
   }
   if (h->ener_on != (on != 0)) drop_step_graph(h);  // the captured step holds the other kernel set
   h->ener_on = on != 0;
@@ -2152,10 +2152,12 @@ const double* host_rec(const harm_handle* h, int loc, int i, int j)
 }
 
 // dump_header_general (dump.c:321-431), text form
-void dump_header(harm_handle* h, FILE* f, int whichdump, int ncol)
+harm_status dump_header(harm_handle* h, FILE* f, int whichdump, int ncol)
 {
   double v[DS_N];
-  (void)hipMemcpy(v, h->ds, sizeof(v), hipMemcpyDeviceToHost);
+  // the clock after the work already on the handle's (non-blocking) stream
+  HH_HIP(hipStreamSynchronize(h->stream));
+  HH_HIP(hipMemcpy(v, h->ds, sizeof(v), hipMemcpyDeviceToHost));
   const harm_config& c = h->cfg;
   fprintf(f,
           "%21.15g %d %d %d %21.15g %21.15g %21.15g %21.15g %21.15g %21.15g %ld %21.15g %21.15g %21.15g %21.15g "
@@ -2163,6 +2165,7 @@ void dump_header(harm_handle* h, FILE* f, int whichdump, int ncol)
           v[DS_T], c.n1, c.n2, c.n3, h->P.startx[1], h->P.startx[2], h->P.startx[3], h->P.dx[1], h->P.dx[2],
           h->P.dx[3], (long)v[DS_NSTEP], c.gam, c.a, c.R0, c.Rin, c.Rout, c.hslope, v[DS_DT], 0 /* LOGRSINTH */,
           1.0 /* MBH */, 0.0 /* QBH */, 0, c.n1 - 1, 0, c.n2 - 1, 0, c.n3 - 1, whichdump, 0, ncol);
+  return HARM_OK;
 }
 
 harm_status write_rows(FILE* f, const double* rows, long nz, int ncol)
@@ -2247,8 +2250,8 @@ extern "C" harm_status harm_write_dump(harm_handle* h, int32_t which, const char
   HH_TRY(harm_dump_columns(h, which, rows.data()));
   FILE* f = fopen(path, "w");
   if (!f) return hh::set_error(HARM_EINVAL, "harm_write_dump: cannot open %s", path);
-  dump_header(h, f, 3 /* MAINDUMPTYPE */, HARM_DUMP_NCOL);
-  const harm_status st = write_rows(f, rows.data(), nz, HARM_DUMP_NCOL);
+  harm_status st = dump_header(h, f, 3 /* MAINDUMPTYPE */, HARM_DUMP_NCOL);
+  if (st == HARM_OK) st = write_rows(f, rows.data(), nz, HARM_DUMP_NCOL);
   fclose(f);
   return st;
 }
@@ -2262,8 +2265,8 @@ extern "C" harm_status harm_write_gdump(harm_handle* h, const char* path)
   HH_TRY(harm_gdump_columns(h, rows.data()));
   FILE* f = fopen(path, "w");
   if (!f) return hh::set_error(HARM_EINVAL, "harm_write_gdump: cannot open %s", path);
-  dump_header(h, f, 4 /* GRIDDUMPTYPE */, HARM_GDUMP_NCOL);
-  const harm_status st = write_rows(f, rows.data(), nz, HARM_GDUMP_NCOL);
+  harm_status st = dump_header(h, f, 4 /* GRIDDUMPTYPE */, HARM_GDUMP_NCOL);
+  if (st == HARM_OK) st = write_rows(f, rows.data(), nz, HARM_GDUMP_NCOL);
   fclose(f);
   return st;
 }
