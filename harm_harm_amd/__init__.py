@@ -122,6 +122,8 @@ _OPTIONAL_PROTOS = [
     ("harm_gdump_columns", _I, [_P, _P]),
     ("harm_write_dump", _I, [_P, _I, ctypes.c_char_p]),
     ("harm_write_gdump", _I, [_P, ctypes.c_char_p]),
+    ("harm_write_restart", _I, [_P, ctypes.c_char_p, _I, ctypes.c_double]),
+    ("harm_read_restart", _I, [_P, ctypes.c_char_p, _P]),
     ("harm_get_stream", _I, [_P, ctypes.POINTER(_P)]),
     ("harm_profile", _I, [_P, _I]),
     ("harm_profile_read", _I, [_P, _P, _P]),

@@ -2271,6 +2271,147 @@ extern "C" harm_status harm_write_gdump(harm_handle* h, const char* path)
   return st;
 }
 
+// ---------------------------------------------------------------------------
+// Restart files (restart.c)
+// ---------------------------------------------------------------------------
+namespace {
+// lim[] codes of global.nondepnmemonics.h:527-541 for HARM_RECON_*
+const int kLimCode[4] = {0 /* DONOR */, 3 /* MC */, 4 /* PARA */, 10 /* WENO5 */};
+// pdot, pcum, fladdterms, sourceadd, the source-sample clock (en[0..EN_WILLBE])
+constexpr int kRestartEn = EN_WILLBE + 1;
+static_assert(EN_PCUM == 48 && EN_FLTERMS == 96 && EN_SRCADD == 192 && EN_DT4 == 200, "restart block order");
+
+struct RestartHead {
+  int n[3], lim, timeorder, fluxmethod, defcoord, binary;
+  long nstep;
+  double t, tf, a, MBH, QBH, gam, gamideal, cour, dt, R0, Rin, Rout, hslope;
+  double en[kRestartEn];
+};
+
+// the active zones of `which` in rdump order (i fastest), `per` values apart
+void active_rows(const harm_handle* h, const std::vector<double>& aos, double* out, int per, int off)
+{
+  const Dims& d = h->d;
+  const long nz = (long)d.n1 * d.n2 * d.n3;
+  for (long t = 0; t < nz; t++) {
+    const long i = t % d.n1, j = (t / d.n1) % d.n2, k = t / ((long)d.n1 * d.n2);
+    const long c = ((k + d.ng3) * d.ns2 + (j + d.ng2)) * d.ns1 + (i + d.ng1);
+    for (int pl = 0; pl < HARM_NPR; pl++) out[t * per + off + pl] = aos[c * HARM_NPR + pl];
+  }
+}
+}  // namespace
+
+extern "C" harm_status harm_write_restart(harm_handle* h, const char* path, int32_t binary, double tf)
+{
+  HH_TRY(check_handle(h));
+  if (!path) return hh::set_error(HARM_EINVAL, "harm_write_restart: null path");
+  if (binary != HARM_TEXTOUTPUT && binary != HARM_MIXEDOUTPUT)
+    return hh::set_error(HARM_EINVAL, "harm_write_restart: binary must be 0 (text) or 2 (mixed), got %d", binary);
+  const Dims& d = h->d;
+  const long nl = (long)d.ns1 * d.ns2 * d.ns3, nz = (long)d.n1 * d.n2 * d.n3;
+  std::vector<double> aos((size_t)nl * HARM_NPR), rows((size_t)nz * 2 * HARM_NPR);
+  HH_TRY(harm_get_array(h, HARM_ARR_P, aos.data()));
+  active_rows(h, aos, rows.data(), 2 * HARM_NPR, 0);
+  HH_TRY(harm_get_array(h, HARM_ARR_UCUM, aos.data()));
+  active_rows(h, aos, rows.data(), 2 * HARM_NPR, HARM_NPR);
+  double v[DS_N], en[EN_N];
+  HH_HIP(hipMemcpy(v, h->ds, sizeof(v), hipMemcpyDeviceToHost));
+  HH_HIP(hipMemcpy(en, h->en, sizeof(en), hipMemcpyDeviceToHost));
+  const harm_config& c = h->cfg;
+  FILE* f = fopen(path, binary ? "wb" : "w");
+  if (!f) return hh::set_error(HARM_EINVAL, "harm_write_restart: cannot open %s", path);
+  const int lim = kLimCode[c.recon];
+  fprintf(f, "%d %d %d %31.26g %31.26g %ld %31.26g %31.26g %31.26g %26.20g %26.20g %26.20g %31.26g %d %d %d %d %d ",
+          c.n1, c.n2, c.n3, v[DS_T], tf, (long)v[DS_NSTEP], c.a, 1.0 /* MBH */, 0.0 /* QBH */, c.gam, c.gam, c.cour,
+          v[DS_DT], lim, lim, lim, c.timeorder, c.flux);
+  fprintf(f, "%d %26.20g %26.20g %26.20g %26.20g %d ", 0 /* defcoord LOGRSINTH */, c.R0, c.Rin, c.Rout, c.hslope,
+          binary);
+  for (int q = 0; q < kRestartEn; q++) fprintf(f, "%31.26g ", en[q]);
+  fprintf(f, "\n");
+  if (binary) {
+    fwrite(rows.data(), sizeof(double), rows.size(), f);
+  } else {
+    for (long z = 0; z < nz; z++) {
+      for (int q = 0; q < 2 * HARM_NPR; q++) fprintf(f, "%21.15g ", rows[z * 2 * HARM_NPR + q]);
+      fprintf(f, "\n");
+    }
+  }
+  const bool bad = ferror(f) != 0;
+  fclose(f);
+  return bad ? hh::set_error(HARM_EINVAL, "harm_write_restart: write failed on %s", path) : HARM_OK;
+}
+
+extern "C" harm_status harm_read_restart(harm_handle* h, const char* path, double* tf)
+{
+  HH_TRY(check_handle(h));
+  if (!path) return hh::set_error(HARM_EINVAL, "harm_read_restart: null path");
+  FILE* f = fopen(path, "rb");
+  if (!f) return hh::set_error(HARM_EINVAL, "harm_read_restart: cannot open %s", path);
+  RestartHead r;
+  int lim2, lim3;
+  int got = fscanf(f, "%d %d %d %lf %lf %ld %lf %lf %lf %lf %lf %lf %lf %d %d %d %d %d %d %lf %lf %lf %lf %d", &r.n[0],
+                   &r.n[1], &r.n[2], &r.t, &r.tf, &r.nstep, &r.a, &r.MBH, &r.QBH, &r.gam, &r.gamideal, &r.cour, &r.dt,
+                   &r.lim, &lim2, &lim3, &r.timeorder, &r.fluxmethod, &r.defcoord, &r.R0, &r.Rin, &r.Rout, &r.hslope,
+                   &r.binary);
+  int q = 0;
+  while (got == 24 && q < kRestartEn && fscanf(f, "%lf", &r.en[q]) == 1) q++;
+  got += q;
+  int ch;
+  while (got == 24 + kRestartEn && (ch = fgetc(f)) != EOF && ch != '\n') {
+  }
+  if (got != 24 + kRestartEn) {
+    fclose(f);
+    return hh::set_error(HARM_EINVAL, "harm_read_restart: %s: header has %d of %d fields", path, got,
+                         24 + kRestartEn);
+  }
+  const harm_config& c = h->cfg;
+  const char* bad = nullptr;
+  if (r.n[0] != c.n1) bad = "N1";
+  else if (r.n[1] != c.n2) bad = "N2";
+  else if (r.n[2] != c.n3) bad = "N3";
+  else if (r.a != c.a) bad = "a";
+  else if (r.gam != c.gam) bad = "gam";
+  else if (r.lim != kLimCode[c.recon] || lim2 != r.lim || lim3 != r.lim) bad = "lim";
+  else if (r.timeorder != c.timeorder) bad = "TIMEORDER";
+  else if (r.fluxmethod != c.flux) bad = "fluxmethod";
+  else if (r.defcoord != 0) bad = "defcoord";
+  else if (r.R0 != c.R0 || r.Rin != c.Rin || r.Rout != c.Rout || r.hslope != c.hslope) bad = "grid map";
+  else if (r.binary != HARM_TEXTOUTPUT && r.binary != HARM_MIXEDOUTPUT) bad = "binaryoutput";
+  if (bad) {
+    fclose(f);
+    return hh::set_error(HARM_EINVAL, "harm_read_restart: %s: %s differs from the handle's", path, bad);
+  }
+  const Dims& d = h->d;
+  const long nl = (long)d.ns1 * d.ns2 * d.ns3, nz = (long)d.n1 * d.n2 * d.n3;
+  std::vector<double> rows((size_t)nz * 2 * HARM_NPR);
+  size_t nread = 0;
+  if (r.binary) {
+    nread = fread(rows.data(), sizeof(double), rows.size(), f);
+  } else {
+    while (nread < rows.size() && fscanf(f, "%lf", &rows[nread]) == 1) nread++;
+  }
+  fclose(f);
+  if (nread != rows.size())
+    return hh::set_error(HARM_EINVAL, "harm_read_restart: %s: %zu of %zu values", path, nread, rows.size());
+  std::vector<double> aos((size_t)nl * HARM_NPR, 0.0);
+  for (int half = 0; half < 2; half++) {
+    for (long t = 0; t < nz; t++) {
+      const long i = t % d.n1, j = (t / d.n1) % d.n2, k = t / ((long)d.n1 * d.n2);
+      const long cc = ((k + d.ng3) * d.ns2 + (j + d.ng2)) * d.ns1 + (i + d.ng1);
+      for (int pl = 0; pl < HARM_NPR; pl++) aos[cc * HARM_NPR + pl] = rows[t * 2 * HARM_NPR + half * HARM_NPR + pl];
+    }
+    HH_TRY(harm_set_array(h, half ? HARM_ARR_UCUM : HARM_ARR_P, aos.data()));
+  }
+  HH_TRY(enq_bound(h, HARM_ARR_P));
+  HH_TRY(hh::check_launch("harm_read_restart: bound"));
+  HH_TRY(harm_set_time(h, r.t, r.dt, r.nstep));
+  double en[EN_N] = {0.0};
+  for (int q = 0; q < kRestartEn; q++) en[q] = r.en[q];
+  HH_HIP(hipMemcpy(h->en, en, sizeof(en), hipMemcpyHostToDevice));
+  if (tf) *tf = r.tf;
+  return HARM_OK;
+}
+
 extern "C" harm_status harm_host_tables(const harm_config* cfg, double* geom_aos, double* conn_aos,
                                         double* corner_gdet, double* rcent, double startx[4], double dx[4])
 {

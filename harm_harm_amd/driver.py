@@ -15,8 +15,14 @@ Mirrors, on the host, what main.c does around step_ch_full():
 Setting the clipped dt goes through harm_set_time, which also restarts the
 Newton sums (harm_get_newton_stats' totals) at the last steps.
 
+* restart files follow the step-count rule (diag.c:95-136, :574-577,
+  post_dump :511-523): every DTr steps (and at the final call) the next of
+  dumps/rdump-0 and dumps/rdump-1 is written, and every main dump also
+  writes dumps/rdump--NNNN with its number (diag.c:319-328).
+
 Files, as HARM names them: dumps/gdump once at the start, dumps/dumpNNNN
-(main dump) and ener.out (one energy-file line per ener output).
+(main dump), the restart files above and ener.out (one energy-file line
+per ener output).
 """
 import math
 import os
@@ -50,13 +56,41 @@ class Cadence:
         self.tlast = t
 
 
-def run(grid, tf, DTdump=None, DTener=None, outdir=".", gdump=True, max_steps=None, which="p"):
+class RestartCadence:
+    """Restart dumps by step count: nrestart, nlastrestart, whichrestart."""
+
+    def __init__(self, DTr, n0):
+        self.DTr = DTr
+        self.which = 0
+        self.count = 0
+        if DTr is not None:
+            self.nlast = DTr * (n0 // DTr) - 1
+            self.nnext = DTr * (n0 // DTr + 1)
+
+    def due(self, n, call):
+        if self.DTr is None:
+            return False
+        return (self.nlast != self.nnext and n >= self.nnext) or call == "final"
+
+    def done(self, n):
+        self.which = 1 - self.which
+        c = 1 + max(0, int((n - self.nnext) / self.DTr))
+        self.nnext = (math.floor(n / self.DTr + 0.5) + c) * self.DTr
+        self.nlast = n
+        self.count += 1
+
+
+def run(grid, tf, DTdump=None, DTener=None, outdir=".", gdump=True, max_steps=None, which="p", DTr=None,
+        restart_binary=True):
     """Advance `grid` (harm_harm_amd.grid.Grid, clock already set) to tf with
-    HARM's end-of-run rule, writing dumps every DTdump and energy-file lines
-    every DTener into outdir.  Returns a summary dict."""
+    HARM's end-of-run rule, writing dumps every DTdump, energy-file lines
+    every DTener and restart files every DTr steps into outdir (restart
+    files MIXEDOUTPUT when restart_binary, else TEXTOUTPUT).  Returns a
+    summary dict."""
     os.makedirs(os.path.join(outdir, "dumps"), exist_ok=True)
     t0, _ = grid.time()
     dump_c, ener_c = Cadence(DTdump, t0), Cadence(DTener, t0)
+    rdump_c = RestartCadence(DTr, grid.nstep())
     if DTener is not None:
         grid.set_ener_tracking(True)
     ener_path = os.path.join(outdir, "ener.out")
@@ -69,7 +103,13 @@ def run(grid, tf, DTdump=None, DTener=None, outdir=".", gdump=True, max_steps=No
         t, _ = grid.time()
         if dump_c.due(t, call):
             grid.write_dump(os.path.join(outdir, "dumps", "dump%04d" % dump_c.count), which)
+            if DTr is not None:
+                grid.write_restart(os.path.join(outdir, "dumps", "rdump--%04d" % dump_c.count), restart_binary, tf)
             dump_c.done(t)
+        n = grid.nstep()
+        if rdump_c.due(n, call):
+            grid.write_restart(os.path.join(outdir, "dumps", "rdump-%01d" % rdump_c.which), restart_binary, tf)
+            rdump_c.done(n)
         if ener_c.due(t, call):
             with open(ener_path, "a") as f:
                 f.write(ener_line(grid.ener(which)))
@@ -97,4 +137,4 @@ def run(grid, tf, DTdump=None, DTener=None, outdir=".", gdump=True, max_steps=No
             break
     diag("final")
     t, _ = grid.time()
-    return {"t": t, "steps": steps, "dumps": dump_c.count, "ener_lines": ener_c.count}
+    return {"t": t, "steps": steps, "dumps": dump_c.count, "ener_lines": ener_c.count, "rdumps": rdump_c.count}

@@ -238,6 +238,19 @@ class Grid:
     def write_gdump(self, path):
         check(lib().harm_write_gdump(self.h, str(path).encode()), "harm_write_gdump")
 
+    # ---- restart files (restart.c) ----
+    def write_restart(self, path, binary=True, tf=0.0):
+        """rdump: binary=True writes MIXEDOUTPUT (text header, raw doubles; an
+        exact restart), False TEXTOUTPUT (15 digits, as the reference)."""
+        check(lib().harm_write_restart(self.h, str(path).encode(), 2 if binary else 0, float(tf)),
+              "harm_write_restart")
+
+    def read_restart(self, path):
+        """Loads an rdump of the same grid and configuration; returns its tf."""
+        tf = ctypes.c_double()
+        check(lib().harm_read_restart(self.h, str(path).encode(), ctypes.byref(tf)), "harm_read_restart")
+        return tf.value
+
     def boundary_fluxes(self):
         out = (ctypes.c_double * 4)()
         check(lib().harm_get_boundary_fluxes(self.h, out), "harm_get_boundary_fluxes")

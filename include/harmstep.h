@@ -292,6 +292,29 @@ harm_status harm_gdump_columns(harm_handle* h, double* rows);
 harm_status harm_write_dump(harm_handle* h, int32_t which, const char* path);
 harm_status harm_write_gdump(harm_handle* h, const char* path);
 
+/* ---- restart files (restart.c:163-285, readwrite_restart_header :593-850) ----
+ * One text header line holding, in readwrite_restart_header's order, the
+ * part of its list that this library has state for: N1 N2 N3, t, tf, nstep,
+ * a, MBH, QBH, gam, gamideal, cour, dt, lim[1..3], TIMEORDER, fluxmethod,
+ * defcoord, R0, Rin, Rout, hslope, binaryoutput, then the energy-file
+ * accumulators (pdot[6][8], pcum[6][8], fladdterms[12][8], sourceadd[8], the
+ * source-sample clock's two sums).  Doubles print as "%26.20g" (FTYPE) or
+ * "%31.26g" (SFTYPE), which read back exactly.  Then per active zone, i
+ * fastest: the 8 primitives of HARM_ARR_P and the 8 conserved of
+ * HARM_ARR_UCUM (rdump_content).  binary = 0 (TEXTOUTPUT) writes each
+ * value as "%21.15g ", one zone per line, as the reference does: 15 digits,
+ * not an exact restart.  binary = 2 (MIXEDOUTPUT) writes raw native doubles:
+ * stepping after harm_read_restart then gives the same bits as stepping on. */
+enum { HARM_TEXTOUTPUT = 0, HARM_MIXEDOUTPUT = 2 };
+harm_status harm_write_restart(harm_handle* h, const char* path, int32_t binary, double tf);
+/* Replaces restart_read(): the grid sizes and the configuration in the
+ * header must equal the handle's (HARM_EINVAL naming the field otherwise).
+ * Loads the active zones, fills the ghost zones with the problem's boundary
+ * conditions, sets t, dt and nstep and restores the energy accumulators
+ * (switch tracking on before the call: switching clears them).  tf may be
+ * NULL. */
+harm_status harm_read_restart(harm_handle* h, const char* path, double* tf);
+
 /* Clears pdot, pcum, fladd, sourceadd and the source-sample clock. */
 harm_status harm_reset_ener(harm_handle* h);
 /* Replaces the record-forming side of dump_ener(): totals and div B from

@@ -34,3 +34,29 @@ def test_run_to_tf_with_outputs(tmp_path):
     th = [float(open(d).readline().split()[0]) for d in dumps]
     assert th == sorted(th) and th[-1] == pytest.approx(tf, rel=1e-14)
 
+
+def test_run_writes_restart_files_that_resume(tmp_path):
+    """DTr = 4 steps over 11 steps: rdump-0 at step 4, rdump-1 at 8, rdump-0
+    again at 11 (the final call), and rdump--NNNN beside each main dump; a
+    grid read from the last one stands where the run ended."""
+    cfg = H.default_config(32, 32, 1, recon="para", flux="laxf", timeorder=2)
+    g = Grid(cfg)
+    g.init_torus()
+    g.set_time(0.0, 0.5 * min(g.fluxcalc("p")))
+    out = driver.run(g, 1e9, DTdump=None, DTener=None, outdir=str(tmp_path), max_steps=11, DTr=4)
+    assert out["steps"] == 11 and out["rdumps"] == 3
+    d = tmp_path / "dumps"
+    assert (d / "rdump-0").exists() and (d / "rdump-1").exists()
+    assert int(open(d / "rdump-1", "rb").readline().split()[5]) == 8
+    assert int(open(d / "rdump-0", "rb").readline().split()[5]) == 11
+    b = Grid(cfg)
+    assert b.read_restart(d / "rdump-0") == 1e9
+    assert np.array_equal(b.get("p"), g.get("p")) and b.time() == g.time()
+    # a main dump at t = 0 and after each of two steps (the final call repeats
+    # no time), each with its numbered restart file
+    g = Grid(cfg)
+    g.init_torus()
+    g.set_time(0.0, 0.5 * min(g.fluxcalc("p")))
+    driver.run(g, 1e9, DTdump=1e-9, outdir=str(tmp_path / "m"), gdump=False, max_steps=2, DTr=100)
+    names = sorted(x.name for x in (tmp_path / "m" / "dumps").glob("rdump--*"))
+    assert names == ["rdump--0000", "rdump--0001", "rdump--0002"]
