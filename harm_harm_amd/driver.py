@@ -33,19 +33,30 @@ SMALL = 1.0e-300  # SMALL for doubles (global.depnmemonics.h)
 
 
 class Cadence:
-    """One output type's schedule: tdumpgen, DTdumpgen, dumpcntgen, tlastgen."""
+    """One output type's schedule: tdumpgen, DTdumpgen, dumpcntgen, tlastgen.
 
-    def __init__(self, DT, t0):
+    A fresh run writes at t0 (RESTARTMODE 0, diag.c:111-113); a resumed run
+    writes next at the first multiple of DT after t0 (diag.c:137-139) and
+    numbers its files from `count` (dumpcntgen as the restart file kept it).
+    at_init: the type is also written by the initial call whatever its next
+    time (the energy file, diag.c:683; main dumps only when due, :586)."""
+
+    def __init__(self, DT, t0, resume=False, count=0, at_init=False):
         self.DT = DT
-        self.tnext = t0
-        self.count = 0
+        self.count = count
+        self.at_init = at_init
         self.tlast = None
+        if DT is not None and resume:
+            self.tnext = DT * (int(t0 / DT) + 1)
+            self.tlast = DT * int(t0 / DT) - SMALL
+        else:
+            self.tnext = t0
 
     def due(self, t, call):
         """call: "init", "step" or "final" (INIT_OUT, DUMP_OUT, FINAL_OUT)"""
         if self.DT is None or t == self.tlast:
             return False
-        return t >= self.tnext or call in ("init", "final")
+        return t >= self.tnext or call == "final" or (call == "init" and self.at_init)
 
     def done(self, t):
         # post_dump: dumpc = 1 + max(0, (long long)((t - tdump)/DT));
@@ -57,15 +68,19 @@ class Cadence:
 
 
 class RestartCadence:
-    """Restart dumps by step count: nrestart, nlastrestart, whichrestart."""
+    """Restart dumps by step count: nrestart, nlastrestart, whichrestart.
 
-    def __init__(self, DTr, n0):
+    nlastrestart starts at DTr (n0 / DTr) - 1 (diag.c:98); a fresh run has
+    nrestart = n0, so it writes rdump-0 at once (:111), a resumed one the next
+    multiple of DTr (:134) and continues the 0/1 alternation from `which`."""
+
+    def __init__(self, DTr, n0, resume=False, which=0):
         self.DTr = DTr
-        self.which = 0
+        self.which = which
         self.count = 0
         if DTr is not None:
             self.nlast = DTr * (n0 // DTr) - 1
-            self.nnext = DTr * (n0 // DTr + 1)
+            self.nnext = DTr * (n0 // DTr + 1) if resume else n0
 
     def due(self, n, call):
         if self.DTr is None:
@@ -80,23 +95,48 @@ class RestartCadence:
         self.count += 1
 
 
+def _resume_counters(outdir):
+    """what a resumed run continues from in outdir/dumps: the number of the
+    next main dump (one past the highest dumpNNNN) and the next restart slot
+    (the other one than the rdump-0/1 with the later step count)"""
+    d = os.path.join(outdir, "dumps")
+    nums = [int(n[4:]) for n in os.listdir(d) if n.startswith("dump") and n[4:].isdigit()] if os.path.isdir(d) else []
+    steps = {}
+    for w in (0, 1):
+        f = os.path.join(d, "rdump-%d" % w)
+        if os.path.exists(f):
+            with open(f, "rb") as fh:
+                steps[w] = int(fh.readline().split()[5])
+    which = 0 if not steps else 1 - max(steps, key=lambda w: steps[w])
+    return (max(nums) + 1 if nums else 0), which
+
+
 def run(grid, tf, DTdump=None, DTener=None, outdir=".", gdump=True, max_steps=None, which="p", DTr=None,
-        restart_binary=True):
+        restart_binary=True, resume=False):
     """Advance `grid` (harm_harm_amd.grid.Grid, clock already set) to tf with
     HARM's end-of-run rule, writing dumps every DTdump, energy-file lines
     every DTener and restart files every DTr steps into outdir (restart
-    files MIXEDOUTPUT when restart_binary, else TEXTOUTPUT).  Returns a
-    summary dict."""
+    files MIXEDOUTPUT when restart_binary, else TEXTOUTPUT).
+
+    resume: the grid continues a run of the same outdir (RESTARTMODE 1,
+    after harm_read_restart with tracking switched on): ener.out is appended
+    to and the energy sums the restart file restored go on accumulating, the
+    next output times are the first multiples of DT after t (no initial
+    outputs), main dumps are numbered on from the highest existing one, and
+    restart files continue the rdump-0/1 alternation.  Returns a summary
+    dict."""
     os.makedirs(os.path.join(outdir, "dumps"), exist_ok=True)
     t0, _ = grid.time()
-    dump_c, ener_c = Cadence(DTdump, t0), Cadence(DTener, t0)
-    rdump_c = RestartCadence(DTr, grid.nstep())
+    first_dump, first_which = _resume_counters(outdir) if resume else (0, 0)
+    dump_c = Cadence(DTdump, t0, resume, first_dump)
+    ener_c = Cadence(DTener, t0, resume, at_init=True)
+    rdump_c = RestartCadence(DTr, grid.nstep(), resume, first_which)
     if DTener is not None:
-        grid.set_ener_tracking(True)
+        grid.set_ener_tracking(True)  # keeps the restored sums when tracking is on already
     ener_path = os.path.join(outdir, "ener.out")
-    if DTener is not None:
+    if DTener is not None and not resume:
         open(ener_path, "w").close()
-    if gdump:
+    if gdump and not resume:
         grid.write_gdump(os.path.join(outdir, "dumps", "gdump"))
 
     def diag(call):
@@ -137,4 +177,5 @@ def run(grid, tf, DTdump=None, DTener=None, outdir=".", gdump=True, max_steps=No
             break
     diag("final")
     t, _ = grid.time()
-    return {"t": t, "steps": steps, "dumps": dump_c.count, "ener_lines": ener_c.count, "rdumps": rdump_c.count}
+    return {"t": t, "steps": steps, "dumps": dump_c.count - first_dump, "ener_lines": ener_c.count,
+            "rdumps": rdump_c.count}

@@ -16,16 +16,31 @@ def test_cadence_rule():
 
 def test_restart_cadence_rule():
     """Every DTr steps the next of rdump-0 / rdump-1, and always at the final
-    call (diag.c:95-136, :574-577, post_dump :511-523)."""
+    call (diag.c:95-136, :574-577, post_dump :511-523); a fresh run writes
+    rdump-0 at its first call (nrestart = nstep, :111)."""
     c = driver.RestartCadence(3, 0)
     written = []
     for n in range(0, 8):
-        if c.due(n, "step"):
+        if c.due(n, "init" if n == 0 else "step"):
             written.append((n, c.which))
             c.done(n)
-    assert written == [(3, 0), (6, 1)]
-    assert c.due(7, "final") and c.which == 0
-    # started from a restart at step 5: the next is at 6, not 5
-    c = driver.RestartCadence(3, 5)
-    assert not c.due(5, "init") and c.due(6, "step")
+    assert written == [(0, 0), (3, 1), (6, 0)]
+    assert c.due(7, "final") and c.which == 1
+    # resumed at step 5 after rdump-0: the next is at 6 (not 5), into rdump-1
+    c = driver.RestartCadence(3, 5, resume=True, which=1)
+    assert not c.due(5, "init") and c.due(6, "step") and c.which == 1
     assert not driver.RestartCadence(None, 0).due(10, "final")
+
+
+def test_resumed_cadence():
+    """A resumed run writes its next main dump at the first multiple of DT
+    after t0, not at t0 (diag.c:137-139, :586), and numbers it on; the energy
+    file also writes at the initial call (:683)."""
+    c = driver.Cadence(1.0, 2.5, resume=True, count=3)
+    assert c.tnext == 3.0 and not c.due(2.5, "init") and c.due(3.0, "step")
+    c.done(3.0)
+    assert c.count == 4 and c.tnext == 4.0
+    e = driver.Cadence(1.0, 2.5, resume=True, at_init=True)
+    assert e.due(2.5, "init")
+    f = driver.Cadence(1.0, 0.0)
+    assert f.due(0.0, "init")

@@ -36,21 +36,22 @@ def test_run_to_tf_with_outputs(tmp_path):
 
 
 def test_run_writes_restart_files_that_resume(tmp_path):
-    """DTr = 4 steps over 11 steps: rdump-0 at step 4, rdump-1 at 8, rdump-0
-    again at 11 (the final call), and rdump--NNNN beside each main dump; a
-    grid read from the last one stands where the run ended."""
+    """DTr = 4 steps over 11 steps of a fresh run: rdump-0 at step 0, rdump-1
+    at 4, rdump-0 at 8, rdump-1 again at 11 (the final call), and
+    rdump--NNNN beside each main dump; a grid read from the last one stands
+    where the run ended."""
     cfg = H.default_config(32, 32, 1, recon="para", flux="laxf", timeorder=2)
     g = Grid(cfg)
     g.init_torus()
     g.set_time(0.0, 0.5 * min(g.fluxcalc("p")))
     out = driver.run(g, 1e9, DTdump=None, DTener=None, outdir=str(tmp_path), max_steps=11, DTr=4)
-    assert out["steps"] == 11 and out["rdumps"] == 3
+    assert out["steps"] == 11 and out["rdumps"] == 4
     d = tmp_path / "dumps"
     assert (d / "rdump-0").exists() and (d / "rdump-1").exists()
-    assert int(open(d / "rdump-1", "rb").readline().split()[5]) == 8
-    assert int(open(d / "rdump-0", "rb").readline().split()[5]) == 11
+    assert int(open(d / "rdump-0", "rb").readline().split()[5]) == 8
+    assert int(open(d / "rdump-1", "rb").readline().split()[5]) == 11
     b = Grid(cfg)
-    assert b.read_restart(d / "rdump-0") == 1e9
+    assert b.read_restart(d / "rdump-1") == 1e9
     assert np.array_equal(b.get("p"), g.get("p")) and b.time() == g.time()
     # a main dump at t = 0 and after each of two steps (the final call repeats
     # no time), each with its numbered restart file
@@ -60,3 +61,49 @@ def test_run_writes_restart_files_that_resume(tmp_path):
     driver.run(g, 1e9, DTdump=1e-9, outdir=str(tmp_path / "m"), gdump=False, max_steps=2, DTr=100)
     names = sorted(x.name for x in (tmp_path / "m" / "dumps").glob("rdump--*"))
     assert names == ["rdump--0000", "rdump--0001", "rdump--0002"]
+
+
+def test_resumed_run_continues_the_outputs(tmp_path):
+    """A run of 6 steps, read back from its last restart file and resumed for
+    5 more, leaves the same ener.out lines and the same final state and
+    energy sums as one run of 11 steps: the restored sums keep accumulating,
+    ener.out is appended to (its line at the resume time repeated, as HARM's
+    initial call writes one), main dumps are numbered on."""
+    cfg = H.default_config(32, 32, 1, recon="para", flux="laxf", timeorder=2)
+
+    def fresh():
+        g = Grid(cfg)
+        g.init_torus()
+        g.set_time(0.0, 0.5 * min(g.fluxcalc("p")))
+        return g
+
+    dt0 = fresh().time()[1]
+    kw = dict(DTdump=2.5 * dt0, DTener=dt0, gdump=False, DTr=3)
+    one = fresh()
+    driver.run(one, 1e9, outdir=str(tmp_path / "one"), max_steps=11, **kw)
+    a = fresh()
+    out_a = driver.run(a, 1e9, outdir=str(tmp_path / "two"), max_steps=6, **kw)
+    d = tmp_path / "two" / "dumps"
+    last = max((d / "rdump-0", d / "rdump-1"), key=lambda f: int(open(f, "rb").readline().split()[5]))
+    b = Grid(cfg)
+    b.set_ener_tracking(True)
+    b.read_restart(last)
+    assert b.nstep() == 6
+    out_b = driver.run(b, 1e9, outdir=str(tmp_path / "two"), max_steps=5, resume=True, **kw)
+    assert np.array_equal(b.get("p"), one.get("p")) and b.time() == one.time()
+    e1, e2 = one.ener(), b.ener()
+    for k in ("pcum", "fladdterms", "sourceadd"):
+        assert np.array_equal(e1[k], e2[k]), k
+    l1 = (tmp_path / "one" / "ener.out").read_text().splitlines()
+    l2 = (tmp_path / "two" / "ener.out").read_text().splitlines()
+    # the resumed run's initial call repeats the line at the resume time
+    t2 = [float(x.split()[0]) for x in l2]
+    dup = [i for i in range(1, len(t2)) if t2[i] == t2[i - 1]]
+    assert len(dup) == 1
+    del l2[dup[0]]
+    assert l1 == l2
+    # the main dumps of both parts, numbered on without overwriting, in time order
+    n2 = sorted(x.name for x in d.glob("dump[0-9]*"))
+    assert n2 == ["dump%04d" % i for i in range(out_a["dumps"] + out_b["dumps"])] and out_b["dumps"] > 0
+    th = [float(open(d / n).readline().split()[0]) for n in n2]
+    assert all(x < y for x, y in zip(th, th[1:]))
