@@ -38,6 +38,7 @@ class HarmConfig(ctypes.Structure):
         ("Rout", ctypes.c_double), ("hslope", ctypes.c_double), ("cour", ctypes.c_double),
         ("safe", ctypes.c_double), ("rhomin", ctypes.c_double), ("uumin", ctypes.c_double),
         ("gammamax", ctypes.c_double), ("gammafail", ctypes.c_double),
+        ("hot2cold", ctypes.c_int32), ("checkinv", ctypes.c_int32), ("resetregion", ctypes.c_int32),
     ]
 
 
@@ -111,7 +112,7 @@ _OPTIONAL_PROTOS = [
     ("harm_update", _I, [_P, _I, _I, _P, _P, _D, _I, _P]),
     ("harm_invert_grid", _I, [_P, _I, _I, _I, ctypes.POINTER(NewtonStats)]),
     ("harm_bound", _I, [_P, _I]),
-    ("harm_post_fixup", _I, [_P, _I]),
+    ("harm_post_fixup", _I, [_P, _I, _I]),
     ("harm_advance", _I, [_P, _I, _P]),
     ("harm_step", _I, [_P, _I]),
     ("harm_get_boundary_fluxes", _I, [_P, _P]),

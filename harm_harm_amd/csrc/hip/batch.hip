@@ -1,7 +1,8 @@
 // batch.hip -- per-zone kernels over n independent zones (SoA device buffers).
 //
 // harm_invert_batch: Utoprimgen (utoprimgen.c:32) for the inversion
-// microbench; harm_prim_to_cons_batch: primtoU(UEVOLVE) (phys.tools.c:475).
+// microbench, as the torus configuration compiles it (the hot solve, then the
+// cold retry of a failed one; no inversion check); harm_prim_to_cons_batch: primtoU(UEVOLVE) (phys.tools.c:475).
 #include <hip/hip_runtime.h>
 #include "dphys.h"
 #include "hhost.h"
@@ -27,7 +28,17 @@ __global__ __launch_bounds__(kBlock) void k_invert_batch(const double* __restric
   }
   int ns = 0;
   const double seed = wpseed ? wpseed[z] : __builtin_nan("");
-  const int fl = hd::utoprimgen(Uz, g, gam, gammafail, pz, ns, seed);
+  int fl = hd::utoprimgen(Uz, g, gam, gammafail, pz, ns, seed);
+  if (hd::needs_after_hot(fl, true, false)) {  // HOT2COLD (the torus configuration)
+    double Ugf[HARM_NPR], p0[HARM_NPR];
+#pragma unroll
+    for (int pl = 0; pl < HARM_NPR; pl++) {
+      Ugf[pl] = U[pl * n + z];
+      p0[pl] = p[pl * n + z];
+    }
+    hd::geometry_free(Ugf, g, Ugf);
+    fl = hd::after_hot(fl, Ugf, g, gam, gammafail, p0, pz, ns, true, false);
+  }
 #pragma unroll
   for (int pl = 0; pl < HARM_NPR; pl++) p[pl * n + z] = pz[pl];
   pflag[z] = fl;

@@ -72,6 +72,9 @@ extern "C" void harm_default_config(harm_config* c, int32_t n1, int32_t n2, int3
   c->uumin = 1e-6;
   c->gammamax = 25.0;
   c->gammafail = 2500.0;
+  c->hot2cold = 1;     // init.fishmon.h:325
+  c->checkinv = 0;     // init.fishmon.h:74, 225
+  c->resetregion = 1;  // init.fishmon.h:356
 }
 
 extern "C" void harm_bondi_config(harm_config* c, int32_t n1, int32_t n2)
@@ -88,6 +91,9 @@ extern "C" void harm_bondi_config(harm_config* c, int32_t n1, int32_t n2)
   c->uumin = 0.0;
   c->gammamax = 100.0;
   c->gammafail = 100.0 * c->gammamax;
+  c->hot2cold = 0;     // init.sasha.h:186, 379
+  c->checkinv = 1;     // init.sasha.h:73, 198
+  c->resetregion = 0;  // init.sasha.h leaves USERRESETREGION unset
 }
 
 extern "C" harm_status harm_device_count(int32_t* n)

@@ -1,7 +1,8 @@
-// dinvert.h -- one zone of the step's inversion: Jon's Newton solve
-// (Utoprimgen, utoprimgen.c:32) and, on the final substep, fixup1zone's
-// floors (fixup.c:561), as the inversion kernels of the step run it (the
-// plain one in grid.hip, the learned-guess one in mlp.hip).
+// dinvert.h -- one zone of the step's inversion: Utoprimgen (utoprimgen.c:32:
+// Jon's hot Newton solve, the inversion check and the cold retry as the
+// configuration turns them on) and, on the final substep, fixup1zone's floors
+// (fixup.c:561), as the inversion kernels of the step run it (the plain one in
+// grid.hip, the learned-guess one in mlp.hip).
 #pragma once
 #include <hip/hip_runtime.h>
 #include "dgrid.h"
@@ -46,8 +47,12 @@ __device__ __forceinline__ void fix_delta_to(const Params& P, const Geom& G, con
 // limit did (fixup.c:784); fx_p0 (SoA, stride ncell) then holds the
 // primitives ahead of both fixes and, when the floors changed them, fx_pfl
 // the primitives after the floors.  Only fixed zones store their states.
+//
+// Ug: the conserved state in global memory (Uz's source, SoA at o); the rare
+// checks and retries after the hot solve read it again rather than keeping
+// Uz in registers through the Newton iterations.
 template <bool SEEDED = false, bool DIAG = false>
-__device__ __forceinline__ int invert_zone(const Dims& d, const Params& P, double* Uz,
+__device__ __forceinline__ int invert_zone(const Dims& d, const Params& P, double* Uz, const double* __restrict__ Ug,
                                            const double* __restrict__ pg, double* __restrict__ pf,
                                            int32_t* __restrict__ pflag, const double* __restrict__ gcent,
                                            const double* __restrict__ floors, int finalstep, long o, long pp, int& ns,
@@ -72,6 +77,16 @@ __device__ __forceinline__ int invert_zone(const Dims& d, const Params& P, doubl
   // through a pointer the compiler cannot see through frees its 54
   // registers during the Newton iterations (they come back from L1/L2).
   hd::load_geom(hd::fresh(gcent), pp, d.nplane, G);
+  if (hd::needs_after_hot(fl, P.hot2cold, P.checkinv)) {
+    double Ur[HARM_NPR], Ugf[HARM_NPR], p0[HARM_NPR];
+#pragma unroll
+    for (int pl = 0; pl < HARM_NPR; pl++) {
+      Ur[pl] = Ug[pl * nc + o];
+      p0[pl] = pg[pl * nc + o];
+    }
+    hd::geometry_free(Ur, G, Ugf);
+    fl = hd::after_hot(fl, Ugf, G, P.gam, P.gammafail, p0, pz, ns, P.hot2cold, P.checkinv);
+  }
   if (finalstep) {
     double prfloor[2] = {floors[pp], floors[d.nplane + pp]};
     for (int pl = 0; pl < 2; pl++)
@@ -101,7 +116,14 @@ __device__ __forceinline__ int invert_zone(const Dims& d, const Params& P, doubl
       hd::primtoflux(dpr, dq, 0, G1, dUa);
 #pragma unroll
       for (int pl = 0; pl < HARM_NPR; pl++) Ua[pl] += dUa[pl];
+      // the floors' own Utoprimgen (UNOTHING), from the primitives so far;
+      // any flag > 0, the cold retry's mark included, restores prnew
+      double p0[HARM_NPR];
+#pragma unroll
+      for (int pl = 0; pl < HARM_NPR; pl++) p0[pl] = pz[pl];
       fl = hd::jon(Ua, G, P.gam, P.gammafail, pz, ns, __builtin_nan(""));
+      if (hd::needs_after_hot(fl, P.hot2cold, P.checkinv))
+        fl = hd::after_hot(fl, Ua, G, P.gam, P.gammafail, p0, pz, ns, P.hot2cold, P.checkinv);
       if (fl > 0) {
 #pragma unroll
         for (int pl = 0; pl < HARM_NPR; pl++) pz[pl] = prnew[pl];

@@ -414,6 +414,34 @@ __device__ __forceinline__ void func_eprime(double Wp, const Inv& s, double& dx,
   df = -2. * f;
 }
 
+// Cold GRMHD residual W^2 P^2 of the momentum equation and its W' derivative
+// (func_Psq, utoprim_jon.c:3103, with Psq_Wp :1923 and dPsqdWp_Wp :1944).
+__device__ __forceinline__ void func_psq(double Wp, const Inv& s, double& dx, double& resid, double& jac, double& f,
+                                         double& df)
+{
+  const double D = s.D, Bsq = s.Bsq, Qtsq = s.Qtsq;
+  const double W = Wp + D;
+  const double Y = Bsq + 2.0 * W;
+  const double W2 = W * W;
+  const double S = s.QdotB;
+  const double r = -Qtsq * W2 + Wp * (D + W) * (W2 + Bsq * Y) - Y * S * S;
+  const double drdW = 2.0 * (D * ((Bsq + D) * (Bsq + D) - Qtsq) - (S * S) + ((Bsq + D) * (Bsq + 5.0 * D) - Qtsq) * Wp +
+                             3.0 * (Bsq + 2.0 * D) * (Wp * Wp) + 2.0 * (Wp * Wp * Wp));
+  resid = r;
+  jac = drdW;
+  dx = -r / drdW;
+  f = 0.5 * r * r;
+  df = -2. * f;
+}
+
+template <bool COLD>
+__device__ __forceinline__ void residual(double Wp, const Inv& s, double& dx, double& resid, double& jac, double& f,
+                                         double& df)
+{
+  if (COLD) func_psq(Wp, s, dx, resid, jac, f, df);
+  else func_eprime(Wp, s, dx, resid, jac, f, df);
+}
+
 __device__ __forceinline__ bool ultrarel(double x0, const Inv& s) { return (fabs(s.w1) > s.w2) || (fabs(x0) > s.w2); }
 __device__ __forceinline__ bool errcheck(double errx, double x0, const Inv& s)
 {
@@ -431,7 +459,9 @@ __device__ __forceinline__ void validate_wp(double Wpold, double& Wpnew, const I
   Wpnew = (fabs(Wpnew / s.w1) > s.gsq_too_big) ? 0.5 * (Wpold + Wpnew) : Wpnew;
 }
 
-// general_newton_raphson for n=1 without line search.  Returns retval.
+// general_newton_raphson for n=1 without line search (COLD: on the cold
+// residual).  Returns retval.
+template <bool COLD = false>
 __device__ __forceinline__ int newton(int& pflag, double& xio, const Inv& s, int& nstroke)
 {
   double x = xio, x_old = x, x_older = x, dx = 0.0, dx_old = D_VERYBIG, resid, jac, f = 1., df = 1.;
@@ -442,7 +472,7 @@ __device__ __forceinline__ int newton(int& pflag, double& xio, const Inv& s, int
     nstroke++;
     lntries++;
     if (lntries > 10 && !diddamp) damp = 0.5;
-    func_eprime(x, s, dx, resid, jac, f, df);
+    residual<COLD>(x, s, dx, resid, jac, f, df);
     x_older = x_old;
     x_old = x;
     diddamp = false;
@@ -456,7 +486,7 @@ __device__ __forceinline__ int newton(int& pflag, double& xio, const Inv& s, int
         x = x_older;
         dx = dx_old;
         diddamp = true;
-        func_eprime(x, s, dx, resid, jac, f, df);
+        residual<COLD>(x, s, dx, resid, jac, f, df);
         if (resid <= -D_VERYBIG || jac == 0.0) { xio = x; return 1; }
         x_older = x;
         x_old = x;
@@ -539,6 +569,9 @@ __device__ __forceinline__ void inv_setup(const double* Ut, const Geom& g, doubl
   }
 }
 
+// COLD: the cold GRMHD solve (eomtype EOMCOLDGRMHD): the cold EOS (pressure 0,
+// pickeos_eomtype eos.c:259, coldeos.c) in the guess and the result, u = 0.
+template <bool COLD = false>
 __device__ __forceinline__ int new_body(int& pflag, const double* Ut, const Geom& g, double gam, double gammafail,
                                         double* prim, int& nstroke, double wp_seed)
 {
@@ -564,7 +597,7 @@ __device__ __forceinline__ int new_body(int& pflag, const double* Ut, const Geom
     const double gamma = sqrt(gammasq);
     const double rho0 = (s.D) / gamma;
     const double u = prim[HARM_UU];
-    const double p = (gam - 1.0) * u;
+    const double p = COLD ? 0.0 : (gam - 1.0) * u;
     const double BB[4] = {0.0, prim[HARM_B1], prim[HARM_B2], prim[HARM_B3]};
     double BBc[4];
     lower(BB, g, BBc);
@@ -612,7 +645,7 @@ __device__ __forceinline__ int new_body(int& pflag, const double* Ut, const Geom
   }
 
   double Wp = Wp_last;
-  const int rv = newton(pflag, Wp, s, nstroke);
+  const int rv = newton<COLD>(pflag, Wp, s, nstroke);
   if (rv != 0) return rv;
 
   if (Wp == 1.e100) pflag = 51;
@@ -632,15 +665,15 @@ __device__ __forceinline__ int new_body(int& pflag, const double* Ut, const Geom
   const double gamma = sqrt(gammasq);
   const double wmrho0 = (Wp - s.D * utsq / (1.0 + gamma)) / gammasq;
   const double rho0 = s.D / gamma;
-  const double p = ((gam - 1.0) / gam) * wmrho0;
+  const double p = COLD ? 0.0 : ((gam - 1.0) / gam) * wmrho0;
   const double u = wmrho0 - p;
-  if ((rho0 <= 0.) || (p < 0.)) {
+  if ((rho0 <= 0.) || ((p < 0.) && !COLD)) {
     if ((rho0 <= 0.) && (p >= 0.)) pflag = 9;
     if ((rho0 > 0.) && (p < 0.)) pflag = 10;
     if ((rho0 <= 0.) && (p < 0.)) pflag = 11;
   }
   prim[HARM_RHO] = rho0;
-  prim[HARM_UU] = u;
+  prim[HARM_UU] = COLD ? 0.0 : u;
 #pragma unroll
   for (int i = 1; i < 4; i++) prim[HARM_U1 + i - 1] = gamma / (W + s.Bsq) * (Qtcon[i] + s.QdotB * Bcon[i] / W);
   bool foundnan = false;
@@ -657,6 +690,7 @@ __device__ __forceinline__ int new_body(int& pflag, const double* Ut, const Geom
 }
 
 // Utoprim_jon_nonrelcompat_inputnorestmass on geometry-free Ugf.
+template <bool COLD = false>
 __device__ __forceinline__ int jon(const double* Ugf, const Geom& g, double gam, double gammafail, double* pr,
                                    int& nstroke, double wp_seed)
 {
@@ -673,7 +707,7 @@ __device__ __forceinline__ int jon(const double* Ugf, const Geom& g, double gam,
   pt[HARM_B2] = alpha * pr[HARM_B2];
   pt[HARM_B3] = alpha * pr[HARM_B3];
   int pflag = 0;
-  const int ret = new_body(pflag, Ut, g, gam, gammafail, pt, nstroke, wp_seed);
+  const int ret = new_body<COLD>(pflag, Ut, g, gam, gammafail, pt, nstroke, wp_seed);
   if (ret == 0) {
 #pragma unroll
     for (int i = 0; i <= HARM_U3; i++) pr[i] = pt[i];
@@ -683,15 +717,104 @@ __device__ __forceinline__ int jon(const double* Ugf, const Geom& g, double gam,
   return pflag;
 }
 
-// Utoprimgen(UEVOLVE): strip gdet then invert.
+// UtoU(UEVOLVE -> UNOTHING): U without its gdet factor.
+__device__ __forceinline__ void geometry_free(const double* U, const Geom& g, double* Ugf)
+{
+  const double ig = (g.gdet >= 0.0 ? 1.0 : -1.0) / (fabs(g.gdet) + D_SMALL);
+#pragma unroll
+  for (int pl = 0; pl < HARM_NPR; pl++) Ugf[pl] = U[pl] * ig;
+}
+
+// The hot solve of Utoprimgen(UEVOLVE): strip gdet then invert.
 __device__ __forceinline__ int utoprimgen(const double* U, const Geom& g, double gam, double gammafail, double* pr,
                                           int& nstroke, double wp_seed)
 {
   double Ugf[HARM_NPR];
-  const double ig = (g.gdet >= 0.0 ? 1.0 : -1.0) / (fabs(g.gdet) + D_SMALL);
-#pragma unroll
-  for (int pl = 0; pl < HARM_NPR; pl++) Ugf[pl] = U[pl] * ig;
+  geometry_free(U, g, Ugf);
   return jon(Ugf, g, gam, gammafail, pr, nstroke, wp_seed);
+}
+
+// check_on_inversion with FAILIFBADCHECK (utoprimgen.c:697-850, :18) after the
+// hot solve: a converged zone (flag 0) whose U(p) differs from the inverted
+// geometry-free Ugf by more than 10% in rho, u, a momentum or a field
+// component, each against its own norm (the momenta made comparable through
+// u^i/u^t, REMOVERESTMASSFROMUU 2), gets UTOPRIMFAILCONVBADINVERTCOMPARE.  The
+// momentum norm sums the U2 terms twice and leaves U1 out, as the reference
+// does.  The reference forms a third of those sums with a long double 1/3;
+// the double quotient by 3 below is the same correctly rounded value (the
+// quotient of a double by 3 is never within that rounding's error of a
+// halfway point), and the field ratio's long double division can differ from
+// this double one in the last bit only, which matters for a ratio equal to
+// 0.1 to 16 digits.
+__device__ __forceinline__ int check_inversion(int flag, const double* Ugf, const double* pr, const Geom& g, double gam)
+{
+  if (flag != 0) return flag;
+  Geom G1 = g;
+  G1.gdet = 1.0;  // primtoU(UNOTHING)
+  State q;
+  double Un[HARM_NPR];
+  get_state(pr, g, gam, q);
+  primtoflux(pr, q, 0, G1, Un);
+  double on[HARM_NPR], nn[HARM_NPR];
+#pragma unroll
+  for (int pl = 0; pl < HARM_NPR; pl++) { on[pl] = Ugf[pl]; nn[pl] = Un[pl]; }
+#pragma unroll
+  for (int jj = 1; jj <= 3; jj++) {
+    on[HARM_UU + jj] = Ugf[HARM_UU + jj] * (q.ucon[jj] / q.ucon[0]);
+    nn[HARM_UU + jj] = Un[HARM_UU + jj] * (q.ucon[jj] / q.ucon[0]);
+  }
+  int bad = 0;
+#pragma unroll
+  for (int pl = 0; pl < HARM_NPR; pl++) {
+    double fdiff;
+    if (pl == HARM_RHO || pl == HARM_UU) {
+      fdiff = fabs(nn[pl] - on[pl]) / (fabs(nn[pl]) + fabs(on[pl]) + D_SMALL);
+    } else if (pl <= HARM_U3) {
+      double en = (fabs(nn[HARM_U2]) + fabs(on[HARM_U2]) + fabs(nn[HARM_U2]) + fabs(on[HARM_U2]) + fabs(nn[HARM_U3]) +
+                   fabs(on[HARM_U3])) / 3.0;
+      const double eu = 0.5 * (fabs(on[HARM_UU]) + fabs(nn[HARM_UU]));
+      en = en > eu ? en : eu;  // MAX, global.nondepnmemonics.h:761
+      fdiff = fabs(nn[pl] - on[pl]) / (en + D_SMALL);
+    } else {
+      fdiff = fabs(nn[pl] - on[pl]) / ((fabs(nn[HARM_B1]) + fabs(on[HARM_B1]) + fabs(nn[HARM_B2]) + fabs(on[HARM_B2]) +
+                                        fabs(nn[HARM_B3]) + fabs(on[HARM_B3])) / 3.0 + D_SMALL);
+    }
+    if (fdiff > 1E-1 && fabs(on[pl]) > D_SMALL && fabs(nn[pl]) > D_SMALL) bad++;
+  }
+  return bad ? HARM_UTOPRIMFAILCONVBADINVERTCOMPARE : flag;
+}
+
+// The rest of Utoprimgen's EOMGRMHD branch after the hot solve
+// (utoprimgen.c:215-263): check_on_inversion when checkinv, then for a failed
+// hot solve (any failure: USECOLDIFHOT* = 1, :587-591) the cold solve from
+// the same guess pg when hot2cold (trycoldinversion, :597-690).  A converged
+// cold solve keeps its density and velocities, takes u back from the guess
+// and marks the zone UTOPRIMFAILU2AVG1, so that failure averaging recomputes
+// u alone (:655-660); otherwise the hot result and flag stand.  Ugf: the
+// geometry-free conserved state; the cold solve's strokes add to ns.
+__device__ __forceinline__ int after_hot(int fl, const double* Ugf, const Geom& g, double gam, double gammafail,
+                                         const double* pg, double* pr, int& ns, bool hot2cold, bool checkinv)
+{
+  if (checkinv) fl = check_inversion(fl, Ugf, pr, g, gam);
+  if (hot2cold && fl > 0) {
+    double pc[HARM_NPR];
+#pragma unroll
+    for (int pl = 0; pl < HARM_NPR; pl++) pc[pl] = pg[pl];
+    const int cf = jon<true>(Ugf, g, gam, gammafail, pc, ns, __builtin_nan(""));
+    if (cf >= HARM_UTOPRIMFAILFIXEDCOLD && cf <= 0) {  // IFUTOPRIMNOFAILORFIXED
+#pragma unroll
+      for (int pl = 0; pl < HARM_NPR; pl++) pr[pl] = pc[pl];
+      pr[HARM_UU] = pg[HARM_UU];
+      fl = HARM_UTOPRIMFAILU2AVG1;
+    }
+  }
+  return fl;
+}
+
+// whether after_hot has anything to do for a hot-solve flag
+__device__ __forceinline__ bool needs_after_hot(int fl, bool hot2cold, bool checkinv)
+{
+  return (checkinv && fl == 0) || (hot2cold && fl > 0);
 }
 
 }  // namespace hd

@@ -36,7 +36,9 @@ enum { FL_CONV = 0, FL_FLOOR = 1, FL_GAMMA = 2, FL_INFLOW = 3, FL_RHONEG = 4, FL
 constexpr int kFailCats[4] = {FL_CONV, FL_RHONEG, FL_UNEG, FL_RHOUNEG};
 __host__ __device__ inline int fail_slot(int pflag)  // index into kFailCats
 {
-  return pflag == HARM_UTOPRIMFAILRHONEG ? 1 : pflag == HARM_UTOPRIMFAILUNEG ? 2 : pflag == HARM_UTOPRIMFAILRHOUNEG ? 3 : 0;
+  return pflag == HARM_UTOPRIMFAILRHONEG ? 1
+         : (pflag == HARM_UTOPRIMFAILUNEG || pflag == HARM_UTOPRIMFAILU2AVG1 || pflag == HARM_UTOPRIMFAILU2AVG2) ? 2
+         : pflag == HARM_UTOPRIMFAILRHOUNEG ? 3 : 0;
 }
 constexpr int kEnFaceBlocks = 32;  // blocks per boundary of the boundary-flux sums
 constexpr int kDiagSourceStep = 4;  // DIAGSOURCECOMPSTEP (definit.h:19): full steps between source samples
@@ -46,6 +48,7 @@ struct Params {
   double dx[4], startx[4];
   double R0;  // r = R0 + exp(X1)
   int bc;     // HARM_PROBLEM_*: which radial and polar boundary conditions
+  int hot2cold, checkinv, resetregion;  // the configuration's failure-handling switches (harm_config)
 };
 
 __device__ __forceinline__ void atomic_min_pos(double* addr, double v)
@@ -165,6 +168,7 @@ struct GridRef {
   int fluxtype;
   double* arr[HARM_NARR];
   int32_t* pflag;
+  double* pfix;
   const double* geom[HARM_NLOC];
   const double* gcorn3;
   const double* conn;

@@ -306,7 +306,7 @@ __device__ __forceinline__ void invert_body(Dims d, Params P, const double* __re
     double Uz[HARM_NPR];
 #pragma unroll
     for (int pl = 0; pl < HARM_NPR; pl++) Uz[pl] = U[pl * nc + o];
-    nf = invert_zone<false, DIAG>(d, P, Uz, pg, pf, pflag, gcent, floors, finalstep, o, pp, ns, 0.0, nullptr, fx_p0,
+    nf = invert_zone<false, DIAG>(d, P, Uz, U, pg, pf, pflag, gcent, floors, finalstep, o, pp, ns, 0.0, nullptr, fx_p0,
                                   fx_pfl, fx_flag) > 0;
   }
   newton_counts_to(ns, nf, di);
@@ -328,75 +328,205 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
 }
 
 // ---------------------------------------------------------------------------
-// failure averaging (fixup_utoprim with the simple_average cascade)
+// failure averaging (post_fixup, fixup.c:53 -> fixup_utoprim :1061)
 // ---------------------------------------------------------------------------
-// DIAG (final substep): the change of each averaged zone, dV (U(after) -
-// U(before)), is added to dfl and `true` returned (fixuputoprim_accounting,
-// fixup.c:1370, 1595; diag_fixup with pr0 the primitives before averaging);
-// *cat is the zone's failure slot (fail_slot of its flag).
+// bound_pflag (step_ch.c:1637 -> bound_pflag_user, boundsint.c:18-330): one
+// ghost layer of flags (numbnd = 1 for BOUNDINTTYPE,
+// initbase.boundloop.c:83-88), filled in the reference's order by two launches:
+//  phase 0: phi periodic over the (i, j) plane of -1..n (boundsint.c:80-92),
+//           which reads the i and j ghost flags as the previous call left them;
+//  phase 1: at active k, the outer radial OUTFLOW copy (torus X1UP,
+//           init.fishmon.c:247; over active j, initbase.boundloop.c:358-363)
+//           and the polar mirror (POLARAXIS, boundsint.c:178-200) over i in
+//           -1..n1, which sees the radial copy: the inner radial FREEOUTFLOW
+//           and both Bondi radial boundaries are not handled there, so those
+//           ghost flags keep NANPFLAG (set_arrays_multidimen.c:404).
+// Item t of a phase: phase 0 one (i, j) column; phase 1 first the radial
+// copies (n2 n3 of them, torus only), then one (i, k) pair of polar mirrors.
+__host__ __device__ __forceinline__ long bound_pflag_items(const Dims& d, int phase)
+{
+  const int s1 = d.n1 > 1, s2 = d.n2 > 1;
+  if (phase == 0) return d.n3 > 1 ? (long)(d.n1 + 2 * s1) * (d.n2 + 2 * s2) : 0;
+  return (long)d.n2 * d.n3 + (s2 ? (long)(d.n1 + 2 * s1) * d.n3 : 0);
+}
+
+__device__ __forceinline__ void bound_pflag_body(const Dims& d, int bc, int32_t* __restrict__ pflag, int phase, long t)
+{
+  const int s1 = d.n1 > 1, s2 = d.n2 > 1;
+  if (phase == 0) {
+    const int w = d.n1 + 2 * s1;
+    const int i = (int)(t % w) - s1, j = (int)(t / w) - s2;
+    pflag[hd::cidx(d, i, j, -1)] = pflag[hd::cidx(d, i, j, d.n3 - 1)];
+    pflag[hd::cidx(d, i, j, d.n3)] = pflag[hd::cidx(d, i, j, 0)];
+    return;
+  }
+  const long nr = (long)d.n2 * d.n3;
+  if (t < nr) {
+    if (bc == HARM_PROBLEM_TORUS && s1) {
+      const int j = (int)(t % d.n2), k = (int)(t / d.n2);
+      pflag[hd::cidx(d, d.n1, j, k)] = pflag[hd::cidx(d, d.n1 - 1, j, k)];
+    }
+    return;
+  }
+  t -= nr;
+  const int w = d.n1 + 2 * s1;
+  const int i = (int)(t % w) - s1, k = (int)(t / w);
+  // the value at (i, 0 or n2-1, k) once the radial copy is done; the copy at
+  // i = n1 is taken from its source so that this launch reads no flag it writes
+  const int ii = (i == d.n1 && bc == HARM_PROBLEM_TORUS) ? d.n1 - 1 : i;
+  pflag[hd::cidx(d, i, -1, k)] = pflag[hd::cidx(d, ii, 0, k)];
+  pflag[hd::cidx(d, i, d.n2, k)] = pflag[hd::cidx(d, ii, d.n2 - 1, k)];
+}
+
+__global__ __launch_bounds__(kBlock) void k_bound_pflag(Dims d, int bc, int32_t* __restrict__ pflag, int phase)
+{
+  const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (t < bound_pflag_items(d, phase)) bound_pflag_body(d, bc, pflag, phase, t);
+}
+
+// IFUTOPRIMNOFAILORFIXED (global.nondepnmemonics.h:854)
+__device__ __forceinline__ int good_flag(int f) { return f >= HARM_UTOPRIMFAILFIXEDCOLD && f <= 0; }
+
+// Whether failure averaging accounts a zone with flag f (fixuputoprim_accounting,
+// fixup.c:1605-1745): the convergence failures on every substep, the soft
+// ones (rho < 0, u < 0, both, the u-only marks) on the final substep only
+// (DOCOUNTNEG* = 1, STEPOVERNEG* = NEGDENSITY_NEVERFIXUP).  Accounted zones
+// end with UTOPRIMFAILFIXEDUTOPRIM (:1761).
+__device__ __forceinline__ bool accounted(int f, int finalstep)
+{
+  if (f == HARM_UTOPRIMFAILRHONEG || f == HARM_UTOPRIMFAILUNEG || f == HARM_UTOPRIMFAILRHOUNEG ||
+      f == HARM_UTOPRIMFAILU2AVG1 || f == HARM_UTOPRIMFAILU2AVG2)
+    return finalstep != 0;
+  return f > 0 || f == HARM_UTOPRIMFAILFIXEDCOLD || f == -2;
+}
+
+// fixup_utoprim's averaging of one failed zone (fixup.c:1061-1330), the flags
+// bounded: UTOPRIMADJUST = UTOPRIMAVG, GENERALAVERAGE 1 (:1035), HANDLE*NEG 0
+// (so fixup_negdensities changes nothing, :1457), CAUSALAVG 0 (:1817),
+// HOWTOAVG_WHEN_U2AVG = MAXUPOSAVG (:1827), TODONOGOOD = NOGOODAVERAGE
+// (:2278); P.resetregion is USERRESETREGION (init.fishmon.h:356).  Reads pf
+// and the flags as they stand (ptoavg, pflagfailorig); writes the zone's new
+// primitives to pout at its own place, for the write-back launch.
+//  - u-only failures (the U2AVG marks of the cold retry) take u from
+//    general_average's second loop: the smallest u over the symmetric
+//    neighbour pairs whose two u's are both at least the zone's own (:1990);
+//  - every other failure takes rho..U3 as the mean of the good neighbours in
+//    the 3^d box, summed pair by pair into two alternating sums (:1960-2030);
+//  - with no usable neighbour, fixup_nogood (:2291): the mean of the 4+4
+//    in-plane neighbours (AVG4_1, AVG4_2, :1812-1813), failed or not; in the
+//    reset region (j <= 1, j >= n2 - 2 or i < 10) rho and u so and the
+//    velocity the normal observer's (set_atmosphere VELREL4, init.tools.c:797);
+//  - after a velocity average the Lorentz factor is capped at the zone's own
+//    pre-average value, itself capped at GAMMAMAX (:1290-1322).
+// DIAG (final substep): an accounted zone's change dV (U(after) - U(before))
+// goes to dfl and `true` is returned, *cat being its category slot.
 template <bool DIAG = false>
-__device__ __forceinline__ bool post_fixup_body(Dims d, Params P, const double* pa, double* pf,
-                                                       const int32_t* __restrict__ pflag,
-                                                       const double* __restrict__ gcent, const int blk,
-                                                       double* dfl = nullptr, int* cat = nullptr)
+__device__ __forceinline__ bool post_fixup_body(Dims d, Params P, const double* __restrict__ pf,
+                                                const int32_t* __restrict__ pflag, double* __restrict__ pout,
+                                                const double* __restrict__ gcent, int finalstep, const int blk,
+                                                double* dfl = nullptr, int* cat = nullptr)
 {
   const long t = (long)blk * kBlock + threadIdx.x;
   const long nact = (long)d.n1 * d.n2 * d.n3;
   if (t >= nact) return false;
   const int i = (int)(t % d.n1), j = (int)((t / d.n1) % d.n2), k = (int)(t / ((long)d.n1 * d.n2));
   const long o = hd::cidx(d, i, j, k);
-  if (pflag[o] <= 0) return false;
-  const int e1 = d.n2 > 1;
-  auto good = [&](int ii, int jj) {
-    if (ii < 0 || ii >= d.n1 || jj < 0 || jj >= d.n2) return true;
-    return pflag[hd::cidx(d, ii, jj, k)] <= 0;
-  };
+  const int f = pflag[o];
+  if (f <= 0) return false;
   const long nc = d.ncell;
-  auto A = [&](int pl, int ii, int jj) { return pa[pl * nc + hd::cidx(d, ii, jj, k)]; };
-  const int jp = j + e1, jm = j - e1;
+  const int s1 = d.n1 > 1, s2 = d.n2 > 1, s3 = d.n3 > 1;
+  const int rnx = s1 ? 3 : 1, rny = s2 ? 3 : 1;
+  const int npow = (s1 + s2 + s3) == 3 ? 27 : (s1 + s2 + s3) == 2 ? 9 : 3;
+  const int numupairs = (npow - 1) / 2;
+  const bool urange = f == HARM_UTOPRIMFAILU2AVG1 || f == HARM_UTOPRIMFAILU2AVG2 || f == 13;  // UPERC
+  const bool uonly = f == HARM_UTOPRIMFAILU2AVG1 || f == HARM_UTOPRIMFAILU2AVG2;
+  const int startpl = urange ? HARM_UU : HARM_RHO, endpl = urange ? HARM_UU : HARM_U3;
+  auto A = [&](int pl, int ii, int jj, int kk) { return pf[pl * nc + hd::cidx(d, ii, jj, kk)]; };
   double pz[HARM_NPR];
 #pragma unroll
   for (int pl = 0; pl < HARM_NPR; pl++) pz[pl] = pf[pl * nc + o];
-  for (int pl = HARM_RHO; pl <= HARM_U3; pl++) {
-    double v;
-    if (good(i, jp) && good(i, jm) && good(i + 1, j) && good(i - 1, j))
-      v = 0.25 * (A(pl, i, jp) + A(pl, i, jm) + A(pl, i + 1, j) + A(pl, i - 1, j));
-    else if (good(i + 1, jp) && good(i + 1, jm) && good(i - 1, jp) && good(i - 1, jm))
-      v = 0.25 * (A(pl, i + 1, jp) + A(pl, i + 1, jm) + A(pl, i - 1, jp) + A(pl, i - 1, jm));
-    else if (good(i, jp) && good(i, jm)) v = 0.5 * (A(pl, i, jp) + A(pl, i, jm));
-    else if (good(i + 1, j) && good(i - 1, j)) v = 0.5 * (A(pl, i + 1, j) + A(pl, i - 1, j));
-    else if (good(i + 1, jp) && good(i - 1, jm)) v = 0.5 * (A(pl, i + 1, jp) + A(pl, i - 1, jm));
-    else if (good(i + 1, jm) && good(i - 1, jp)) v = 0.5 * (A(pl, i + 1, jm) + A(pl, i - 1, jp));
-    else if (good(i + 1, jp)) v = A(pl, i + 1, jp);
-    else if (good(i + 1, j)) v = A(pl, i + 1, j);
-    else v = A(pl, i, j);
-    pz[pl] = v;
+  // general_average (fixup.c:1830)
+  double sum0[HARM_NPR], sum1[HARM_NPR], lastmin[HARM_NPR];
+#pragma unroll
+  for (int pl = 0; pl < HARM_NPR; pl++) { sum0[pl] = 0.0; sum1[pl] = 0.0; lastmin[pl] = 1E50; }
+  int numavg0 = 0, numavg1 = 0;
+  const double ref = pz[HARM_UU];
+  for (int qq = 0; qq < numupairs; qq++) {
+    const int ii = qq % rnx - s1, jj = (qq % (rnx * rny)) / rnx - s2, kk = qq / (rnx * rny) - s3;
+    const int th = good_flag(pflag[hd::cidx(d, i + ii, j + jj, k + kk)]);
+    const int at = good_flag(pflag[hd::cidx(d, i - ii, j - jj, k - kk)]);
+    numavg0 += th + at;
+    for (int pl = startpl; pl <= endpl; pl++) {
+      const double f1 = A(pl, i + ii, j + jj, k + kk), f2 = A(pl, i - ii, j - jj, k - kk);
+      if (!uonly) {
+        const double v = f1 * th + f2 * at;
+        if (qq & 1) sum1[pl] += v;
+        else sum0[pl] += v;
+      } else if (f1 >= ref && f2 >= ref) {
+        const double m1 = lastmin[pl] < f1 ? lastmin[pl] : f1;
+        lastmin[pl] = m1 < f2 ? m1 : f2;
+        numavg1++;
+      }
+    }
+  }
+  int numavg;
+  if (!uonly) {
+    if (numavg0 != 0)
+      for (int pl = startpl; pl <= endpl; pl++) pz[pl] = (sum0[pl] + sum1[pl]) / ((double)numavg0);
+    numavg = numavg0;
+  } else {
+    if (numavg1 != 0)
+      for (int pl = startpl; pl <= endpl; pl++) pz[pl] = lastmin[pl];
+    numavg = numavg1;
+  }
+  if (f == HARM_UTOPRIMFAILU2AVG2) numavg++;
+  if (numavg == 0) {  // fixup_nogood
+    const int ip = i + s1, im = i - s1, jp = j + s2, jm = j - s2;
+    auto avg8 = [&](int pl) {
+      return 0.5 * ((0.25 * ((A(pl, i, jp, k) + A(pl, i, jm, k)) + (A(pl, im, j, k) + A(pl, ip, j, k)))) +
+                    (0.25 * ((A(pl, ip, jp, k) + A(pl, ip, jm, k)) + (A(pl, im, jp, k) + A(pl, im, jm, k)))));
+    };
+    const bool reset = P.resetregion && ((j <= 1 || j >= d.n2 - 2) || (i < 10));
+    if (reset) {
+      for (int pl = HARM_RHO; pl <= HARM_UU; pl++) pz[pl] = avg8(pl);
+      for (int pl = HARM_U1; pl <= HARM_U3; pl++) pz[pl] = 0.0;
+    } else {
+      for (int pl = startpl; pl <= endpl; pl++) pz[pl] = avg8(pl);
+    }
   }
   Geom G;
   hd::load_geom(gcent, hd::pidx(d, i, j), d.nplane, G);
-  hd::limit_gamma(P.gammamax, pz, G);
-  if (DIAG) {
+  if (startpl <= HARM_U1 && endpl >= HARM_U3) {
     double p0[HARM_NPR];
 #pragma unroll
-    for (int pl = 0; pl < HARM_NPR; pl++) p0[pl] = pa[pl * nc + o];
-    fix_delta_to(P, G, p0, pz, dfl);
-    *cat = fail_slot(pflag[o]);
+    for (int pl = 0; pl < HARM_NPR; pl++) p0[pl] = pf[pl * nc + o];
+    double gamma = hd::gamma_of(p0, G);
+    if (gamma > P.gammamax) gamma = P.gammamax;
+    hd::limit_gamma(gamma, pz, G);
   }
 #pragma unroll
-  for (int pl = 0; pl < HARM_NPR; pl++) pf[pl * nc + o] = pz[pl];
-  return true;
+  for (int pl = 0; pl < HARM_NPR; pl++) pout[pl * nc + o] = pz[pl];
+  if (DIAG && accounted(f, finalstep)) {
+    double p0[HARM_NPR];
+#pragma unroll
+    for (int pl = 0; pl < HARM_NPR; pl++) p0[pl] = pf[pl * nc + o];
+    fix_delta_to(P, G, p0, pz, dfl);
+    *cat = fail_slot(f);
+    return true;
+  }
+  return false;
 }
 
 template <bool DIAG>
-__global__ __launch_bounds__(kBlock) void k_post_fixup(Dims d, Params P, const double* pa, double* pf,
-                                                       const int32_t* __restrict__ pflag,
-                                                       const double* __restrict__ gcent,
+__global__ __launch_bounds__(kBlock) void k_post_fixup(Dims d, Params P, const double* __restrict__ pf,
+                                                       const int32_t* __restrict__ pflag, double* __restrict__ pout,
+                                                       const double* __restrict__ gcent, int finalstep,
                                                        double* __restrict__ fl_part)
 {
   double dfl[HARM_NPR] = {0, 0, 0, 0, 0, 0, 0, 0};
   int cat = 0;
-  const bool ch = post_fixup_body<DIAG>(d, P, pa, pf, pflag, gcent, blockIdx.x, dfl, &cat);
-  if (DIAG && !__syncthreads_or(ch)) {  // no failed zone in the block
+  const bool ch = post_fixup_body<DIAG>(d, P, pf, pflag, pout, gcent, finalstep, blockIdx.x, dfl, &cat);
+  if (DIAG && !__syncthreads_or(ch)) {  // no accounted zone in the block
     if (threadIdx.x < 32) fl_part[(long)(threadIdx.x >> 3) * gridDim.x * 8 + (long)blockIdx.x * 8 + (threadIdx.x & 7)] = 0.0;
     return;
   }
@@ -408,6 +538,33 @@ __global__ __launch_bounds__(kBlock) void k_post_fixup(Dims d, Params P, const d
       for (int pl = 0; pl < HARM_NPR; pl++) v[pl] = mine ? dfl[pl] : 0.0;
       block_sum8_part(v, mine, fl_part + (long)c * gridDim.x * 8, blockIdx.x);
     }
+}
+
+// The write-back after k_post_fixup: every failed zone takes its averaged
+// primitives, and accounted zones (also the fixed-flag ones, whose
+// accounting changes nothing) get UTOPRIMFAILFIXEDUTOPRIM.
+__device__ __forceinline__ void fixup_apply_body(const Dims& d, double* __restrict__ pf, int32_t* __restrict__ pflag,
+                                                 const double* __restrict__ pin, int finalstep, const int blk)
+{
+  const long t = (long)blk * kBlock + threadIdx.x;
+  const long nact = (long)d.n1 * d.n2 * d.n3;
+  if (t >= nact) return;
+  const int i = (int)(t % d.n1), j = (int)((t / d.n1) % d.n2), k = (int)(t / ((long)d.n1 * d.n2));
+  const long o = hd::cidx(d, i, j, k);
+  const int f = pflag[o];
+  if (f == 0) return;
+  if (f > 0) {
+    const long nc = d.ncell;
+#pragma unroll
+    for (int pl = 0; pl < HARM_NPR; pl++) pf[pl * nc + o] = pin[pl * nc + o];
+  }
+  if (accounted(f, finalstep)) pflag[o] = HARM_UTOPRIMFAILFIXEDUTOPRIM;
+}
+
+__global__ __launch_bounds__(kBlock) void k_fixup_apply(Dims d, double* __restrict__ pf, int32_t* __restrict__ pflag,
+                                                        const double* __restrict__ pin, int finalstep)
+{
+  fixup_apply_body(d, pf, pflag, pin, finalstep, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -838,7 +995,8 @@ struct harm_handle {
   Params P;
   hipStream_t stream = nullptr;
   double* arr[HARM_NARR] = {nullptr};
-  int32_t* pflag = nullptr;
+  int32_t* pflag = nullptr;  // over the storage box; the one-layer ghost band holds bound_pflag's values
+  double* pfix = nullptr;    // failure averaging's new primitives of the failed zones (SoA, own place)
   double* geom[HARM_NLOC] = {nullptr};
   double* gcorn3 = nullptr;
   double* conn = nullptr;
@@ -883,6 +1041,14 @@ harm_status upload(harm_handle* h, double* dst, const double* src, size_t n)
 
 harm_status alloc(void** p, size_t bytes) { return hh::check_hip(hipMalloc(p, bytes ? bytes : 8), "hipMalloc"); }
 
+// every flag NANPFLAG, as a run starts (set_arrays_multidimen.c:404): ghost
+// flags that bound_pflag never sets keep that value, and the flags' phi ghost
+// corners are read before they are first set
+harm_status reset_pflag(harm_handle* h)
+{
+  return hh::check_hip(hipMemsetD32((hipDeviceptr_t)h->pflag, HARM_NANPFLAG, (size_t)h->d.ncell), "hipMemsetD32");
+}
+
 void make_dims(const harm_config* cfg, Dims& d, Params& P)
 {
   d.n1 = cfg->n1; d.n2 = cfg->n2; d.n3 = cfg->n3;
@@ -896,6 +1062,9 @@ void make_dims(const harm_config* cfg, Dims& d, Params& P)
   P.gam = cfg->gam; P.cour = cfg->cour; P.safe = cfg->safe; P.gammamax = cfg->gammamax; P.gammafail = cfg->gammafail;
   P.R0 = cfg->R0;
   P.bc = cfg->problem;
+  P.hot2cold = cfg->hot2cold != 0;
+  P.checkinv = cfg->checkinv != 0;
+  P.resetregion = cfg->resetregion != 0;
   P.startx[0] = 0.0; P.dx[0] = 0.0;
   P.startx[1] = log(cfg->Rin - cfg->R0);
   P.dx[1] = (log(cfg->Rout - cfg->R0) - log(cfg->Rin - cfg->R0)) / cfg->n1;
@@ -1187,20 +1356,29 @@ harm_status enq_bound(harm_handle* h, int which)
   return hh::check_launch("bounds");
 }
 
+// post_fixup (fixup.c:53): bound the flags, average the failed zones into the
+// scratch pfix (reading pf and the flags unchanged: failed neighbours are read
+// too), then write them back and mark the accounted zones.
 // ener: the averaged zones' changes go to their fladdterms (final substep only)
-harm_status enq_post_fixup(harm_handle* h, int which, bool ener = false)
+harm_status enq_post_fixup(harm_handle* h, int which, int finalstep, bool ener = false)
 {
   const Dims& d = h->d;
   const unsigned nb = nblocks((long)d.n1 * d.n2 * d.n3);
   prof_begin(h);
-  // No snapshot is needed: a failed zone averages only good neighbours (or
-  // ghost zones, or itself), and only failed zones are written.
+  for (int ph = 0; ph < 2; ph++) {
+    const long n = bound_pflag_items(d, ph);
+    if (n > 0)
+      hipLaunchKernelGGL(k_bound_pflag, dim3(nblocks(n)), dim3(kBlock), 0, h->stream, d, h->cfg.problem, h->pflag,
+                         ph);
+  }
   if (ener)
-    hipLaunchKernelGGL(k_post_fixup<true>, dim3(nb), dim3(kBlock), 0, h->stream, d, h->P, h->arr[which],
-                       h->arr[which], h->pflag, h->geom[HARM_LOC_CENT], h->enpart);
+    hipLaunchKernelGGL(k_post_fixup<true>, dim3(nb), dim3(kBlock), 0, h->stream, d, h->P, h->arr[which], h->pflag,
+                       h->pfix, h->geom[HARM_LOC_CENT], finalstep, h->enpart);
   else
-    hipLaunchKernelGGL(k_post_fixup<false>, dim3(nb), dim3(kBlock), 0, h->stream, d, h->P, h->arr[which],
-                       h->arr[which], h->pflag, h->geom[HARM_LOC_CENT], nullptr);
+    hipLaunchKernelGGL(k_post_fixup<false>, dim3(nb), dim3(kBlock), 0, h->stream, d, h->P, h->arr[which], h->pflag,
+                       h->pfix, h->geom[HARM_LOC_CENT], finalstep, nullptr);
+  hipLaunchKernelGGL(k_fixup_apply, dim3(nb), dim3(kBlock), 0, h->stream, d, h->arr[which], h->pflag, h->pfix,
+                     finalstep);
   prof_end(h, HARM_K_FIXUP);
   if (ener) {
     int dst[4];
@@ -1268,7 +1446,7 @@ harm_status enq_substep(harm_handle* h, int to, int nto, const double CUf[4][4],
   HH_TRY(enq_invert(h, pbb[to], pff[to], finalstep, false, ener));
   hipLaunchKernelGGL(k_scal_substep_dt, dim3(1), dim3(1), 0, h->stream, h->ds);
   HH_TRY(enq_bound(h, pff[to]));
-  HH_TRY(enq_post_fixup(h, pff[to], ener && finalstep));
+  HH_TRY(enq_post_fixup(h, pff[to], finalstep, ener && finalstep));
   HH_TRY(enq_bound(h, pff[to]));
   return hh::check_launch("substep");
 }
@@ -1299,7 +1477,7 @@ void rk_flux_weights(int timeorder, double* w)
 // replayed.
 // ---------------------------------------------------------------------------
 enum { BS_FLUX1 = 0, BS_FLUX2, BS_FLUX3, BS_EMF, BS_CT, BS_FACC, BS_ZONE, BS_BX1, BS_X2R, BS_X2P,
-       BS_X2M, BS_BX3, BS_ONE, BS_N };
+       BS_X2M, BS_BX3, BS_ONE, BS_PFB, BS_N };
 
 __device__ __forceinline__ void batch_decode(const int* __restrict__ pre, int ng, int b, int& gi, int& lb)
 {
@@ -1374,11 +1552,27 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
               gmem(g.di), lb);
 }
 
-__global__ __launch_bounds__(kBlock) void kb_post_fixup(const GridRef* __restrict__ G, const int* __restrict__ pre,
-                                                        int ng, int which)
+__global__ __launch_bounds__(kBlock) void kb_bound_pflag(const GridRef* __restrict__ G, const int* __restrict__ pre,
+                                                         int ng, int phase)
 {
   BATCH_DECODE(false)
-  post_fixup_body(g.d, g.P, gmem(g.arr[which]), gmem(g.arr[which]), gmem(g.pflag), gmem(g.geom[HARM_LOC_CENT]), lb);
+  const long t = (long)lb * kBlock + threadIdx.x;
+  if (t < bound_pflag_items(g.d, phase)) bound_pflag_body(g.d, g.P.bc, gmem(g.pflag), phase, t);
+}
+
+__global__ __launch_bounds__(kBlock) void kb_post_fixup(const GridRef* __restrict__ G, const int* __restrict__ pre,
+                                                        int ng, int which, int finalstep)
+{
+  BATCH_DECODE(false)
+  post_fixup_body(g.d, g.P, gmem(g.arr[which]), gmem(g.pflag), gmem(g.pfix), gmem(g.geom[HARM_LOC_CENT]), finalstep,
+                  lb);
+}
+
+__global__ __launch_bounds__(kBlock) void kb_fixup_apply(const GridRef* __restrict__ G, const int* __restrict__ pre,
+                                                         int ng, int which, int finalstep)
+{
+  BATCH_DECODE(false)
+  fixup_apply_body(g.d, gmem(g.arr[which]), gmem(g.pflag), gmem(g.pfix), finalstep, lb);
 }
 
 __global__ __launch_bounds__(kBlock) void kb_bound_x1(const GridRef* __restrict__ G, const int* __restrict__ pre,
@@ -1456,6 +1650,7 @@ GridRef make_ref(const harm_handle* h)
   r.fluxtype = h->cfg.flux;
   for (int a = 0; a < HARM_NARR; a++) r.arr[a] = h->arr[a];
   r.pflag = h->pflag;
+  r.pfix = h->pfix;
   for (int l = 0; l < HARM_NLOC; l++) r.geom[l] = h->geom[l];
   r.gcorn3 = h->gcorn3;
   r.conn = h->conn;
@@ -1513,6 +1708,7 @@ harm_status batch_build(harm_batch* b)
     if (e1 && d.n2 <= d.ng2) b->x2_merged = false;
     cnt[BS_BX3] = e2 ? nblocks(d.nplane * 2L * d.ng3) : 0;
     cnt[BS_ONE] = 1;
+    cnt[BS_PFB] = nblocks(std::max(bound_pflag_items(d, 0), bound_pflag_items(d, 1)));
     for (int st = 0; st < BS_N; st++) {
       const long nx = (long)pre[(size_t)st * (ng + 1) + gi] + cnt[st];
       if (nx > 0x7fffffffL) return hh::set_error(HARM_EINVAL, "harm_batch: too many blocks in one stage");
@@ -1583,7 +1779,12 @@ harm_status enq_batch_substep(harm_batch* b, int to, int nto, const double CUf[4
   hipLaunchKernelGGL(kb_scal, dim3(g1), dim3(64), 0, s, G, ng, 2);
   // bounds, failure averaging, bounds
   for (int pass = 0; pass < 2; pass++) {
-    if (pass == 1) hipLaunchKernelGGL(kb_post_fixup, gz, blk, 0, s, G, P(BS_ZONE), ng, pff[to]);
+    if (pass == 1) {
+      for (int ph = 0; ph < 2; ph++)
+        hipLaunchKernelGGL(kb_bound_pflag, dim3((unsigned)b->ntot[BS_PFB]), blk, 0, s, G, P(BS_PFB), ng, ph);
+      hipLaunchKernelGGL(kb_post_fixup, gz, blk, 0, s, G, P(BS_ZONE), ng, pff[to], finalstep);
+      hipLaunchKernelGGL(kb_fixup_apply, gz, blk, 0, s, G, P(BS_ZONE), ng, pff[to], finalstep);
+    }
     if (b->ntot[BS_BX1]) hipLaunchKernelGGL(kb_bound_x1, dim3((unsigned)b->ntot[BS_BX1]), blk, 0, s, G, P(BS_BX1), ng,
                                             pff[to]);
     if (b->ntot[BS_X2M] && b->x2_merged)
@@ -1662,7 +1863,8 @@ extern "C" harm_status harm_create(const harm_config* cfg, harm_handle** out)
     TRYC(hh::check_hip(hipMemset(h->arr[a], 0, sizeof(double) * HARM_NPR * d.ncell), "hipMemset"));
   }
   TRYC(alloc((void**)&h->pflag, sizeof(int32_t) * d.ncell));
-  TRYC(hh::check_hip(hipMemset(h->pflag, 0, sizeof(int32_t) * d.ncell), "hipMemset"));
+  TRYC(reset_pflag(h));
+  TRYC(alloc((void**)&h->pfix, sizeof(double) * HARM_NPR * d.ncell));
   h->emf_ni = d.n1 + (d.n1 > 1);
   h->emf_nj = d.n2 + (d.n2 > 1);
   h->emf_nk = d.n3 + (d.n3 > 1);
@@ -1699,7 +1901,7 @@ extern "C" harm_status harm_destroy(harm_handle* h)
     (void)hipEventDestroy(r.second.second);
   }
   for (hipEvent_t e : h->evpool) (void)hipEventDestroy(e);
-  void* bufs[] = {h->pflag, h->gcorn3, h->conn, h->floors, h->rcent, h->emf, h->facc, h->en, h->enpart, h->entot,
+  void* bufs[] = {h->pflag, h->pfix, h->gcorn3, h->conn, h->floors, h->rcent, h->emf, h->facc, h->en, h->enpart, h->entot,
                   h->fx_p0, h->fx_pfl, h->fx_flag, h->ds, h->di};
   for (void* b : bufs) (void)hipFree(b);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -1929,6 +2131,7 @@ extern "C" harm_status harm_init_torus(harm_handle* h, double rin, double rmax, 
   for (int a = 0; a < 4; a++) { G.startx[a] = h->P.startx[a]; G.dx[a] = h->P.dx[a]; }
   if (hi_init_torus(&h->cfg, &G, rin, rmax, kappa, beta, randfact, seed, p.data()) != 0)
     return hh::set_error(HARM_EINVAL, "harm_init_torus: no torus zones on this grid");
+  HH_TRY(reset_pflag(h));  // a new run's flags
   HH_TRY(harm_set_array(h, HARM_ARR_P, p.data()));
   HH_TRY(enq_bound(h, HARM_ARR_P));
   return harm_sync(h);
@@ -1999,11 +2202,11 @@ extern "C" harm_status harm_bound(harm_handle* h, int32_t which)
   return enq_bound(h, which);
 }
 
-extern "C" harm_status harm_post_fixup(harm_handle* h, int32_t which)
+extern "C" harm_status harm_post_fixup(harm_handle* h, int32_t which, int32_t finalstep)
 {
   HH_TRY(check_handle(h));
   HH_TRY(check_which(which));
-  return enq_post_fixup(h, which);
+  return enq_post_fixup(h, which, finalstep != 0);
 }
 
 extern "C" harm_status harm_advance(harm_handle* h, int32_t timeorder, double* ndt)
@@ -2087,9 +2290,12 @@ extern "C" harm_status harm_set_ener_tracking(harm_handle* h, int32_t on)
     if (!h->fx_pfl) HH_TRY(alloc((void**)&h->fx_pfl, sizeof(double) * n));
     if (!h->fx_flag) HH_TRY(alloc((void**)&h->fx_flag, (size_t)h->d.ncell));
   }
-  if (h->ener_on != (on != 0)) drop_step_graph(h);  // the captured step holds the other kernel set
+  const bool was_on = h->ener_on;
+  if (was_on != (on != 0)) drop_step_graph(h);  // the captured step holds the other kernel set
   h->ener_on = on != 0;
-  return harm_reset_ener(h);
+  // switching tracking on starts the sums from zero; a call that leaves it on
+  // keeps them (the sums a restart file restored, for one)
+  return (!was_on && h->ener_on) ? harm_reset_ener(h) : HARM_OK;
 }
 
 extern "C" harm_status harm_reset_ener(harm_handle* h)
@@ -2402,6 +2608,7 @@ extern "C" harm_status harm_read_restart(harm_handle* h, const char* path, doubl
     }
     HH_TRY(harm_set_array(h, half ? HARM_ARR_UCUM : HARM_ARR_P, aos.data()));
   }
+  HH_TRY(reset_pflag(h));  // a restarted run's flags start as a new run's (rdump holds none)
   HH_TRY(enq_bound(h, HARM_ARR_P));
   HH_TRY(hh::check_launch("harm_read_restart: bound"));
   HH_TRY(harm_set_time(h, r.t, r.dt, r.nstep));
@@ -2473,6 +2680,7 @@ extern "C" harm_status harm_init_bondi(harm_handle* h)
   const Dims& d = h->d;
   std::vector<double> p((size_t)HARM_NPR * d.ns1 * d.ns2 * d.ns3, 0.0);
   HH_TRY(harm_host_init_bondi(&h->cfg, p.data()));
+  HH_TRY(reset_pflag(h));  // a new run's flags
   HH_TRY(harm_set_array(h, HARM_ARR_P, p.data()));
   HH_TRY(enq_bound(h, HARM_ARR_P));
   return harm_sync(h);

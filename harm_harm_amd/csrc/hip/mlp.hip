@@ -459,6 +459,11 @@ __global__ __launch_bounds__(kBlock) void k_invert_batch_mlp(const uint16_t* __r
       for (int pl = 0; pl < HARM_NPR; pl++) pz[pl] = pg[pl];
       fl = hd::utoprimgen(Uz, g, gam, gammafail, pz, ns, __builtin_nan(""));
     }
+    if (hd::needs_after_hot(fl, true, false)) {  // the cold retry, as harm_invert_batch
+      double Ugf[HARM_NPR];
+      hd::geometry_free(Uz, g, Ugf);
+      fl = hd::after_hot(fl, Ugf, g, gam, gammafail, pg, pz, ns, true, false);
+    }
 #pragma unroll
     for (int pl = 0; pl < HARM_NPR; pl++) p[pl * n + z] = pz[pl];
     pflag[z] = fl;
@@ -564,7 +569,7 @@ __global__ __launch_bounds__(kBlock) void k_invert_grid_mlp(const uint16_t* __re
     double Uz[HARM_NPR];
 #pragma unroll
     for (int pl = 0; pl < HARM_NPR; pl++) Uz[pl] = U[pl * d.ncell + o];
-    nf = hk::invert_zone<true, DIAG>(d, P, Uz, pg, pf, pflag, gcent, floors, finalstep, o, pp, ns, seed, &fb,
+    nf = hk::invert_zone<true, DIAG>(d, P, Uz, U, pg, pf, pflag, gcent, floors, finalstep, o, pp, ns, seed, &fb,
                                      fx_p0, fx_pfl, fx_flag) > 0;
   }
   hk::newton_counts_to(ns, nf, di);

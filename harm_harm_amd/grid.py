@@ -182,8 +182,8 @@ class Grid:
     def bound(self, which="p"):
         check(lib().harm_bound(self.h, self._which(which)), "harm_bound")
 
-    def post_fixup(self, which="p"):
-        check(lib().harm_post_fixup(self.h, self._which(which)), "harm_post_fixup")
+    def post_fixup(self, which="p", finalstep=True):
+        check(lib().harm_post_fixup(self.h, self._which(which), int(finalstep)), "harm_post_fixup")
 
     def step(self, nsteps=1):
         check(lib().harm_step(self.h, nsteps), "harm_step")

@@ -36,17 +36,23 @@ typedef enum {
   HARM_EFAIL = 5       /* numerical failure that HARM would treat as fatal */
 } harm_status;
 
-/* ---- per-zone inversion flags (global.nondepnmemonics.h:824-844) ---- */
+/* ---- per-zone inversion flags (global.nondepnmemonics.h:823-844) ---- */
 enum {
+  HARM_NANPFLAG = -100,                 /* a flag never set (ghost zones the flag bounds skip) */
+  HARM_UTOPRIMFAILFIXEDCOLD = -3,
+  HARM_UTOPRIMFAILFIXEDUTOPRIM = -1,    /* failure averaging fixed and accounted the zone */
   HARM_UTOPRIMNOFAIL = 0,
   HARM_UTOPRIMFAILCONV = 1,
   HARM_UTOPRIMFAILCONVW = 2,
   HARM_UTOPRIMFAILCONVUTSQ = 3,
   HARM_UTOPRIMFAILCONVGUESSUTSQ = 4,
+  HARM_UTOPRIMFAILCONVBADINVERTCOMPARE = 6,
   HARM_UTOPRIMFAILNANRESULT = 8,
   HARM_UTOPRIMFAILRHONEG = 9,
   HARM_UTOPRIMFAILUNEG = 10,
   HARM_UTOPRIMFAILRHOUNEG = 11,
+  HARM_UTOPRIMFAILU2AVG1 = 14,          /* the cold retry converged: u to be averaged */
+  HARM_UTOPRIMFAILU2AVG2 = 15,
   HARM_UTOPRIMFAILCONVRET = 50
 };
 
@@ -98,6 +104,19 @@ typedef struct harm_config {
   double rhomin, uumin;     /* floors rho=rhomin r^-1.5, u=uumin r^-2.5 (rescaletype 0) */
   double gammamax;          /* limit_gamma ceiling (GAMMAMAX)                 */
   double gammafail;         /* inversion failure Lorentz factor (GAMMAFAIL)   */
+  /* failure handling switches of the problem's header (1 on, 0 off):       */
+  int32_t hot2cold;         /* HOT2COLD: a failed hot inversion is retried with
+                               the cold GRMHD solve (utoprimgen.c:243-262);
+                               torus 1 (init.fishmon.h:325), Bondi 0
+                               (init.sasha.h:379)                            */
+  int32_t checkinv;         /* CHECKONINVERSION with FAILIFBADCHECK: a converged
+                               zone whose U(p) is off by >10% fails
+                               (utoprimgen.c:697-850); torus 0
+                               (init.fishmon.h:225), Bondi 1 (init.sasha.h:198) */
+  int32_t resetregion;      /* USERRESETREGION: fixup_nogood resets the velocity
+                               near the poles and the inner radial edge
+                               (fixup.c:2330); torus 1 (init.fishmon.h:356),
+                               Bondi 0                                       */
 } harm_config;
 
 typedef struct harm_handle harm_handle;
@@ -222,9 +241,15 @@ harm_status harm_invert_grid(harm_handle* h, int32_t which_pb, int32_t which_pf,
  * `which`: x1 outflow (inner inflow check), x2 polar axis, x3 periodic. */
 harm_status harm_bound(harm_handle* h, int32_t which);
 
-/* Replaces post_fixup (fixup.c:53 -> fixup_utoprim :1061): failed zones get
- * the average of good neighbours. */
-harm_status harm_post_fixup(harm_handle* h, int32_t which);
+/* Replaces post_fixup (fixup.c:53 -> fixup_utoprim :1061) on arr `which`:
+ * bounds the flags' first ghost layer (bound_pflag, boundsint.c:18), then each
+ * failed zone gets general_average (fixup.c:1830: the mean of the good
+ * neighbours in the 3^d box, or for the cold retry's u-only mark the least u
+ * over neighbour pairs that both exceed its own), fixup_nogood (:2291) when
+ * there is none, and the Lorentz factor cap; accounted zones end flagged
+ * UTOPRIMFAILFIXEDUTOPRIM (-1; the soft failures only when finalstep != 0,
+ * as the reference counts them). */
+harm_status harm_post_fixup(harm_handle* h, int32_t which, int32_t finalstep);
 
 /* Replaces advance() (advance.c:85) for one RK substep. */
 harm_status harm_advance(harm_handle* h, int32_t timeorder, double* ndt);
@@ -269,7 +294,8 @@ typedef struct harm_ener {
 
 /* Replaces the accumulation side of dump_ener()/diag(): with on != 0 every
  * later harm_step / harm_advance of this handle adds to pdot, pcum, fladd
- * and sourceadd (batch steps do not).  Switching clears the accumulators. */
+ * and sourceadd (batch steps do not).  Switching from off to on clears the
+ * accumulators; a call that finds tracking on already keeps them. */
 harm_status harm_set_ener_tracking(harm_handle* h, int32_t on);
 /* ---- HARM-format dumps (dump.c:256-600, 983-1060; docs/datadesc.txt) ----
  * Main dump: HARM_DUMP_NCOL columns per active zone, rows with i fastest,
@@ -310,9 +336,9 @@ harm_status harm_write_restart(harm_handle* h, const char* path, int32_t binary,
 /* Replaces restart_read(): the grid sizes and the configuration in the
  * header must equal the handle's (HARM_EINVAL naming the field otherwise).
  * Loads the active zones, fills the ghost zones with the problem's boundary
- * conditions, sets t, dt and nstep and restores the energy accumulators
- * (switch tracking on before the call: switching clears them).  tf may be
- * NULL. */
+ * conditions, sets t, dt and nstep, restores the energy accumulators (switch
+ * tracking on before the call: switching on clears them) and starts the
+ * flags as a new run's (rdump holds none).  tf may be NULL. */
 harm_status harm_read_restart(harm_handle* h, const char* path, double* tf);
 
 /* Clears pdot, pcum, fladd, sourceadd and the source-sample clock. */

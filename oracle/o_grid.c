@@ -841,7 +841,7 @@ void o_update(const o_grid* g, const double* pi, const double* pb, const double*
 static void fixup1zone(const o_grid* g, const o_geom* G, double r, double* pr, int32_t* pflag, int* nstroke,
                        double* fl)
 {
-  o_params P = {g->gam, g->gammafail, g->gammamax};
+  o_params P = {g->gam, g->gammafail, g->gammamax, g->hot2cold, g->checkinv};
   double prfloor[2], prnew[NPR], pr0[NPR];
   for (int pl = 0; pl < NPR; pl++) pr0[pl] = pr[pl];
   prfloor[0] = g->rhomin * pow(r, -1.5);
@@ -882,7 +882,7 @@ static void invert(const o_grid* g, const double* pb, const double* U, double* p
   const long ntot = (long)d.ns1 * d.ns2 * d.ns3 * NPR;
   memcpy(pf, pb, sizeof(double) * ntot);
   int64_t ns = 0;
-  o_params P = {g->gam, g->gammafail, g->gammamax};
+  o_params P = {g->gam, g->gammafail, g->gammamax, g->hot2cold, g->checkinv};
   double fls[3 * NPR] = {0};  /* categories 0..2: only FLOOR and GAMMA are reached here */
   const int acc = fl != NULL;
 #pragma omp parallel for collapse(3) reduction(+ : ns, fls[:3 * NPR]) schedule(dynamic, 64) num_threads(g->nthreads > 0 ? g->nthreads : 1)
@@ -901,6 +901,14 @@ static void invert(const o_grid* g, const double* pb, const double* U, double* p
   if (nstroke) *nstroke = ns;
   if (acc)
     for (int c = 0; c < 3 * NPR; c++) fl[c] += fls[c];
+  if (finalstep && g->failcount)
+    for (int k = 0; k < d.n3; k++)
+      for (int j = 0; j < d.n2; j++)
+        for (int i = 0; i < d.n1; i++) {
+          const int f = pflag[((long)(k + d.ng3) * d.ns2 + (j + d.ng2)) * d.ns1 + (i + d.ng1)];
+          if (f > 0) g->failcount[0]++;
+          if (f > 0 && (f < 9 || f > 11) && f != O_FAIL_U2AVG1 && f != O_FAIL_U2AVG2) g->failcount[1]++;
+        }
 }
 
 void o_invert(const o_grid* g, const double* pb, const double* U, double* pf, int32_t* pflag, int finalstep,
@@ -909,62 +917,179 @@ void o_invert(const o_grid* g, const double* pb, const double* U, double* pf, in
   invert(g, pb, U, pf, pflag, finalstep, nstroke, NULL);
 }
 
-/* fixup_utoprim (fixup.c:1061) with the simple_average cascade (:2087); every
- * failure averages rho..U3 (HANDLE*NEG=0); averages read a copy of pf and the
- * original flags; ghost flags count as good. */
-/* fl (final substep, the fladdterms block): each averaged zone's change from
- * its value before the averaging, in the category of its flag
- * (fixuputoprim_accounting, fixup.c:1370, 1595-1720), or NULL */
-static void post_fixup(const o_grid* g, double* pf, const int32_t* pflag, double* fl)
+/* bound_pflag (step_ch.c:1637 -> bound_pflag_user, boundsint.c:18-330):
+ * one ghost layer of flags (numbnd = 1 for BOUNDINTTYPE,
+ * initbase.boundloop.c:83-88), filled in the reference's order:
+ *  1. phi periodic (X3 PERIODIC, :80-92) over i and j in -1..n, which reads
+ *     the i and j ghost cells as the previous call left them;
+ *  2. the outer radial OUTFLOW copy of the last zone (:110-121) over active
+ *     j and k (LOOPX1dir covers the active range, initbase.boundloop.c:
+ *     358-363): the torus's X1UP (init.fishmon.c:247).  The inner radial
+ *     FREEOUTFLOW (init.fishmon.c:248) and both BONDI*OUTFLOW boundaries
+ *     (init.sasha.c:1503-1504) are not handled there, so those ghost flags
+ *     keep what they held: NANPFLAG from set_arrays_multidimen.c:404;
+ *  3. the polar mirror (POLARAXIS, :178-200) over i in -1..n1, active k. */
+void o_bound_pflag(const o_grid* g, int32_t* pflag)
 {
   dims_t d = D(g);
-  const long ntot = (long)d.ns1 * d.ns2 * d.ns3 * NPR;
-  double* pa = (double*)malloc(sizeof(double) * ntot);
-  memcpy(pa, pf, sizeof(double) * ntot);
-  const int e1 = d.n2 > 1;
-#define FL(i, j, k) (((i) < 0 || (i) >= d.n1 || (j) < 0 || (j) >= d.n2) ? 0 : pflag[((long)((k) + d.ng3) * d.ns2 + ((j) + d.ng2)) * d.ns1 + ((i) + d.ng1)])
-#define GOOD(i, j, k) (FL(i, j, k) <= 0)
+#define PF(i, j, k) pflag[((long)((k) + d.ng3) * d.ns2 + ((j) + d.ng2)) * d.ns1 + ((i) + d.ng1)]
+  const int s1 = d.n1 > 1, s2 = d.n2 > 1, s3 = d.n3 > 1;
+  if (s3)
+    for (int j = -s2; j < d.n2 + s2; j++)
+      for (int i = -s1; i < d.n1 + s1; i++) {
+        PF(i, j, -1) = PF(i, j, d.n3 - 1);
+        PF(i, j, d.n3) = PF(i, j, 0);
+      }
+  if (s1 && g->problem == 0)
+    for (int k = 0; k < d.n3; k++)
+      for (int j = 0; j < d.n2; j++) PF(d.n1, j, k) = PF(d.n1 - 1, j, k);
+  if (s2)
+    for (int k = 0; k < d.n3; k++)
+      for (int i = -s1; i < d.n1 + s1; i++) {
+        PF(i, -1, k) = PF(i, 0, k);
+        PF(i, d.n2, k) = PF(i, d.n2 - 1, k);
+      }
+#undef PF
+}
+
+/* IFUTOPRIMNOFAILORFIXED (global.nondepnmemonics.h:854) */
+static inline int good_flag(int f) { return f >= O_FAIL_FIXEDCOLD && f <= 0; }
+
+/* the fix-up category of a failure flag (fixuputoprim_accounting,
+ * fixup.c:1605-1745); the soft failures are counted on the final substep only
+ * (DOCOUNTNEG* = 1, STEPOVERNEG* = NEGDENSITY_NEVERFIXUP), the convergence
+ * failures on every substep; -1 when not accounted */
+static int fail_category(int f, int finalstep)
+{
+  if (f == 9) return finalstep ? O_FL_RHONEG : -1;
+  if (f == 10 || f == O_FAIL_U2AVG1 || f == O_FAIL_U2AVG2) return finalstep ? O_FL_UNEG : -1;
+  if (f == 11) return finalstep ? O_FL_RHOUNEG : -1;
+  return O_FL_CONV;
+}
+
+/* fixup_utoprim (fixup.c:1061) as the torus compiles it: UTOPRIMADJUST =
+ * UTOPRIMAVG, GENERALAVERAGE 1 (:1035), HANDLE*NEG 0 (:1042-1055; so
+ * fixup_negdensities changes nothing, :1457), CAUSALAVG 0 (:1817),
+ * HOWTOAVG_WHEN_U2AVG = MAXUPOSAVG (:1827), TODONOGOOD = NOGOODAVERAGE
+ * (:2278) with USERRESETREGION 1 (init.fishmon.h:356; off for the Bondi
+ * problem, whose header does not set it).  The flags are bounded first
+ * (o_bound_pflag); averages read a copy of pf (ptoavg) and of the flags
+ * (pflagfailorig).  Each failed zone:
+ *  - u-only failures (UTOPRIMFAILU2AVG1/2, the cold retry's mark) take u
+ *    from general_average's second loop: the smallest u over the symmetric
+ *    neighbour pairs whose two u's are both at least the zone's own (:1990);
+ *  - every other failure takes rho..U3 as the mean of the good neighbours in
+ *    the 3^d box, summed pair by pair into two alternating sums (:1960-2030);
+ *  - with no usable neighbour, fixup_nogood (:2291): rho..U3 (u alone for
+ *    u-only failures) become the mean of the 4+4 in-plane neighbours
+ *    (AVG4_1, AVG4_2 at :1812-1813), failed or not; in the reset region
+ *    (j <= 1, j >= n2 - 2 or i < 10) rho and u do and the velocity is set to
+ *    the normal observer's (set_atmosphere, init.tools.c:797, VELREL4: 0);
+ *  - after a velocity average the Lorentz factor is capped at the zone's own
+ *    pre-average value (itself capped at GAMMAMAX) (:1290-1322);
+ *  - accounted zones get UTOPRIMFAILFIXEDUTOPRIM (:1761).
+ * fl (final substep, the fladdterms block): each accounted zone's change from
+ * its value before the averaging, in its category (diag_fixup), or NULL. */
+static void post_fixup(const o_grid* g, double* pf, int32_t* pflag, int finalstep, double* fl)
+{
+  dims_t d = D(g);
+  o_bound_pflag(g, pflag);
+  const long ntot = (long)d.ns1 * d.ns2 * d.ns3;
+  double* pa = (double*)malloc(sizeof(double) * ntot * NPR);
+  int32_t* fo = (int32_t*)malloc(sizeof(int32_t) * ntot);
+  memcpy(pa, pf, sizeof(double) * ntot * NPR);
+  memcpy(fo, pflag, sizeof(int32_t) * ntot);
+  const int s1 = d.n1 > 1, s2 = d.n2 > 1, s3 = d.n3 > 1;
+  const int rnx = s1 ? 3 : 1, rny = s2 ? 3 : 1;
+  int npow = 1;
+  for (int q = 0; q < s1 + s2 + s3; q++) npow *= 3;
+  const int numupairs = (npow - 1) / 2;
+#define FO(i, j, k) fo[((long)((k) + d.ng3) * d.ns2 + ((j) + d.ng2)) * d.ns1 + ((i) + d.ng1)]
+#define A(i, j, k, pl) pa[ix(&d, (i), (j), (k)) + (pl)]
   for (int k = 0; k < d.n3; k++)
     for (int j = 0; j < d.n2; j++)
       for (int i = 0; i < d.n1; i++) {
-        if (FL(i, j, k) <= 0) continue;
-        const int jp = j + e1, jm = j - e1;
+        const int f = FO(i, j, k);
+        int32_t* pfl = &pflag[((long)(k + d.ng3) * d.ns2 + (j + d.ng2)) * d.ns1 + (i + d.ng1)];
+        if (f == O_FAIL_FIXEDCOLD || f == -2) { *pfl = -1; continue; } /* accounting of no change */
+        if (f <= 0) continue;
+        /* u alone for UTOPRIMFAILU2AVG1/2 and UTOPRIMFAILUPERC (:1207); the
+         * min-of-pairs loop for the U2AVG flags only (:1854-1860) */
+        const int urange = (f == O_FAIL_U2AVG1 || f == O_FAIL_U2AVG2 || f == 13);
+        const int uonly = (f == O_FAIL_U2AVG1 || f == O_FAIL_U2AVG2);
+        const int startpl = urange ? O_UU : O_RHO, endpl = urange ? O_UU : O_U3;
         double* p = pf + ix(&d, i, j, k);
-        const double* a = pa;
-        for (int pl = O_RHO; pl <= O_U3; pl++) {
-          double v;
-          if (GOOD(i, jp, k) && GOOD(i, jm, k) && GOOD(i + 1, j, k) && GOOD(i - 1, j, k))
-            v = 0.25 * (a[ix(&d, i, jp, k) + pl] + a[ix(&d, i, jm, k) + pl] + a[ix(&d, i + 1, j, k) + pl] + a[ix(&d, i - 1, j, k) + pl]);
-          else if (GOOD(i + 1, jp, k) && GOOD(i + 1, jm, k) && GOOD(i - 1, jp, k) && GOOD(i - 1, jm, k))
-            v = 0.25 * (a[ix(&d, i + 1, jp, k) + pl] + a[ix(&d, i + 1, jm, k) + pl] + a[ix(&d, i - 1, jp, k) + pl] + a[ix(&d, i - 1, jm, k) + pl]);
-          else if (GOOD(i, jp, k) && GOOD(i, jm, k))
-            v = 0.5 * (a[ix(&d, i, jp, k) + pl] + a[ix(&d, i, jm, k) + pl]);
-          else if (GOOD(i + 1, j, k) && GOOD(i - 1, j, k))
-            v = 0.5 * (a[ix(&d, i + 1, j, k) + pl] + a[ix(&d, i - 1, j, k) + pl]);
-          else if (GOOD(i + 1, jp, k) && GOOD(i - 1, jm, k))
-            v = 0.5 * (a[ix(&d, i + 1, jp, k) + pl] + a[ix(&d, i - 1, jm, k) + pl]);
-          else if (GOOD(i + 1, jm, k) && GOOD(i - 1, jp, k))
-            v = 0.5 * (a[ix(&d, i + 1, jm, k) + pl] + a[ix(&d, i - 1, jp, k) + pl]);
-          else if (GOOD(i + 1, jp, k)) v = a[ix(&d, i + 1, jp, k) + pl];
-          else if (GOOD(i + 1, j, k)) v = a[ix(&d, i + 1, j, k) + pl];
-          else v = a[ix(&d, i, j, k) + pl];
-          p[pl] = v;
+        /* general_average */
+        double mysum[2][NPR], lastmin[NPR];
+        for (int pl = 0; pl < NPR; pl++) { mysum[0][pl] = 0.0; mysum[1][pl] = 0.0; lastmin[pl] = 1E50; }
+        int numavg0 = 0, numavg1 = 0;
+        const double ref = A(i, j, k, O_UU);
+        for (int qq = 0; qq < numupairs; qq++) {
+          const int ii = qq % rnx - s1, jj = (qq % (rnx * rny)) / rnx - s2, kk = qq / (rnx * rny) - s3;
+          const int th = good_flag(FO(i + ii, j + jj, k + kk)), at = good_flag(FO(i - ii, j - jj, k - kk));
+          numavg0 += th + at;
+          if (!uonly) {
+            for (int pl = startpl; pl <= endpl; pl++)
+              mysum[qq % 2][pl] += A(i + ii, j + jj, k + kk, pl) * th + A(i - ii, j - jj, k - kk, pl) * at;
+          } else {
+            for (int pl = startpl; pl <= endpl; pl++) {
+              const double f1 = A(i + ii, j + jj, k + kk, pl), f2 = A(i - ii, j - jj, k - kk, pl);
+              if (f1 >= ref && f2 >= ref) {
+                const double m1 = lastmin[pl] < f1 ? lastmin[pl] : f1;
+                lastmin[pl] = m1 < f2 ? m1 : f2;
+                numavg1++;
+              }
+            }
+          }
         }
+        int numavg;
+        if (!uonly) {
+          if (numavg0 != 0)
+            for (int pl = startpl; pl <= endpl; pl++) p[pl] = (mysum[0][pl] + mysum[1][pl]) / ((double)numavg0);
+          numavg = numavg0;
+        } else {
+          if (numavg1 != 0)
+            for (int pl = startpl; pl <= endpl; pl++) p[pl] = lastmin[pl];
+          numavg = numavg1;
+        }
+        if (f == O_FAIL_U2AVG2) numavg++;
         o_geom G;
         geom_at(g, &d, 0, i, j, &G);
-        limit_gamma(g->gammamax, p, &G);
-        if (fl) {
-          const int f = FL(i, j, k);
-          const int cat = f == 9 ? O_FL_RHONEG : f == 10 ? O_FL_UNEG : f == 11 ? O_FL_RHOUNEG : O_FL_CONV;
-          diag_fixup_acc(g, &G, pa + ix(&d, i, j, k), p, fl + cat * NPR);
+        if (numavg == 0) {
+          /* fixup_nogood */
+          const int ip = i + s1, im = i - s1, jp = j + s2, jm = j - s2;
+#define AVG8(pl) (0.5 * ((0.25 * ((A(i, jp, k, pl) + A(i, jm, k, pl)) + (A(im, j, k, pl) + A(ip, j, k, pl)))) + \
+                         (0.25 * ((A(ip, jp, k, pl) + A(ip, jm, k, pl)) + (A(im, jp, k, pl) + A(im, jm, k, pl))))))
+          const int reset = g->resetregion && ((j <= 1 || j >= d.n2 - 2) || (i < 10));
+          if (reset) {
+            for (int pl = O_RHO; pl <= O_UU; pl++) p[pl] = AVG8(pl);
+            for (int pl = O_U1; pl <= O_U3; pl++) p[pl] = 0.0;
+          } else {
+            for (int pl = startpl; pl <= endpl; pl++) p[pl] = AVG8(pl);
+          }
+#undef AVG8
+        }
+        if (startpl <= O_U1 && endpl >= O_U3) {
+          double gamma = gamma_of(pa + ix(&d, i, j, k), &G);
+          if (gamma > g->gammamax) gamma = g->gammamax;
+          limit_gamma(gamma, p, &G);
+        }
+        const int cat = fail_category(f, finalstep);
+        if (cat >= 0) {
+          if (fl) diag_fixup_acc(g, &G, pa + ix(&d, i, j, k), p, fl + cat * NPR);
+          *pfl = -1;
         }
       }
-#undef GOOD
-#undef FL
+#undef A
+#undef FO
+  free(fo);
   free(pa);
 }
 
-void o_post_fixup(const o_grid* g, double* pf, const int32_t* pflag) { post_fixup(g, pf, pflag, NULL); }
+void o_post_fixup(const o_grid* g, double* pf, int32_t* pflag, int finalstep)
+{
+  post_fixup(g, pf, pflag, finalstep, NULL);
+}
 
 /* ---------------------------------------------------------------------- */
 /* Step driver                                                             */
@@ -1239,7 +1364,7 @@ void o_step(const o_grid* g, double* p, double* pk0, double* pk1, double* ui, do
     const double nd = 1.0 * fmin(wavedt, accdt);
     if (nd < lastndt) lastndt = nd;
     o_bound(g, pff[to]);
-    post_fixup(g, pff[to], pflag, fl);
+    post_fixup(g, pff[to], pflag, finalstep, fl);
     o_bound(g, pff[to]);
   }
   if (en) en[O_EN_NSTEP] += 1.0;

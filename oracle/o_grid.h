@@ -29,6 +29,12 @@ typedef struct o_grid {
   int problem;              /* 0 torus, 1 Bondi: the boundary-condition set */
   double R0;                /* r = R0 + exp(X1) */
   double* ener;             /* energy-file accumulators (O_EN_*), or NULL: o_step adds to them */
+  int hot2cold;             /* HOT2COLD: cold retry of failed hot inversions                 */
+  int checkinv;             /* CHECKONINVERSION with FAILIFBADCHECK                          */
+  int resetregion;          /* USERRESETREGION: fixup_nogood's reset region                  */
+  int64_t* failcount;       /* or NULL: the final substeps' inversions add the zones they
+                               leave failed ([0]) and of those the ones with neither a
+                               converged hot nor a converged cold solve ([1])            */
 } o_grid;
 
 /* Energy-file accumulators (dump_ener.c:391-407): boundary flux sums of the
@@ -56,8 +62,11 @@ void o_update(const o_grid* g, const double* pi, const double* pb, const double*
 /* pf = pb; invert utoinvert into pf over active zones; fixup1zone on finalstep */
 void o_invert(const o_grid* g, const double* pb, const double* U, double* pf, int32_t* pflag, int finalstep,
               int64_t* nstroke);
-/* fixup_utoprim restatement (simple_average cascade) */
-void o_post_fixup(const o_grid* g, double* pf, const int32_t* pflag);
+/* bound_pflag: one ghost layer of the flag array pflag (storage box) */
+void o_bound_pflag(const o_grid* g, int32_t* pflag);
+/* post_fixup -> fixup_utoprim with general_average and fixup_nogood; bounds
+ * the flags first and marks accounted zones UTOPRIMFAILFIXEDUTOPRIM */
+void o_post_fixup(const o_grid* g, double* pf, int32_t* pflag, int finalstep);
 /* one full RK step (step_ch_simplempi + post_advance + post_stepch).
  * Arrays: p (in/out), pk0, pk1, ui, uf, ucum, F1..F3 scratch, pflag.
  * *dt in/out; fluxacc[4] accumulates boundary fluxes (may be NULL). */

@@ -4,11 +4,18 @@
  * (utoprim_jon.c:136) -> Utoprim_new_body (:406) with WHICHHOTINVERTER=3
  * (1-D Newton on W' for E'(W'), func_Eprime_opt :2953), no line search
  * (USE_LINE_SEARCH=-2), PRECISEINVERSION tolerances (u2p_defs.h:20-60),
- * ideal gas (idealgaseos.c:295-410, utoprim_jon_eos.c:652).
+ * ideal gas (idealgaseos.c:295-410, utoprim_jon_eos.c:652); then, as the
+ * torus configuration compiles it (HOT2COLD 1, init.fishmon.h:325), the cold
+ * GRMHD retry of a failed hot solve (trycoldinversion, utoprimgen.c:597-690:
+ * WHICHCOLDINVERTER=2, the P^2[W'] residual Psq_Wp :1923 / dPsqdWp_Wp :1944
+ * with the cold EOS p = 0, coldeos.c), and, where the configuration turns it
+ * on (CHECKONINVERSION, on for the Bondi problem's init.sasha.h:198, off for
+ * the torus's init.fishmon.h:74,225), check_on_inversion with FAILIFBADCHECK
+ * (utoprimgen.c:18, :697-850).
  * TEST INFRASTRUCTURE ONLY (see oracle.h).
  *
- * Not restated: the HOT2COLD / HOT2ENTROPY retries (utoprimgen.c:140-190);
- * a failed hot inversion keeps its flag and is repaired by post_fixup.
+ * Not restated: HOT2ENTROPY (utoprimgen.c:200-226), which needs the entropy
+ * variable that NPR=8 leaves out.
  */
 #include <math.h>
 #include <float.h>
@@ -65,6 +72,9 @@ static int coldgrhd(int* pflag, double Qtsq, double D, double* Wp)
   return 0;
 }
 
+/* which equations the Newton solve works on (eomtype) */
+enum { EOM_HOT = 0, EOM_COLD = 1 };
+
 static double wmrho0_utsq(double Wp, double D, double utsq, double gamma, double gammasq)
 {
   return (Wp - D * utsq / (1.0 + gamma)) / gammasq;
@@ -115,6 +125,33 @@ static void func_Eprime(double Wp, double* dx, double* resid, double* jac, doubl
   *df = -2. * (*f);
 }
 
+/* func_Psq (utoprim_jon.c:3103): the cold GRMHD residual W^2 P^2 of the
+ * momentum equation (Psq_Wp :1923) and its W' derivative (dPsqdWp_Wp :1944);
+ * neither uses the energy equation. */
+static void func_Psq(double Wp, double* dx, double* resid, double* jac, double* f, double* df, const inv_t* s)
+{
+  const double D = s->D, Bsq = s->Bsq, Qtsq = s->Qtsq;
+  const double W = Wp + D;
+  const double Y = Bsq + 2.0 * W;
+  const double W2 = W * W;
+  const double S = s->QdotB;
+  const double r = -Qtsq * W2 + Wp * (D + W) * (W2 + Bsq * Y) - Y * S * S;
+  const double drdW = 2.0 * (D * ((Bsq + D) * (Bsq + D) - Qtsq) - (S * S) + ((Bsq + D) * (Bsq + 5.0 * D) - Qtsq) * Wp +
+                             3.0 * (Bsq + 2.0 * D) * (Wp * Wp) + 2.0 * (Wp * Wp * Wp));
+  *resid = r;
+  *jac = drdW;
+  *dx = -r / drdW;
+  *f = 0.5 * r * r;
+  *df = -2. * (*f);
+}
+
+static void residual(int eom, double Wp, double* dx, double* resid, double* jac, double* f, double* df,
+                     const inv_t* s)
+{
+  if (eom == EOM_COLD) func_Psq(Wp, dx, resid, jac, f, df, s);
+  else func_Eprime(Wp, dx, resid, jac, f, df, s);
+}
+
 /* newt_errorcheck / newt_extracheck, utoprim_jon.c:4488-4520 */
 static int ultrarel(double x0, const inv_t* s) { return (fabs(s->w[1]) > s->w[2]) || (fabs(x0) > s->w[2]); }
 static int errcheck(double errx, double x0, const inv_t* s)
@@ -137,7 +174,7 @@ static void validate_Wp(double Wpold, double* Wpnew, const inv_t* s)
 
 /* general_newton_raphson (utoprim_jon.c:3853), n=1, no line search,
  * JONGENNEWTSTUFF=1, NEWCONVERGE=1.  Returns retval; x updated. */
-static int newton(int* pflag, double* xio, const inv_t* s, int* nstroke)
+static int newton(int* pflag, double* xio, const inv_t* s, int* nstroke, int eom)
 {
   double x = *xio, x_old, x_older, dx, dx_old, resid, jac, f, df;
   double errx = 1.;
@@ -150,7 +187,7 @@ static int newton(int* pflag, double* xio, const inv_t* s, int* nstroke)
     (*nstroke)++;
     lntries++;
     if (lntries > ITERDAMPSTART && diddamp == 0) damp = 0.5;
-    func_Eprime(x, &dx, &resid, &jac, &f, &df, s);
+    residual(eom, x, &dx, &resid, &jac, &f, &df, s);
     errx = 0.;
     x_older = x_old;
     x_old = x;
@@ -167,7 +204,7 @@ static int newton(int* pflag, double* xio, const inv_t* s, int* nstroke)
         diddamp = 1;
       }
       if (diddamp) {
-        func_Eprime(x, &dx, &resid, &jac, &f, &df, s);
+        residual(eom, x, &dx, &resid, &jac, &f, &df, s);
         if (resid <= -VERYBIG || jac == 0.0) { *xio = x; return 1; }
         x_older = x;
         x_old = x;
@@ -200,8 +237,10 @@ static int newton(int* pflag, double* xio, const inv_t* s, int* nstroke)
 }
 
 /* Utoprim_new_body (utoprim_jon.c:406): U and prim are the alpha-scaled
- * U_tmp / prim_tmp of the caller. */
-static int new_body(int* pflag, const double* U, const o_geom* g, const o_params* P, double* prim, int* nstroke)
+ * U_tmp / prim_tmp of the caller; eom EOM_COLD uses the cold EOS (pressure 0,
+ * pickeos_eomtype eos.c:259) in the guess and the result. */
+static int new_body(int* pflag, const double* U, const o_geom* g, const o_params* P, double* prim, int* nstroke,
+                    int eom)
 {
   inv_t s;
   double Bcon[4], Bcov[4], Qcovp[4], Qconp[4], Qtcon[4], Qtcov[4], Qcov[4], Qcon[4], ncov[4], ncon[4];
@@ -252,7 +291,7 @@ static int new_body(int* pflag, const double* U, const o_geom* g, const o_params
     const double gamma = sqrt(gammasq);
     const double rho0 = (s.D) / gamma;
     const double u = prim[O_UU];
-    const double p = (P->gam - 1.0) * u;
+    const double p = eom == EOM_COLD ? 0.0 : (P->gam - 1.0) * u;
     double BBcon[4], BBcov[4], bsq = 0.0;
     BBcon[0] = 0.0;
     for (int jj = 1; jj < 4; jj++) BBcon[jj] = prim[O_B1 - 1 + jj];
@@ -299,7 +338,7 @@ static int new_body(int* pflag, const double* U, const o_geom* g, const o_params
 
   /* find_root_1D_gen_Eprime, :3665 */
   double Wp = Wp_last;
-  retval += newton(pflag, &Wp, &s, nstroke);
+  retval += newton(pflag, &Wp, &s, nstroke, eom);
   if (retval != 0) return retval;
 
   /* check_Wp, :1188: every branch returns retval (= 0 here), so it only
@@ -323,15 +362,15 @@ static int new_body(int* pflag, const double* U, const o_geom* g, const o_params
     const double gamma = sqrt(gammasq);
     const double wmrho0 = wmrho0_utsq(Wp, s.D, utsq, gamma, gammasq);
     const double rho0 = s.D / gamma;
-    const double p = ((P->gam - 1.0) / P->gam) * wmrho0;
+    const double p = eom == EOM_COLD ? 0.0 : ((P->gam - 1.0) / P->gam) * wmrho0;
     const double u = wmrho0 - p;
-    if ((rho0 <= 0.) || (p < 0.)) {
+    if ((rho0 <= 0.) || ((p < 0.) && eom != EOM_COLD)) {
       if ((rho0 <= 0.) && (p >= 0.)) *pflag = 9;
       if ((rho0 > 0.) && (p < 0.)) *pflag = 10;
       if ((rho0 <= 0.) && (p < 0.)) *pflag = 11;
     }
     prim[O_RHO] = rho0;
-    prim[O_UU] = u;
+    prim[O_UU] = eom == EOM_COLD ? 0.0 : u;
     for (int i = 1; i < 4; i++) prim[O_U1 + i - 1] = gamma / (W + s.Bsq) * (Qtcon[i] + s.QdotB * Bcon[i] / W);
     int foundnan = 0;
     for (int pl = 0; pl < O_NPR; pl++)
@@ -347,7 +386,7 @@ static int new_body(int* pflag, const double* U, const o_geom* g, const o_params
 
 /* Utoprim_jon_nonrelcompat_inputnorestmass (utoprim_jon.c:136) on
  * geometry-free U. */
-static int jon(const double* Ugf, const o_geom* g, const o_params* P, double* pr, int* nstroke)
+static int jon(const double* Ugf, const o_geom* g, const o_params* P, double* pr, int* nstroke, int eom)
 {
   double Utmp[O_NPR], ptmp[O_NPR];
   int pflag = 0;
@@ -356,7 +395,7 @@ static int jon(const double* Ugf, const o_geom* g, const o_params* P, double* pr
   for (int i = 0; i < O_NPR; i++) Utmp[i] = alpha * Ugf[i];
   for (int i = 0; i < O_NPR; i++) ptmp[i] = pr[i];
   for (int i = O_B1; i <= O_B3; i++) ptmp[i] = alpha * pr[i];
-  const int ret = new_body(&pflag, Utmp, g, P, ptmp, nstroke);
+  const int ret = new_body(&pflag, Utmp, g, P, ptmp, nstroke, eom);
   if (ret == 0) {
     for (int i = 0; i <= O_U3; i++) pr[i] = ptmp[i];
   } else {
@@ -365,9 +404,81 @@ static int jon(const double* Ugf, const o_geom* g, const o_params* P, double* pr
   return pflag;
 }
 
+#define THIRD_L 0.3333333333333333333333333333333333L /* THIRD, global.nondepnmemonics.h:746 (long double) */
+#define CHECKONINVFRACFAIL 1E-1
+
+/* check_on_inversion (utoprimgen.c:697-850) after the hot solve with
+ * FAILIFBADCHECK=1 (:18): a converged zone whose U(p) differs from the input
+ * by more than 10% in any of rho, u, the momenta or the fields (each against
+ * its own norm, momenta made comparable through u^i/u^t) gets
+ * UTOPRIMFAILCONVBADINVERTCOMPARE.  Uold: the geometry-free input. */
+static int check_on_inversion(int flag, const double* Uold, const double* pr, const o_geom* g, const o_params* P)
+{
+  if (flag != 0) return flag;
+  o_state q;
+  o_geom G1 = *g;
+  G1.gdet = 1.0; /* primtoU(UNOTHING) */
+  double Unew[O_NPR], on[O_NPR], nn[O_NPR], fdiff[O_NPR];
+  o_get_state(pr, g, P->gam, &q);
+  o_primtoflux(pr, &q, 0, &G1, Unew);
+  for (int pl = 0; pl < O_NPR; pl++) { on[pl] = Uold[pl]; nn[pl] = Unew[pl]; }
+  for (int jj = 1; jj <= 3; jj++) {  /* REMOVERESTMASSFROMUU==2 */
+    on[O_UU + jj] = Uold[O_UU + jj] * (q.ucon[jj] / q.ucon[0]);
+    nn[O_UU + jj] = Unew[O_UU + jj] * (q.ucon[jj] / q.ucon[0]);
+  }
+  for (int pl = 0; pl < O_NPR; pl++) {
+    fdiff[pl] = 0.0;
+    if (pl == O_RHO || pl == O_UU) {
+      fdiff[pl] = fabs(nn[pl] - on[pl]) / (fabs(nn[pl]) + fabs(on[pl]) + SMALL);
+    } else if (pl >= O_U1 && pl <= O_U3) {
+      /* the norm sums the U2 terms twice and leaves U1 out, as the reference does */
+      double errornorm = (double)(THIRD_L * (long double)(fabs(nn[O_U2]) + fabs(on[O_U2]) + fabs(nn[O_U2]) + fabs(on[O_U2]) +
+                                                         fabs(nn[O_U3]) + fabs(on[O_U3])));
+      errornorm = dmax(errornorm, 0.5 * (fabs(on[O_UU]) + fabs(nn[O_UU])));
+      fdiff[pl] = fabs(nn[pl] - on[pl]) / (errornorm + SMALL);
+    } else {
+      /* long double throughout, as THIRD makes the reference's expression */
+      fdiff[pl] = (double)((long double)fabs(nn[pl] - on[pl]) /
+                           (THIRD_L * (long double)(fabs(nn[O_B1]) + fabs(on[O_B1]) + fabs(nn[O_B2]) + fabs(on[O_B2]) +
+                                                    fabs(nn[O_B3]) + fabs(on[O_B3])) + (long double)SMALL));
+    }
+  }
+  int bad = 0;
+  for (int pl = 0; pl < O_NPR; pl++)
+    if (fdiff[pl] > CHECKONINVFRACFAIL && fabs(on[pl]) > SMALL && fabs(nn[pl]) > SMALL) bad++;
+  return bad ? O_FAIL_BADINVERTCOMPARE : flag;
+}
+
+/* Utoprimgen's EOMGRMHD branch (utoprimgen.c:188-263) on geometry-free U:
+ * the hot solve from the guess pr0, its check, and on any failure the cold
+ * solve from the same guess (trycoldinversion :597, every failure type
+ * retried: USECOLDIFHOT* = 1 :587-591).  A converged cold solve keeps the
+ * cold density and velocities, puts u back to the guess's and flags the zone
+ * UTOPRIMFAILU2AVG1 so that failure averaging recomputes u alone (:655-660);
+ * otherwise the hot result and flag stand.  Newton strokes of both solves add
+ * to *nstroke. */
+static int utoprimgen_gf(const double* Ugf, const o_geom* g, const o_params* P, double* pr, int* nstroke)
+{
+  double pr0[O_NPR];
+  for (int pl = 0; pl < O_NPR; pl++) pr0[pl] = pr[pl];
+  int fl = jon(Ugf, g, P, pr, nstroke, EOM_HOT);
+  if (P->checkinv) fl = check_on_inversion(fl, Ugf, pr, g, P);
+  if (P->hot2cold && fl > 0) {
+    double prcold[O_NPR];
+    for (int pl = 0; pl < O_NPR; pl++) prcold[pl] = pr0[pl];
+    const int cf = jon(Ugf, g, P, prcold, nstroke, EOM_COLD);
+    if (cf >= O_FAIL_FIXEDCOLD && cf <= 0) { /* IFUTOPRIMNOFAILORFIXED */
+      for (int pl = 0; pl < O_NPR; pl++) pr[pl] = prcold[pl];
+      pr[O_UU] = pr0[O_UU];
+      fl = O_FAIL_U2AVG1;
+    }
+  }
+  return fl;
+}
+
 int o_utoprimgen_unothing(const double* Ugf, const o_geom* g, const o_params* P, double* pr, int* nstroke)
 {
-  return jon(Ugf, g, P, pr, nstroke);
+  return utoprimgen_gf(Ugf, g, P, pr, nstroke);
 }
 
 int o_utoprimgen(const double* U, const o_geom* g, const o_params* P, double* pr, int* nstroke)
@@ -376,13 +487,30 @@ int o_utoprimgen(const double* U, const o_geom* g, const o_params* P, double* pr
   /* UtoU(UEVOLVE -> UNOTHING): x igdetnosing = sign(g)/(|g|+SMALL) */
   const double ig = (g->gdet >= 0.0 ? 1.0 : -1.0) / (fabs(g->gdet) + SMALL);
   for (int pl = 0; pl < O_NPR; pl++) Ugf[pl] = U[pl] * ig;
-  return jon(Ugf, g, P, pr, nstroke);
+  return utoprimgen_gf(Ugf, g, P, pr, nstroke);
+}
+
+/* check_on_inversion alone, for the tests: geomrec is one 27-field record */
+int o_check_inversion(int flag, const double* Ugf, const double* pr, const double* geomrec, double gam)
+{
+  o_geom g;
+  o_geom_from_rec(geomrec, &g);
+  o_params P = {gam, 100.0, 1.0, 0, 1};
+  return check_on_inversion(flag, Ugf, pr, &g, &P);
+}
+
+/* one solve of the given equations alone (no retry, no check): the building
+ * block above, exposed so the tests can pin the cold solve by itself */
+int o_jon_solve(const double* Ugf, const o_geom* g, double gam, double gammafail, int cold, double* pr, int* nstroke)
+{
+  o_params P = {gam, gammafail, gammafail / 100.0, 0, 0};
+  return jon(Ugf, g, &P, pr, nstroke, cold ? EOM_COLD : EOM_HOT);
 }
 
 void o_invert_batch(const double* U, const double* geomrec, double* p, int32_t* pflag, int32_t* niter,
                     int64_t n, double gam, double gammafail, int nthreads)
 {
-  o_params P = {gam, gammafail, gammafail / 100.0};
+  o_params P = {gam, gammafail, gammafail / 100.0, 1, 0}; /* the torus configuration: HOT2COLD on */
   (void)nthreads;
 #pragma omp parallel for schedule(dynamic, 1024) num_threads(nthreads > 0 ? nthreads : 1)
   for (int64_t z = 0; z < n; z++) {

@@ -46,7 +46,13 @@ typedef struct o_params {
   double gam;        /* adiabatic index                 */
   double gammafail;  /* GAMMAFAIL (inversion)           */
   double gammamax;   /* GAMMAMAX (limit_gamma)          */
+  int hot2cold;      /* HOT2COLD: cold retry of a failed hot solve      */
+  int checkinv;      /* CHECKONINVERSION with FAILIFBADCHECK           */
 } o_params;
+
+/* inversion flags beyond 0..11 (global.nondepnmemonics.h:824-844) */
+enum { O_FAIL_FIXEDCOLD = -3, O_FAIL_BADINVERTCOMPARE = 6, O_FAIL_U2AVG1 = 14, O_FAIL_U2AVG2 = 15,
+       O_NANPFLAG = -100 };
 
 void o_geom_from_rec(const double* rec, o_geom* g);
 
@@ -67,6 +73,12 @@ void o_source(const double* pr, const o_geom* g, const o_state* q, const double*
 int o_utoprimgen(const double* U, const o_geom* g, const o_params* P, double* pr, int* nstroke);
 /* same with geometry-free U (inputtype UNOTHING), as fixup1zone uses */
 int o_utoprimgen_unothing(const double* Ugf, const o_geom* g, const o_params* P, double* pr, int* nstroke);
+/* one hot (cold = 0) or cold (cold = 1) solve of geometry-free U from the
+ * guess pr, without the retry or the check */
+int o_jon_solve(const double* Ugf, const o_geom* g, double gam, double gammafail, int cold, double* pr, int* nstroke);
+/* check_on_inversion (utoprimgen.c:697) of a solution pr of geometry-free Ugf
+ * at one 27-field geometry record; returns the (possibly changed) flag */
+int o_check_inversion(int flag, const double* Ugf, const double* pr, const double* geomrec, double gam);
 
 /* batched: SoA like harm_invert_batch */
 void o_invert_batch(const double* U, const double* geomrec, double* p, int32_t* pflag, int32_t* niter,

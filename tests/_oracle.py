@@ -131,6 +131,8 @@ class OGridStruct(ctypes.Structure):
         ("geom", ctypes.c_void_p * 4), ("gdet_corn3", ctypes.c_void_p), ("conn", ctypes.c_void_p),
         ("rcent", ctypes.c_void_p), ("problem", ctypes.c_int), ("R0", ctypes.c_double),
         ("ener", ctypes.c_void_p),
+        ("hot2cold", ctypes.c_int), ("checkinv", ctypes.c_int), ("resetregion", ctypes.c_int),
+        ("failcount", ctypes.c_void_p),
     ]
 
 
@@ -144,7 +146,9 @@ def _grid_protos(L):
     L.o_fluxcalc.argtypes = [P, _P, _P, _P, _P, _P, _I]
     L.o_update.argtypes = [P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _D, _I, _P]
     L.o_invert.argtypes = [P, _P, _P, _P, _P, _I, _P]
-    L.o_post_fixup.argtypes = [P, _P, _P]
+    L.o_post_fixup.argtypes = [P, _P, _P, _I]
+    L.o_bound_pflag.argtypes = [P, _P]
+    L.o_bound_pflag.restype = None
     L.o_step.argtypes = [P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]
     L.o_ener_totals.argtypes = [P, _P, _P]
     L.o_ener_totals.restype = None
@@ -195,6 +199,9 @@ class OGrid:
         g.rcent = self.rc.ctypes.data
         g.problem, g.R0 = cfg.problem, cfg.R0
         g.ener = None
+        g.hot2cold, g.checkinv, g.resetregion = cfg.hot2cold, cfg.checkinv, cfg.resetregion
+        self.failcount = np.zeros(2, dtype=np.int64)
+        g.failcount = self.failcount.ctypes.data
         self.en = None
         self.g = g
         self.dx = [dx[a] for a in range(4)]
@@ -247,8 +254,18 @@ class OGrid:
                        int(finalstep), ctypes.byref(ns))
         return ns.value
 
-    def post_fixup(self, pf, pflag):
-        lib().o_post_fixup(ctypes.byref(self.g), pf.ctypes.data, pflag.ctypes.data)
+    def post_fixup(self, pf, pflag, finalstep=True):
+        """bounds the flags (one ghost layer), averages the failed zones of pf
+        and marks the accounted ones -1, in place"""
+        lib().o_post_fixup(ctypes.byref(self.g), pf.ctypes.data, pflag.ctypes.data, int(finalstep))
+
+    def bound_pflag(self, pflag):
+        lib().o_bound_pflag(ctypes.byref(self.g), pflag.ctypes.data)
+
+    def new_pflag(self):
+        """a flag array over the storage box as a run starts: NANPFLAG (-100)
+        everywhere (set_arrays_multidimen.c:404)"""
+        return np.full(self.shape[:3], -100, dtype=np.int32)
 
     def rk_tables(self):
         CUf = np.zeros((4, 4))
@@ -301,5 +318,5 @@ class OGrid:
     def new_state(self, p):
         st = {k: self.zeros() for k in ("pk0", "pk1", "ui", "uf", "ucum", "F1", "F2", "F3")}
         st["p"] = p
-        st["pflag"] = np.zeros(self.shape[:3], dtype=np.int32)
+        st["pflag"] = self.new_pflag()
         return st

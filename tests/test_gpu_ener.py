@@ -53,12 +53,12 @@ def _run_pair(cfg, p, steps, nstep0=0):
     g.set_ener_tracking(True)
     og.track_ener(nstep0)
     st = og.new_state(p.copy())
-    nfail = nconv = 0
+    og.failcount[:] = 0
     for _ in range(steps):
         dt = og.step(st, dt)
-        f = st["pflag"]
-        nfail += int((f > 0).sum())
-        nconv += int(((f > 0) & ((f < 9) | (f > 11))).sum())  # not the rho<0 / u<0 kinds
+    # final-substep zones left failed by the inversion, and of those the ones
+    # whose hot and cold solves both failed (no rho < 0 / u < 0 kinds)
+    nfail, nconv = (int(x) for x in og.failcount)
     g.step(steps)
     return og, g, st, (nfail, nconv)
 
@@ -75,7 +75,8 @@ def test_ener_matches_oracle(case):
     assert ed["nstep"] == c["steps"]
     for k in ("U", "pdot", "pcum", "sourceadd"):
         _close(k, ed[k], eo[k])
-    # A zone whose Newton solve did not converge keeps its last iterate, and
+    # A zone whose hot and cold Newton solves both failed keeps its last
+    # hot iterate, and
     # the final substep's floors (fixup1zone runs on every zone,
     # advance.c:797-801) and failure averaging (category 0) account U of it.
     # U of a wild iterate amplifies the last-bit differences of its input

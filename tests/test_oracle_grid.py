@@ -360,36 +360,3 @@ def test_radial_extrapolation_limits_inward_growth():
     edge = p[0, J, og.ng1, 0]
     for i in range(-og.ng1, 0):
         assert np.allclose(p[0, J, i + og.ng1, 0], edge * 1.1 ** (-i), rtol=1e-13)
-
-
-def test_failure_averaging_reproduces_linear_fields():
-    """A failed zone takes the average of good neighbours: for a field linear
-    in (i, j) every symmetric choice of neighbours returns the zone's own
-    value, so the result pins which neighbours are used; a zone whose eight
-    neighbours all failed keeps its value."""
-    cfg, og = grid((32, 24, 1), recon="para", flux="laxf", timeorder=2)
-    p = og.zeros()
-    I, Jg = np.meshgrid(np.arange(og.ns1) - og.ng1, np.arange(og.ns2) - og.ng2)
-    lin = {0: 1.0 + 0.01 * I + 0.02 * Jg, 1: 0.1 + 0.001 * I - 0.002 * Jg, 2: 1e-3 * (1 + 0.01 * I),
-           3: 1e-3 * (1 - 0.02 * Jg), 4: 1e-4 * (1 + 0.03 * I + 0.01 * Jg)}
-    for v, f in lin.items():
-        p[0, :, :, v] = f
-    pflag = np.zeros(og.shape[:3], dtype=np.int32)
-    marks = [(5, 5), (10, 7), (11, 7), (20, 12), (20, 13)]  # isolated, radial pair, polar pair
-    block = [(ii, jj) for ii in range(14, 17) for jj in range(16, 19)]  # 3x3 failed block
-    for (i, j) in marks + block:
-        pflag[0, j + og.ng2, i + og.ng1] = 1
-    before = p.copy()
-    for (i, j) in marks + block:  # scramble the failed zones
-        p[0, j + og.ng2, i + og.ng1, :5] = -7.0
-    centre = p[0, 17 + og.ng2, 15 + og.ng1, :5].copy()
-    og.post_fixup(p, pflag)
-    for (i, j) in marks:
-        got = p[0, j + og.ng2, i + og.ng1, :5]
-        want = before[0, j + og.ng2, i + og.ng1, :5]
-        assert np.allclose(got, want, rtol=1e-13, atol=1e-16), ((i, j), got, want)
-    # the centre keeps its own values; the Lorentz-factor cap then scales its
-    # (scrambled, superluminal) velocity down along the same direction
-    got = p[0, 17 + og.ng2, 15 + og.ng1, :5]
-    assert np.array_equal(got[:2], centre[:2])
-    assert got[2] == got[3] == got[4] and -7.0 < got[2] < 0.0
