@@ -1,13 +1,14 @@
 /*
  * o_grid.c -- oracle restatement of HARM's advance() hot path on a grid:
- * boundary conditions, reconstruction (PARA, WENO5 core, donor), face Riemann
+ * boundary conditions, reconstruction (PARA, donor; WENO5BND lines in
+ * o_weno.c), face Riemann
  * fluxes (LAX-F, HLL), Toth flux-CT, the conserved update with connection
  * source terms, the inversion with floors, failure averaging, the timestep
  * and the RK2/RK3 step driver.  TEST INFRASTRUCTURE ONLY (see oracle.h).
  *
  * Reference files restated (HARM tree): flux.c:852-1140 (fluxcalc_standard),
  * flux.c:1952-2110 (getplpr, check_plpr), interppoint.para.c:568-1200 (para4),
- * reconstructeno.c:1693-2030, 2211-2330, 3015 (WENO weights), fluxcompute.c
+ * interpline.c:306-640 (line extraction for WENO5BND), fluxcompute.c
  * (flux_compute, HLL/LAXF), wavespeeds.c:397 (cminmax_calc), fluxct.c:186-453,
  * 1106-1161 (Toth), advance.c:140-865, 1866-2075, 2331-2620 (update, dt),
  * fixup.c:561-830, 1061-1330, 2087-2200, 2723-2850, 3255-3390, bounds.tools.c
@@ -485,79 +486,9 @@ static void para4(const double* y, double* lout, double* rout)
   *rout = r;
 }
 
-/* symmetric pairwise sum (get_sum_of_elements, SYMMETRIZE_WENO) */
-static double symsum(int n, const double* a)
-{
-  double s = 0.0;
-  int i;
-  for (i = 0; i < n / 2; i++) s += a[i] + a[n - 1 - i];
-  s += a[i] * (n % 2);
-  return s;
-}
-
-/* limit_weight (reconstructeno.c:3015) for order 3 */
-static double limit_weight3(double w)
-{
-  const double avg = 1. / 3.;
-  const double w1 = 0.2 * avg, w2 = (1.0 - 0.2) * avg, w3 = (1.0 + 0.2) * avg, w4 = 1.0;
-  if (w < w1) return 0.0;
-  if (w < w2) return avg * (w - w1) / (w2 - w1);
-  if (w < w3) return avg;
-  return (w - w3) * (w4 - avg) / (w4 - w3) + avg;
-}
-
-/* WENO5 c2e core: Jiang-Shu indicators on the three 3-point sub-stencils,
- * value-proportional epsilon, normalised indicators, limited weights, point
- * value sub-stencil interpolants at x -+ 1/2. */
-static void weno5(const double* u, double* lout, double* rout)
-{
-  const double EPS = (500.0 * DBL_EPSILON) * (500.0 * DBL_EPSILON);
-  double b[3], d1[3], d2[3];
-  d1[0] = -(3 * u[0] + u[2]) + 4 * u[1];
-  d1[1] = u[1] - u[-1];
-  d1[2] = (3 * u[0] + u[-2]) - 4 * u[-1];
-  d2[0] = (u[0] + u[2]) - 2 * u[1];
-  d2[1] = (u[-1] + u[1]) - 2 * u[0];
-  d2[2] = (u[0] + u[-2]) - 2 * u[-1];
-  for (int s = 0; s < 3; s++) b[s] = 13. / 12. * d2[s] * d2[s] + 1. / 4. * d1[s] * d1[s];
-  double u2[5];
-  for (int i = -2; i <= 2; i++) u2[i + 2] = u[i] * u[i];
-  const double err = symsum(5, u2) * EPS;
-  for (int s = 0; s < 3; s++) b[s] += err;
-  double sb = symsum(3, b);
-  for (int s = 0; s < 3; s++) b[s] /= (sb + DBL_MIN);
-  double w[3];
-  for (int s = 0; s < 3; s++) {
-    double t = b[s] + EPS / 1.;
-    t *= t;
-    w[s] = 1. / t;
-  }
-  double sw = symsum(3, w);
-  for (int s = 0; s < 3; s++) w[s] /= (sw + 0);
-  for (int s = 0; s < 3; s++) w[s] = limit_weight3(w[s]);
-  sw = symsum(3, w);
-  for (int s = 0; s < 3; s++) w[s] /= (sw + 0);
-  /* optimal weights: right face {5/16, 5/8, 1/16}, left face mirrored */
-  double wr[3] = {w[0] * (5.0 / 16.0), w[1] * (5.0 / 8.0), w[2] * (1.0 / 16.0)};
-  double wl[3] = {w[0] * (1.0 / 16.0), w[1] * (5.0 / 8.0), w[2] * (5.0 / 16.0)};
-  sw = symsum(3, wr);
-  for (int s = 0; s < 3; s++) wr[s] /= (sw + DBL_MIN);
-  sw = symsum(3, wl);
-  for (int s = 0; s < 3; s++) wl[s] /= (sw + DBL_MIN);
-  const double qr0 = (3 * u[0] + 6 * u[1] - u[2]) / 8.0;
-  const double qr1 = (-u[-1] + 6 * u[0] + 3 * u[1]) / 8.0;
-  const double qr2 = (3 * u[-2] - 10 * u[-1] + 15 * u[0]) / 8.0;
-  const double ql0 = (15 * u[0] - 10 * u[1] + 3 * u[2]) / 8.0;
-  const double ql1 = (3 * u[-1] + 6 * u[0] - u[1]) / 8.0;
-  const double ql2 = (-u[-2] + 6 * u[-1] + 3 * u[0]) / 8.0;
-  *rout = (wr[0] * qr0 + wr[2] * qr2) + wr[1] * qr1;
-  *lout = (wl[0] * ql0 + wl[2] * ql2) + wl[1] * ql1;
-}
-
 static void recon_cell(int recon, const double* y, double* l, double* r)
 {
   if (recon == 2) para4(y, l, r);
-  else if (recon == 3) weno5(y, l, r);
   else { *l = y[0]; *r = y[0]; }
 }
 
@@ -613,6 +544,83 @@ void o_face_flux(int flux, int dir, double gam, double gammamax, const double* r
   face_flux(&g, dir, &G, p_l, p_r, F, ctop);
 }
 
+/* The face fluxes of one direction with WENO5BND: the reference reconstructs
+ * whole storage lines along dir (slope_lim_linetype_c2e interpline.c:306,
+ * line bs..be = the storage row, cells -1..N reconstructed), one transverse
+ * position at a time; the face loop then takes p_l from the right edge of
+ * the cell before the face and p_r from the left edge of the cell after it
+ * (getplpr flux.c:1952). */
+static void weno_flux_dir(const o_grid* g, const dims_t* d, int dir, const double* pb, double* F, double* ndt)
+{
+  const int e0 = d->n1 > 1, e1 = d->n2 > 1, e2 = d->n3 > 1;
+  const int nd = dir == 1 ? d->n1 : dir == 2 ? d->n2 : d->n3;
+  const int ngd = dir == 1 ? d->ng1 : dir == 2 ? d->ng2 : d->ng3;
+  const int n = nd + 2 * ngd;
+  /* transverse positions: the face loop's range (active, one more layer
+   * each side for the corner EMFs) */
+  int as = 0, ae = 0, bs2 = 0, be2 = 0;
+  if (dir == 1) { as = -e1; ae = d->n2 - 1 + e1; bs2 = -e2; be2 = d->n3 - 1 + e2; }
+  if (dir == 2) { as = -e0; ae = d->n1 - 1 + e0; bs2 = -e2; be2 = d->n3 - 1 + e2; }
+  if (dir == 3) { as = -e0; ae = d->n1 - 1 + e0; bs2 = -e1; be2 = d->n2 - 1 + e1; }
+  double mindt = BIG;
+#pragma omp parallel for collapse(2) reduction(min : mindt) num_threads(g->nthreads > 0 ? g->nthreads : 1)
+  for (int b = bs2; b <= be2; b++)
+    for (int a = as; a <= ae; a++) {
+      double* y = (double*)malloc(sizeof(double) * (size_t)n * 13 + sizeof(double) * (size_t)n * NPR * 2);
+      double* st = y + n;         /* 4 per cell */
+      double* dP = st + 4 * n;    /* n */
+      double* yl1 = dP + n;       /* n */
+      double* yr1 = yl1 + n;      /* n */
+      double* yl = yr1 + n + n;   /* n * NPR */
+      double* yr = yl + (size_t)n * NPR;
+      int ci[3] = {0, 0, 0};
+      for (int c = 0; c < n; c++) {
+        const int m = c - ngd;
+        if (dir == 1) { ci[0] = m; ci[1] = a; ci[2] = b; }
+        if (dir == 2) { ci[0] = a; ci[1] = m; ci[2] = b; }
+        if (dir == 3) { ci[0] = a; ci[1] = b; ci[2] = m; }
+        o_geom G;
+        geom_at(g, d, 0, ci[0], ci[1], &G);
+        o_dp_cell(pb + ix(d, ci[0], ci[1], ci[2]), &G, g->gam, dir, st + 4 * c);
+      }
+      o_dp_line(st, n, dP);
+      for (int pl = 0; pl < NPR; pl++) {
+        for (int c = 0; c < n; c++) {
+          const int m = c - ngd;
+          if (dir == 1) { ci[0] = m; ci[1] = a; ci[2] = b; }
+          if (dir == 2) { ci[0] = a; ci[1] = m; ci[2] = b; }
+          if (dir == 3) { ci[0] = a; ci[1] = b; ci[2] = m; }
+          y[c] = pb[ix(d, ci[0], ci[1], ci[2]) + pl];
+        }
+        o_weno5bnd_line(y, n, ngd, dP, yl1, yr1);
+        for (int c = 0; c < n; c++) {
+          yl[(size_t)c * NPR + pl] = yl1[c];
+          yr[(size_t)c * NPR + pl] = yr1[c];
+        }
+      }
+      for (int f = 0; f <= nd; f++) {
+        int fi[3] = {0, 0, 0};
+        if (dir == 1) { fi[0] = f; fi[1] = a; fi[2] = b; }
+        if (dir == 2) { fi[0] = a; fi[1] = f; fi[2] = b; }
+        if (dir == 3) { fi[0] = a; fi[1] = b; fi[2] = f; }
+        o_geom G;
+        geom_at(g, d, dir, fi[0], fi[1], &G);
+        double p_l[NPR], p_r[NPR];
+        const int cl = f - 1 + ngd, cr = f + ngd; /* line index of the cells either side */
+        for (int pl = 0; pl < NPR; pl++) {
+          p_l[pl] = yr[(size_t)cl * NPR + pl];
+          p_r[pl] = yl[(size_t)cr * NPR + pl];
+        }
+        double ctop;
+        face_flux(g, dir, &G, p_l, p_r, F + ix(d, fi[0], fi[1], fi[2]), &ctop);
+        const double dtij = g->cour * g->dx[dir] / ctop;
+        if (dtij < mindt) mindt = dtij;
+      }
+      free(y);
+    }
+  *ndt = mindt;
+}
+
 static void flux_dir(const o_grid* g, const dims_t* d, int dir, const double* pb, double* F, double* ndt)
 {
   const int e1 = d->n2 > 1, e2 = d->n3 > 1, e0 = d->n1 > 1;
@@ -623,6 +631,10 @@ static void flux_dir(const o_grid* g, const dims_t* d, int dir, const double* pb
   const int di = dir == 1, dj = dir == 2, dk = dir == 3;
   const long st = (dir == 1) ? NPR : (dir == 2) ? (long)d->ns1 * NPR : (long)d->ns1 * d->ns2 * NPR;
   double mindt = BIG;
+  if (g->recon == 3) {
+    weno_flux_dir(g, d, dir, pb, F, ndt);
+    return;
+  }
 #pragma omp parallel for collapse(3) reduction(min : mindt) num_threads(g->nthreads > 0 ? g->nthreads : 1)
   for (int k = ks; k <= ke; k++)
     for (int j = js; j <= je; j++)

@@ -78,6 +78,17 @@ void o_ener_totals(const o_grid* g, const double* p, double* out);
 /* dump_content's physics columns for active zone (i,j,k): 55 values, the
  * dump's columns 10-52 and 61-72 (dump.c:439-600) */
 void o_dump_row(const o_grid* g, const double* p, const double* ucum, int i, int j, int k, double* out);
+
+/* WENO5BND centre-to-edge reconstruction of one storage line of one primitive
+ * (o_weno.c): y[0..n-1] the line, ng its ghost depth (>= 4), dP the line's
+ * pressure-jump measure; yl/yr the left/right face values of cells ng-1 ..
+ * n-ng */
+void o_weno5bnd_line(const double* y, int n, int ng, const double* dP, double* yl, double* yr);
+/* SMONO monotonicity indicator of the five values y[-2..2] */
+double o_smono_indicator(const double* y, double eps);
+/* per-cell terms of the dP line (rho, par, veff, upar) and the line itself */
+void o_dp_cell(const double* pr, const o_geom* G, double gam, int dir, double* out4);
+void o_dp_line(const double* st, int n, double* dP);
 /* one cell's (left, right) edge values from y5[0..4] = y[-2..2] (recon 0 donor,
  * 2 PARA, 3 WENO5), and one face's Riemann flux (flux 0 LAX-F, 1 HLL) from
  * two states at a 27-field geometry record; p_l, p_r are modified as

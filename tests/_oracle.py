@@ -102,6 +102,66 @@ def recon_cell(recon, y5):
     return l.value, r.value
 
 
+def weno5bnd_line(y, dP=None, ng=4):
+    """WENO5BND left/right face values of cells ng-1 .. n-ng of the storage
+    line y (n values); dP is the line's pressure-jump measure (zeros if None).
+    Returns (yl, yr) over the whole line (zeros outside those cells)."""
+    y = _c(y)
+    n = y.size
+    dP = np.zeros(n) if dP is None else _c(dP)
+    yl, yr = np.zeros(n), np.zeros(n)
+    L = lib()
+    L.o_weno5bnd_line.argtypes = [_P, _I, _I, _P, _P, _P]
+    L.o_weno5bnd_line.restype = None
+    L.o_weno5bnd_line(y.ctypes.data, n, int(ng), dP.ctypes.data, yl.ctypes.data, yr.ctypes.data)
+    return yl, yr
+
+
+def weno5bnd_line_lof(y, dP, ng=4):
+    """weno5bnd_line plus each reconstructed cell's final lower-order fraction
+    and the fraction before the dP reduction"""
+    y = _c(y)
+    n = y.size
+    dP = _c(dP)
+    yl, yr, lof, lof0 = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(n)
+    L = lib()
+    L.o_weno5bnd_line_lof.argtypes = [_P, _I, _I, _P, _P, _P, _P, _P]
+    L.o_weno5bnd_line_lof.restype = None
+    L.o_weno5bnd_line_lof(y.ctypes.data, n, int(ng), dP.ctypes.data, yl.ctypes.data, yr.ctypes.data,
+                          lof.ctypes.data, lof0.ctypes.data)
+    return yl, yr, lof, lof0
+
+
+def smono_indicator(y5, eps=500.0 * np.finfo(float).eps):
+    """SMONO indicator of the five values y5 = y[-2..2]."""
+    y = _c(y5)
+    L = lib()
+    L.o_smono_indicator.argtypes = [_P, _D]
+    L.o_smono_indicator.restype = ctypes.c_double
+    return L.o_smono_indicator(y.ctypes.data + 2 * 8, eps)
+
+
+def dp_line(cells, geomrec, dir, gam=4.0 / 3.0):
+    """dP line of the primitives cells[n, 8] at one 27-field geometry record."""
+    L = lib()
+    L.o_dp_cell_rec.argtypes = [_P, _P, _D, _I, _P]
+    L.o_dp_cell_rec.restype = None
+    L.o_dp_line.argtypes = [_P, _I, _P]
+    L.o_dp_line.restype = None
+    cells = _c(cells)
+    rec = _c(geomrec)
+    n = cells.shape[0]
+    st = np.zeros((n, 4))
+    for c in range(n):
+        row = np.ascontiguousarray(cells[c])
+        out = np.zeros(4)
+        L.o_dp_cell_rec(row.ctypes.data, rec.ctypes.data, gam, int(dir), out.ctypes.data)
+        st[c] = out
+    dP = np.zeros(n)
+    L.o_dp_line(st.ctypes.data, n, dP.ctypes.data)
+    return dP, st
+
+
 def face_flux(flux, dir, rec, p_l, p_r, gam=4.0 / 3.0, gammamax=25.0):
     """One face's Riemann flux (0 LAX-F, 1 HLL) at a 27-field geometry record."""
     L = lib()
