@@ -224,10 +224,12 @@ static void weno_line(const double* y, int n, int ng, const double* dP, double* 
                       double* fordpp_out)
 {
   const int bs = 0, be = n - 1, ps = ng - 1, pe = n - ng;
-  double* mono = (double*)calloc((size_t)n, sizeof(double));
-  double* el = (double*)calloc((size_t)n, sizeof(double));
-  double* er = (double*)calloc((size_t)n, sizeof(double));
-  wcell* W = (wcell*)calloc((size_t)n, sizeof(wcell));
+  double mono[n], el[n], er[n];
+  wcell W[n];
+  memset(mono, 0, sizeof(mono));
+  memset(el, 0, sizeof(el));
+  memset(er, 0, sizeof(er));
+  memset(W, 0, sizeof(W));
 
   /* SMONO c2e (interpline.smono.c:198): cells bs+2 .. be-2 */
   for (int c = bs + 2; c <= be - 2; c++) {
@@ -314,10 +316,6 @@ static void weno_line(const double* y, int n, int ng, const double* dP, double* 
       lof_out[c] = W[c].lof;
       fordpp_out[c] = W[c].fordpp;
     }
-  free(mono);
-  free(el);
-  free(er);
-  free(W);
 }
 
 void o_weno5bnd_line(const double* y, int n, int ng, const double* dP, double* yl, double* yr)

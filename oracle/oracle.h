@@ -8,16 +8,22 @@
  * over the ghosted storage box; geometry is the library's per-point record
  * (27 doubles) on the 2-D (i,j) storage plane.
  *
- * Pinning (tests/test_oracle_phys.py, tests/test_oracle_grid.py): per-zone
- * state, fluxes, wave speeds and the inversion against textbook GRMHD and
- * round trips; reconstruction, the face Riemann solver, RK tables, ghost
- * zones, flux-CT and the update/source balance against known answers and
- * invariants; the radial outflow extrapolation against power laws (exact
- * on the log-radius grid), constant gdet B^1 and the minmod continuation of
- * B_3; the failure-averaging cascade against linear fields.  Parity
- * unpinned: the WENO5BND extras that are not restated (SMONO, stencil
- * reduction, shock reduction) and the pole-death velocity interpolation,
- * which only the GPU comparison exercises.
+ * Pinning (tests/test_oracle_phys.py, tests/test_oracle_grid.py,
+ * tests/test_oracle_weno.py, tests/test_oracle_fixup.py,
+ * tests/test_connection_closed_form.py): per-zone state, fluxes, wave
+ * speeds and the inversion against textbook GRMHD and round trips; the
+ * WENO5BND line reconstruction against polynomial exactness, measured
+ * fifth-order convergence, the SMONO derivative stencils against numpy's
+ * polynomial derivatives, a hand-derived stencil-reduction case and the dP
+ * ramp; the face Riemann solver, RK tables, ghost zones (the pole-death
+ * velocity against its closed form), flux-CT and the update/source balance
+ * against known answers and invariants; the connection table against the
+ * closed-form MKS Kerr-Schild connection; the radial outflow extrapolation
+ * against power laws (exact on the log-radius grid), constant gdet B^1 and
+ * the minmod continuation of B_3; failure averaging against brute-force
+ * means of good neighbours; the cold inversion against cold states.
+ * Parity unpinned: the reference's uninitialised reads in the WENO dP line
+ * (o_weno.c states the choice made) cannot be pinned against anything.
  */
 #ifndef HARM_ORACLE_H
 #define HARM_ORACLE_H

@@ -202,6 +202,38 @@ def test_bound_periodic_and_polar_reflection():
                 assert np.array_equal(P(i, j, k)[:2], P(i, jr, k)[:2])
 
 
+@pytest.mark.parametrize("n3", [1, 8])
+def test_pole_death_velocity_closed_form(n3):
+    """Pole death (poledeath bounds.tools.c:2004, POLEINTERPTYPE 3,
+    POLEDEATH 1): u~2 of the first active row and of the first ghost row is
+    the linear function of X2 through the reference row's value (the second
+    active row) and 0 on the pole, f (X - X_pole) / (X_ref - X_pole), i.e.
+    f/3 in the pole-side active row and -f/3 in the first ghost row; a
+    reference value pointing away from the pole (u~2 > 0 at the lower pole,
+    < 0 at the upper) is crushed to 0 first.  A swapped pole coordinate, the
+    wrong reference row or the missing clip fails."""
+    cfg, og = grid((16, 12, n3), recon="para", flux="laxf", timeorder=2)
+    n1, n2 = cfg.n1, cfg.n2
+    g1, g2, g3 = og.ng1, og.ng2, og.ng3
+    p = og.zeros()
+    p[..., 0] = 1.0
+    p[..., 1] = 0.1
+    rng = np.random.default_rng(7)
+    f = rng.uniform(0.001, 0.01, size=(og.ns3, og.ns1))
+    f *= np.where(rng.random(f.shape) < 0.5, -1.0, 1.0)
+    p[:, g2 + 1, :, 3] = f          # reference row of the lower pole
+    p[:, g2 + n2 - 2, :, 3] = -f    # reference row of the upper pole
+    og.bound(p)
+    A = lambda j: p[g3:g3 + n3, j + g2, g1:g1 + n1, 3]
+    fa = f[g3:g3 + n3, g1:g1 + n1]
+    keep_lo = np.where(fa < 0, fa, 0.0)      # lower pole keeps u~2 < 0
+    keep_hi = np.where(-fa > 0, -fa, 0.0)    # upper pole keeps u~2 > 0
+    assert np.allclose(A(0), keep_lo / 3, rtol=1e-14, atol=0)
+    assert np.allclose(A(-1), -keep_lo / 3, rtol=1e-14, atol=0)
+    assert np.allclose(A(n2 - 1), keep_hi / 3, rtol=1e-14, atol=0)
+    assert np.allclose(A(n2), -keep_hi / 3, rtol=1e-14, atol=0)
+
+
 def divb_corner(og, p):
     """Toth's corner divergence of gdet B, normalised by |gdet B| / dx."""
     n1, n2, n3 = og.cfg.n1, og.cfg.n2, og.cfg.n3
