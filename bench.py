@@ -2,7 +2,7 @@
 """Benchmark: zone-updates/s of the HARM advance() step on one MI355X.
 
 Workload: the 3-D Fishbone-Moncrief torus in MKS Kerr-Schild coordinates at
-256x128x64 (r, theta, phi), FP64, a=0.9375, Gamma=4/3, WENO5 reconstruction,
+256x128x64 (r, theta, phi), FP64, a=0.9375, Gamma=4/3, WENO5BND reconstruction,
 HLL fluxes, Toth flux-CT, RK3, Courant 0.8, seed-1 internal-energy noise,
 rescaletype-0 floors.  A "step" is one full RK3 step (3 substeps: fluxes, CT,
 update, inversion, floors, bounds, failure averaging, dt).  All state is
@@ -109,7 +109,7 @@ def cpu_baseline(cfg, nsteps):
     el = time.time() - t0
     zones = cfg.n1 * cfg.n2 * cfg.n3
     return {"value": zones * nsteps / el, "unit": "zone-updates/s", "cores": threads, "kind": "port",
-            "sample": f"{nsteps} full RK3 steps of the same {cfg.n1}x{cfg.n2}x{cfg.n3} WENO5/HLL torus "
+            "sample": f"{nsteps} full RK3 steps of the same {cfg.n1}x{cfg.n2}x{cfg.n3} WENO5BND/HLL torus "
                       f"({zones * nsteps} zone-updates, {el:.1f} s) with the oracle's C restatement, "
                       f"OpenMP on {threads} host threads"}
 
@@ -454,7 +454,7 @@ def large_bench(args):
         "dtype": "f64",
         "data": "synthetic (seeded Fishbone-Moncrief torus initial data)",
         "config": {"workload": "3D FM torus 384x192x192 MKS Kerr-Schild a=0.9375, plain step",
-                   "recon": "WENO5", "flux": "HLL", "ct": "Toth", "rk": 3, "cour": cfg.cour,
+                   "recon": "WENO5BND", "flux": "HLL", "ct": "Toth", "rk": 3, "cour": cfg.cour,
                    "zones": zones, "sim_time_end": t_sim, "dt": dt_now},
         "steps_total": steps_done,
         "newton_iterations_per_zone_update": nstroke / (zones * 3.0 * args.steps),
@@ -472,7 +472,7 @@ def main():
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--grid", default="x".join(map(str, GRID)))
-    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--prof-steps", type=int, default=10, help="steps of the separate per-kernel timing run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", choices=["step", "inversion", "batch", "large"], default="step",
@@ -605,7 +605,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded Fishbone-Moncrief torus initial data, random-free geometry)",
         "config": {"workload": f"3D FM torus {n1}x{n2}x{n3} MKS Kerr-Schild a=0.9375, plain step",
-                   "recon": "WENO5", "flux": "HLL", "ct": "Toth", "rk": 3, "cour": cfg.cour,
+                   "recon": "WENO5BND", "flux": "HLL", "ct": "Toth", "rk": 3, "cour": cfg.cour,
                    "form": "plain (multi-kernel step, Newton from previous primitives)",
                    "tile_depth_dir2_dir3": tiles[2:4],
                    "sim_time_end": t_sim, "dt": dt_now},

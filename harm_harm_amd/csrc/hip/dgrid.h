@@ -1,8 +1,7 @@
 // dgrid.h -- grid-level device helpers: storage indexing, reconstruction,
 // face states and the small per-zone fixes (limit_gamma, inflow check).
 //
-// Reference (HARM): interppoint.para.c:568-1200 (para4), reconstructeno.c
-// :1693-2030, 2211-2330, 3015 (WENO weights), flux.c:1952-2110 (getplpr,
+// Reference (HARM): interppoint.para.c:568-1200 (para4), flux.c:1952-2110 (getplpr,
 // check_plpr), fixup.c:2723 (limit_gamma), fixup.c:3255 (inflow_check_rel4vel).
 #pragma once
 #include "dphys.h"
@@ -89,71 +88,13 @@ __device__ __forceinline__ void para4(const double* y, double& lout, double& rou
   rout = r;
 }
 
-// limit_weight (reconstructeno.c:3015) for three weights: zero below 0.2/3,
-// a linear ramp to 1/3, flat to 1.2/3, then linear to 1 at w = 1.  Written
-// without branches as the sum of a clamped first ramp and a one-sided second
-// ramp (the two ramps never overlap: w2 < w3).
-__device__ __forceinline__ double limit_weight3(double w)
-{
-  constexpr double avg = 1. / 3.;
-  constexpr double w1 = 0.2 * avg, w2 = (1.0 - 0.2) * avg, w3 = (1.0 + 0.2) * avg, w4 = 1.0;
-  constexpr double k12 = avg / (w2 - w1), k34 = (w4 - avg) / (w4 - w3);
-  return fmin(fmax((w - w1) * k12, 0.0), avg) + fmax((w - w3) * k34, 0.0);
-}
-
-
-// WENO5 centre-to-edge reconstruction of one cell from point values
-// u[-2..2] (reconstructeno.c:1693-2030 weights, :2211-2330 indicators, :3015
-// limiter): Jiang-Shu indicators of the three 3-point sub-stencils, a
-// value-proportional epsilon, indicators normalised by their sum, nonlinear
-// weights 1/(beta+eps)^2 normalised and limited, then combined with the
-// optimal edge weights {5/16, 5/8, 1/16} (mirrored for the left edge).
-// Both edges share the indicators, so a cell is reconstructed once.  The
-// arithmetic is arranged for few divisions: one reciprocal per
-// normalisation, the three weights 1/t_s normalised through the pairwise
-// products t_a t_b, and the second weight normalisation folded into the
-// edge normalisation (it cancels there).  Results differ from the
-// divide-per-element form by a few ulps.
-__device__ __forceinline__ void weno5(const double* u, double& lout, double& rout)
-{
-  constexpr double EPS = (500.0 * DBL_EPSILON) * (500.0 * DBL_EPSILON);
-  const double d10 = -(3 * u[0] + u[2]) + 4 * u[1];
-  const double d11 = u[1] - u[-1];
-  const double d12 = (3 * u[0] + u[-2]) - 4 * u[-1];
-  const double d20 = (u[0] + u[2]) - 2 * u[1];
-  const double d21 = (u[-1] + u[1]) - 2 * u[0];
-  const double d22 = (u[0] + u[-2]) - 2 * u[-1];
-  const double err = (((u[-2] * u[-2] + u[2] * u[2]) + (u[-1] * u[-1] + u[1] * u[1])) + u[0] * u[0]) * EPS;
-  double b0 = (13. / 12. * d20 * d20 + 1. / 4. * d10 * d10) + err;
-  double b1 = (13. / 12. * d21 * d21 + 1. / 4. * d11 * d11) + err;
-  double b2 = (13. / 12. * d22 * d22 + 1. / 4. * d12 * d12) + err;
-  const double isb = recip_nr((b0 + b2) + b1 + DBL_MIN);  // >= DBL_MIN
-  const double t0 = (b0 * isb + EPS) * (b0 * isb + EPS);
-  const double t1 = (b1 * isb + EPS) * (b1 * isb + EPS);
-  const double t2 = (b2 * isb + EPS) * (b2 * isb + EPS);
-  const double q0 = t1 * t2, q1 = t0 * t2, q2 = t0 * t1;
-  const double iq = recip_nr((q0 + q2) + q1);  // >= EPS^4
-  const double w0 = limit_weight3(q0 * iq), w1 = limit_weight3(q1 * iq), w2 = limit_weight3(q2 * iq);
-  const double wr0 = w0 * (5.0 / 16.0), wr1 = w1 * (5.0 / 8.0), wr2 = w2 * (1.0 / 16.0);
-  const double wl0 = w0 * (1.0 / 16.0), wl1 = w1 * (5.0 / 8.0), wl2 = w2 * (5.0 / 16.0);
-  // the sub-stencil edge values times 8 (the 1/8 is applied with the
-  // normalisation)
-  const double qr0 = 3 * u[0] + 6 * u[1] - u[2];
-  const double qr1 = -u[-1] + 6 * u[0] + 3 * u[1];
-  const double qr2 = 3 * u[-2] - 10 * u[-1] + 15 * u[0];
-  const double ql0 = 15 * u[0] - 10 * u[1] + 3 * u[2];
-  const double ql1 = 3 * u[-1] + 6 * u[0] - u[1];
-  const double ql2 = -u[-2] + 6 * u[-1] + 3 * u[0];
-  // one limited weight is >= 1/3, so each edge's weight sum is >= 1/48
-  rout = ((wr0 * qr0 + wr2 * qr2) + wr1 * qr1) * (0.125 * recip_nr((wr0 + wr2) + wr1));
-  lout = ((wl0 * ql0 + wl2 * ql2) + wl1 * ql1) * (0.125 * recip_nr((wl0 + wl2) + wl1));
-}
-
+// Single-zone reconstruction for the schemes that need only the zone's
+// five-point stencil (PARA, donor).  WENO5BND reconstructs whole lines
+// (weno.hip).
 template <int RECON>
 __device__ __forceinline__ void recon_cell(const double* y, double& l, double& r)
 {
   if (RECON == HARM_RECON_PARA) para4(y, l, r);
-  else if (RECON == HARM_RECON_WENO5) weno5(y, l, r);
   else { l = y[0]; r = y[0]; }
 }
 

@@ -179,6 +179,8 @@ struct GridRef {
   double* ds;
   unsigned long long* di;
   double* facc;
+  const double* wl;  // WENO5BND face values of the current direction (or null)
+  const double* wr;
   FaceTile ft[4];
 };
 
@@ -253,10 +255,19 @@ void launch_ener_totals(const hd::Dims& d, const Params& P, const double* p, con
                         double* out, hipStream_t s);
 
 // Launchers of the face-flux kernels (flux.hip): one direction of one grid,
-// or one direction of every grid of a batch.
+// or one direction of every grid of a batch.  With WENO5BND the face values
+// come from wl, wr (launch_weno_recon) instead of an in-tile reconstruction.
 void launch_flux(int dir, int recon, const FaceTile& T, hipStream_t s, const Dims& d, const double* pb, double* F,
-                 const double* gface, const Params& P, int fluxtype, int do_ct, double* ds);
+                 const double* gface, const Params& P, int fluxtype, int do_ct, double* ds,
+                 const double* wl = nullptr, const double* wr = nullptr);
 void launch_flux_batch(int dir, int recon, unsigned nblk, hipStream_t s, const GridRef* G, const int* pre, int ng,
                        int which, int do_ct);
+// WENO5BND (weno.hip): the dP measure of every active direction, then the
+// face values of one direction's lines
+// (tm: 10 x ncell doubles of per-zone terms, dp: 3 x ncell)
+void launch_weno_dp(const Dims& d, hipStream_t s, const double* pb, const double* gcent, const Params& P, double* tm,
+                    double* dp);
+void launch_weno_recon(int dir, const Dims& d, const Box& b, hipStream_t s, const double* pb, const double* dp,
+                       double* L, double* R);
 
 }  // namespace hk

@@ -1003,6 +1003,10 @@ struct harm_handle {
   double* floors = nullptr;
   double* rcent = nullptr;
   double* emf = nullptr;
+  double* wl = nullptr;    // WENO5BND: face values of the current direction's lines (SoA like p)
+  double* wr = nullptr;
+  double* wdp = nullptr;   // WENO5BND: dP measure per direction (3 x ncell)
+  double* wtm = nullptr;   // WENO5BND: per-zone terms of dP (10 x ncell)
   double* facc = nullptr;  // boundary-flux partial sums
   double* en = nullptr;      // energy-file accumulators (EN_*)
   double* enpart = nullptr;  // their per-block partial sums
@@ -1248,10 +1252,27 @@ harm_status enq_fluxcalc(harm_handle* h, int which_p, int do_ct)
   const double* pb = h->arr[which_p];
   hipLaunchKernelGGL(k_scal_begin_flux, dim3(1), dim3(1), 0, s, h->ds);
   const int recon = h->cfg.recon, ft = h->cfg.flux;
-#define LAUNCH_FLUX(DIR, F, G) launch_flux(DIR, recon, h->ft[DIR], s, d, pb, F, G, h->P, ft, do_ct, h->ds)
-  if (d.n1 > 1) { prof_begin(h); LAUNCH_FLUX(1, h->arr[HARM_ARR_F1], h->geom[HARM_LOC_FACE1]); prof_end(h, HARM_K_FLUX1); }
-  if (d.n2 > 1) { prof_begin(h); LAUNCH_FLUX(2, h->arr[HARM_ARR_F2], h->geom[HARM_LOC_FACE2]); prof_end(h, HARM_K_FLUX2); }
-  if (d.n3 > 1) { prof_begin(h); LAUNCH_FLUX(3, h->arr[HARM_ARR_F3], h->geom[HARM_LOC_FACE3]); prof_end(h, HARM_K_FLUX3); }
+  const bool weno = recon == HARM_RECON_WENO5;
+  if (weno) {
+    prof_begin(h);
+    launch_weno_dp(d, s, pb, h->geom[HARM_LOC_CENT], h->P, h->wtm, h->wdp);
+    prof_end(h, HARM_K_RECON);
+  }
+  // WENO5BND: each direction's lines are reconstructed into wl, wr just before its face kernel reads them
+#define LAUNCH_FLUX(DIR, F, G)                                                                              \
+  do {                                                                                                      \
+    if (weno) {                                                                                             \
+      prof_begin(h);                                                                                        \
+      launch_weno_recon(DIR, d, h->ft[DIR].b, s, pb, h->wdp, h->wl, h->wr);                                 \
+      prof_end(h, HARM_K_RECON);                                                                            \
+    }                                                                                                       \
+    prof_begin(h);                                                                                          \
+    launch_flux(DIR, recon, h->ft[DIR], s, d, pb, F, G, h->P, ft, do_ct, h->ds, h->wl, h->wr);              \
+    prof_end(h, HARM_K_FLUX1 + DIR - 1);                                                                    \
+  } while (0)
+  if (d.n1 > 1) LAUNCH_FLUX(1, h->arr[HARM_ARR_F1], h->geom[HARM_LOC_FACE1]);
+  if (d.n2 > 1) LAUNCH_FLUX(2, h->arr[HARM_ARR_F2], h->geom[HARM_LOC_FACE2]);
+  if (d.n3 > 1) LAUNCH_FLUX(3, h->arr[HARM_ARR_F3], h->geom[HARM_LOC_FACE3]);
 #undef LAUNCH_FLUX
   HH_TRY(hh::check_launch("k_flux"));
   if (do_ct) {
@@ -1663,6 +1684,8 @@ GridRef make_ref(const harm_handle* h)
   r.ds = h->ds;
   r.di = h->di;
   r.facc = h->facc;
+  r.wl = h->wl;
+  r.wr = h->wr;
   for (int dir = 0; dir < 4; dir++) r.ft[dir] = h->ft[dir];
   return r;
 }
@@ -1739,10 +1762,20 @@ harm_status enq_batch_substep(harm_batch* b, int to, int nto, const double CUf[4
   const unsigned g1 = (unsigned)((ng + 63) / 64);
   hipLaunchKernelGGL(kb_scal, dim3(g1), dim3(64), 0, s, G, ng, 0);
   // fluxes
-#define BFLUX(DIR)                                                                                     \
-  if (b->ntot[BS_FLUX1 + DIR - 1] > 0)                                                                  \
+  // WENO5BND: every grid's lines of a direction are reconstructed (one
+  // launch per grid) before the batched face kernel of that direction
+  const bool weno = b->recon == HARM_RECON_WENO5;
+  if (weno)
+    for (harm_handle* h : b->grids) launch_weno_dp(h->d, s, h->arr[pbb[to]], h->geom[HARM_LOC_CENT], h->P, h->wtm, h->wdp);
+#define BFLUX(DIR)                                                                                         \
+  if (b->ntot[BS_FLUX1 + DIR - 1] > 0) {                                                                    \
+    if (weno)                                                                                               \
+      for (harm_handle* h : b->grids)                                                                       \
+        if (h->ft[DIR].nblk > 0 && (DIR == 1 ? h->d.n1 : DIR == 2 ? h->d.n2 : h->d.n3) > 1)                 \
+          launch_weno_recon(DIR, h->d, h->ft[DIR].b, s, h->arr[pbb[to]], h->wdp, h->wl, h->wr);             \
     launch_flux_batch(DIR, b->recon, (unsigned)b->ntot[BS_FLUX1 + DIR - 1], s, G, P(BS_FLUX1 + DIR - 1), ng, \
-                      pbb[to], 1);
+                      pbb[to], 1);                                                                          \
+  }
   BFLUX(1)
   BFLUX(2)
   BFLUX(3)
@@ -1845,6 +1878,9 @@ extern "C" harm_status harm_create(const harm_config* cfg, harm_handle** out)
   }
   if (cfg->recon < HARM_RECON_DONOR || cfg->recon > HARM_RECON_WENO5 || cfg->recon == HARM_RECON_MC)
     return hh::set_error(HARM_EINVAL, "harm_create: recon %d not supported", cfg->recon);
+  if (cfg->recon == HARM_RECON_WENO5 && cfg->ng != 4)
+    return hh::set_error(HARM_EINVAL, "harm_create: WENO5BND lines are built for 4 ghost zones (MAXBND 4), ng = %d",
+                         cfg->ng);
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return hh::set_error(HARM_ENOGPU, "harm_create: no GPU");
   harm_handle* h = new harm_handle();
@@ -1869,6 +1905,15 @@ extern "C" harm_status harm_create(const harm_config* cfg, harm_handle** out)
   h->emf_nj = d.n2 + (d.n2 > 1);
   h->emf_nk = d.n3 + (d.n3 > 1);
   TRYC(alloc((void**)&h->emf, sizeof(double) * 3 * (size_t)h->emf_ni * h->emf_nj * h->emf_nk));
+  if (cfg->recon == HARM_RECON_WENO5) {
+    TRYC(alloc((void**)&h->wl, sizeof(double) * HARM_NPR * d.ncell));
+    TRYC(alloc((void**)&h->wr, sizeof(double) * HARM_NPR * d.ncell));
+    TRYC(alloc((void**)&h->wdp, sizeof(double) * 3 * d.ncell));
+    TRYC(alloc((void**)&h->wtm, sizeof(double) * 10 * d.ncell));
+    TRYC(hh::check_hip(hipMemset(h->wl, 0, sizeof(double) * HARM_NPR * d.ncell), "hipMemset"));
+    TRYC(hh::check_hip(hipMemset(h->wr, 0, sizeof(double) * HARM_NPR * d.ncell), "hipMemset"));
+    TRYC(hh::check_hip(hipMemset(h->wdp, 0, sizeof(double) * 3 * d.ncell), "hipMemset"));
+  }
   TRYC(alloc((void**)&h->ds, sizeof(double) * DS_N));
   TRYC(alloc((void**)&h->facc, sizeof(double) * 4 * kFaccBlocks));
   TRYC(alloc((void**)&h->en, sizeof(double) * EN_N));
@@ -1901,8 +1946,8 @@ extern "C" harm_status harm_destroy(harm_handle* h)
     (void)hipEventDestroy(r.second.second);
   }
   for (hipEvent_t e : h->evpool) (void)hipEventDestroy(e);
-  void* bufs[] = {h->pflag, h->pfix, h->gcorn3, h->conn, h->floors, h->rcent, h->emf, h->facc, h->en, h->enpart, h->entot,
-                  h->fx_p0, h->fx_pfl, h->fx_flag, h->ds, h->di};
+  void* bufs[] = {h->pflag, h->pfix, h->gcorn3, h->conn, h->floors, h->rcent, h->emf, h->wl, h->wr, h->wdp, h->wtm, h->facc,
+                  h->en, h->enpart, h->entot, h->fx_p0, h->fx_pfl, h->fx_flag, h->ds, h->di};
   for (void* b : bufs) (void)hipFree(b);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
@@ -2747,7 +2792,11 @@ extern "C" harm_status harm_tune(harm_handle* h, int32_t reps, int32_t td_out[4]
       const FaceTile T = plan_face_tile(d, dir, td);
       double* F = h->arr[HARM_ARR_F1 + dir - 1];
       const double* G = h->geom[dir];
-      auto launch = [&]() { launch_flux(dir, recon, T, s, d, pb, F, G, h->P, ft, 1, h->ds); };
+      if (recon == HARM_RECON_WENO5) {  // the face kernel reads this direction's line reconstruction
+        launch_weno_dp(d, s, pb, h->geom[HARM_LOC_CENT], h->P, h->wtm, h->wdp);
+        launch_weno_recon(dir, d, T.b, s, pb, h->wdp, h->wl, h->wr);
+      }
+      auto launch = [&]() { launch_flux(dir, recon, T, s, d, pb, F, G, h->P, ft, 1, h->ds, h->wl, h->wr); };
       launch();  // warm
       (void)hipEventRecord(e0, s);
       for (int r = 0; r < reps; r++) launch();

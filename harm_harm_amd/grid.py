@@ -256,7 +256,7 @@ class Grid:
         check(lib().harm_get_boundary_fluxes(self.h, out), "harm_get_boundary_fluxes")
         return list(out)
 
-    KERNELS = ("flux1", "flux2", "flux3", "emf_ct", "update", "invert", "bound", "fixup", "other")
+    KERNELS = ("flux1", "flux2", "flux3", "emf_ct", "update", "invert", "bound", "fixup", "other", "recon")
 
     def profile(self, enable=True):
         check(lib().harm_profile(self.h, int(enable)), "harm_profile")

@@ -369,7 +369,7 @@ harm_status harm_event_elapsed_ms(void* ev0, void* ev1, float* ms);
  * kernel group while enabled; read back (after a sync) as total ms and launch
  * count per group.  Groups: */
 enum { HARM_K_FLUX1 = 0, HARM_K_FLUX2 = 1, HARM_K_FLUX3 = 2, HARM_K_EMFCT = 3, HARM_K_UPDATE = 4,
-       HARM_K_INVERT = 5, HARM_K_BOUND = 6, HARM_K_FIXUP = 7, HARM_K_OTHER = 8, HARM_K_N = 9 };
+       HARM_K_INVERT = 5, HARM_K_BOUND = 6, HARM_K_FIXUP = 7, HARM_K_OTHER = 8, HARM_K_RECON = 9, HARM_K_N = 10 };
 harm_status harm_profile(harm_handle* h, int32_t enable);
 harm_status harm_profile_read(harm_handle* h, double* ms_total, int64_t* count);
 
