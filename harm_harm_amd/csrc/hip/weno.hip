@@ -46,22 +46,22 @@ constexpr double kSqrtWeps = 500.0 * DBL_EPSILON;  // SQRT_WENO_EPSILON
 constexpr double kWeps = kSqrtWeps * kSqrtWeps;     // WENO_EPSILON = REDUCEEPSILON
 constexpr double kMinNum = DBL_MIN;                 // MINNUMREPRESENT
 
-__device__ __forceinline__ double wmin(double a, double b) { return a < b ? a : b; }
-__device__ __forceinline__ double wmax(double a, double b) { return a > b ? a : b; }
+constexpr double wmin(double a, double b) { return a < b ? a : b; }
+constexpr double wmax(double a, double b) { return a > b ? a : b; }
 __device__ __forceinline__ double wminmod(double a, double b)
 {
   return (a * b <= 0) ? 0.0 : ((fabs(a) < fabs(b)) ? a : b);
 }
 
 // get_sum_of_elements with SYMMETRIZE_WENO: mirror pairs first, the middle last
-__device__ __forceinline__ double sym3(double a0, double a1, double a2)
+constexpr double sym3(double a0, double a1, double a2)
 {
   double s = 0.0;
   s += a0 + a2;
   s += a1 * 1;
   return s;
 }
-__device__ __forceinline__ double sym2(double a0, double a1)
+constexpr double sym2(double a0, double a1)
 {
   double s = 0.0;
   s += a0 + a1;
@@ -86,6 +86,28 @@ __device__ __forceinline__ double limit_weight(double w)
   if (w < w3) return avg;
   return (w - w3) * (w4 - avg) / (w4 - w3) + avg;
 }
+
+// The weights of the two uniform cases, formed by the same operations as the
+// general code below (constant-folded, round to nearest):
+//   * all five values zero: the Jiang-Shu indicators are 0, every weight is
+//     1/(WENO_EPSILON^2) before normalisation, the normalised ones lie within
+//     an ulp of 1/3, limit_weight maps them to its flat band's value 1./3,
+//     and (1/3 + 1/3) + 1/3 rounds to 1, so the zone's weights are 1./3 each
+//     and its SMONO indicator is 0 (every derivative estimate is 0);
+//   * three equal values u[-1] = u[0] = u[1]: both WENO3 indicators are the
+//     same, so both weights normalise to exactly 1/2, which limit_weight
+//     keeps.
+// The face weights of a zone whose unoptimized weights are 1./3 each, and
+// the WENO3 face weights of 1/2 each, are these constants.
+constexpr double kThird = 1. / 3;
+constexpr double kL3n = sym3(kThird * (1. / 16.), kThird * (5. / 8.), kThird * (5. / 16.)) + kMinNum;
+constexpr double kR3n = sym3(kThird * (5. / 16.), kThird * (5. / 8.), kThird * (1. / 16.)) + kMinNum;
+constexpr double kL3[3] = {kThird * (1. / 16.) / kL3n, kThird * (5. / 8.) / kL3n, kThird * (5. / 16.) / kL3n};
+constexpr double kR3[3] = {kThird * (5. / 16.) / kR3n, kThird * (5. / 8.) / kR3n, kThird * (1. / 16.) / kR3n};
+constexpr double kL2n = sym2(0.5 * (1. / 4.), 0.5 * (3. / 4.)) + kMinNum;
+constexpr double kR2n = sym2(0.5 * (3. / 4.), 0.5 * (1. / 4.)) + kMinNum;
+constexpr double kL2[2] = {0.5 * (1. / 4.) / kL2n, 0.5 * (3. / 4.) / kL2n};
+constexpr double kR2[2] = {0.5 * (3. / 4.) / kR2n, 0.5 * (1. / 4.) / kR2n};
 
 // Unoptimized WENO5 weights of the zone u[0] (all three sub-stencils lie in
 // the line for every zone the sweep computes).
@@ -238,6 +260,11 @@ struct ZoneW {
 
 __device__ __forceinline__ void zone_weights(const double* y, ZoneW& z)
 {
+  if (y[-2] == 0.0 && y[-1] == 0.0 && y[0] == 0.0 && y[1] == 0.0 && y[2] == 0.0) {
+    z.mono = 0.0;
+    z.u[0] = z.u[1] = z.u[2] = kThird;
+    return;
+  }
   z.mono = smono(y);
   if (z.mono == 1) {
     z.u[0] = z.u[1] = z.u[2] = 1.0 / 3;
@@ -250,11 +277,10 @@ __device__ __forceinline__ void zone_weights(const double* y, ZoneW& z)
 }
 
 // one side of the AND ratio test: the zone's outer sub-stencil weight against
-// the neighbour's weight of the same three zones.  Between two zones that
-// SMONO holds both weights are 1/3 and the ratio is exactly 1.
-__device__ __forceinline__ double side_ratio(double uc, double mc, double un, double mn)
+// the neighbour's weight of the same three zones (equal weights: exactly 1)
+__device__ __forceinline__ double side_ratio(double uc, double un)
 {
-  if (mc == 1 && mn == 1) return 1.0;
+  if (uc == un) return 1.0;
   return (uc + kWeps) / (un + kWeps);
 }
 
@@ -263,6 +289,9 @@ __device__ __forceinline__ double ratio_lof(double wl, double wr)
 {
   const double wrat = wmin(wl, wr);
   const double min_rat = 1.3, max_rat = 1.6;
+  // the clamps below give exactly these at and beyond the band's ends
+  if (wrat <= min_rat) return 0.0;
+  if (wrat >= max_rat) return 1.0;
   double lof = (wrat - min_rat) / (max_rat - min_rat);
   lof = wmax(0., wmin(lof, 1.0));
   return wmax(lof, 0.0);  // the weights' own fraction starts at 0
@@ -280,20 +309,40 @@ __device__ __forceinline__ void zone_faces(const double* y, const ZoneW& z, doub
   }
   if (m >= 0.0 && m < 1.0) {
     // optimized WENO5 weights from the (ramped) unoptimized ones
-    double L[3] = {z.u[0] * (1. / 16.), z.u[1] * (5. / 8.), z.u[2] * (5. / 16.)};
-    double R[3] = {z.u[0] * (5. / 16.), z.u[1] * (5. / 8.), z.u[2] * (1. / 16.)};
-    double n = sym3(L[0], L[1], L[2]) + kMinNum;
-    L[0] /= n;
-    L[1] /= n;
-    L[2] /= n;
-    n = sym3(R[0], R[1], R[2]) + kMinNum;
-    R[0] /= n;
-    R[1] /= n;
-    R[2] /= n;
+    double L[3], R[3];
+    if (z.u[0] == kThird && z.u[1] == kThird && z.u[2] == kThird) {
+#pragma unroll
+      for (int s = 0; s < 3; s++) {
+        L[s] = kL3[s];
+        R[s] = kR3[s];
+      }
+    } else {
+      L[0] = z.u[0] * (1. / 16.);
+      L[1] = z.u[1] * (5. / 8.);
+      L[2] = z.u[2] * (5. / 16.);
+      R[0] = z.u[0] * (5. / 16.);
+      R[1] = z.u[1] * (5. / 8.);
+      R[2] = z.u[2] * (1. / 16.);
+      double n = sym3(L[0], L[1], L[2]) + kMinNum;
+      L[0] /= n;
+      L[1] /= n;
+      L[2] /= n;
+      n = sym3(R[0], R[1], R[2]) + kMinNum;
+      R[0] /= n;
+      R[1] /= n;
+      R[2] /= n;
+    }
     double vl = 0.0, vr = 0.0;
     if (0.0 < lof) {
       double L2[2], R2[2];
-      weights2(y, L2, R2);
+      if (y[-1] == y[0] && y[0] == y[1]) {
+        L2[0] = kL2[0];
+        L2[1] = kL2[1];
+        R2[0] = kR2[0];
+        R2[1] = kR2[1];
+      } else {
+        weights2(y, L2, R2);
+      }
 #pragma unroll
       for (int s = 0; s < 3; s++) {
         L[s] *= (1. - lof);
@@ -453,8 +502,8 @@ __global__ __launch_bounds__(kBlock) void k_weno_recon(Dims d, Box b, int seg, i
     fd[1] = fd[2];
     fd[2] = 0.0;
     if (r >= 2 && r >= f0 - 1) {
-      const double wl = r >= 4 ? side_ratio(zw[2].u[2], zw[2].mono, z6.u[0], z6.mono) : kBIG;
-      const double wr = r <= n - 5 ? side_ratio(zw[2].u[0], zw[2].mono, zw[0].u[2], zw[0].mono) : kBIG;
+      const double wl = r >= 4 ? side_ratio(zw[2].u[2], z6.u[0]) : kBIG;
+      const double wr = r <= n - 5 ? side_ratio(zw[2].u[0], zw[0].u[2]) : kBIG;
       fd[2] = ratio_lof(wl, wr);
     }
     // faces of zone f = s-5 (zw[3], values y[0..4] centred on y[2])
