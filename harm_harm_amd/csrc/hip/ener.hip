@@ -325,12 +325,16 @@ __global__ void k_ener_divb_final(const double* __restrict__ part, long n, doubl
 }
 
 // The physics columns of one zone of HARM's main dump (dump_content,
-// dump.c:439-600, CALCFARADAYANDCURRENTS): out[c * nz + t] with t = i + n1 (j
-// + n2 k), the dump's row order, and c in DC_* order.  U is the conserved
-// state the last inversion consumed (udump = unewglobal, initbase.c:885).
+// dump.c:439-600, CALCFARADAYANDCURRENTS): column c (DC_* order) of zone t =
+// i + n1 (j + n2 k), the dump's row order, at out[c' * cs + t * rs], where
+// c' = c below DC_FCON and c + jgap from there on; with jgap > 0 the jgap
+// columns between gdet and F (the four-current, not kept) are written as 0.
+// U is the conserved state the last inversion consumed (udump =
+// unewglobal, initbase.c:885).
 __global__ __launch_bounds__(kBlock) void k_dump_cols(Dims d, Params P, const double* __restrict__ p,
                                                       const double* __restrict__ ucum,
-                                                      const double* __restrict__ gcent, double* __restrict__ out)
+                                                      const double* __restrict__ gcent, double* __restrict__ out,
+                                                      long cs, long rs, int jgap)
 {
   const long nz = (long)d.n1 * d.n2 * d.n3;
   const long t = (long)blockIdx.x * kBlock + threadIdx.x;
@@ -342,7 +346,8 @@ __global__ __launch_bounds__(kBlock) void k_dump_cols(Dims d, Params P, const do
   double pz[HARM_NPR];
 #pragma unroll
   for (int pl = 0; pl < HARM_NPR; pl++) pz[pl] = p[pl * nc + o];
-  auto put = [&](int c, double v) { out[(long)c * nz + t] = v; };
+  auto put = [&](int c, double v) { out[(long)(c < DC_FCON ? c : c + jgap) * cs + t * rs] = v; };
+  for (int c = 0; c < jgap; c++) out[(long)(DC_FCON + c) * cs + t * rs] = 0.0;
 #pragma unroll
   for (int pl = 0; pl < HARM_NPR; pl++) put(DC_P + pl, pz[pl]);
   // ideal-gas EOS columns (idealgaseos.c:19, 48, 71)
@@ -415,8 +420,16 @@ __global__ __launch_bounds__(kBlock) void k_dump_cols(Dims d, Params P, const do
 void launch_dump_cols(const hd::Dims& d, const Params& P, const double* p, const double* ucum, const double* gcent,
                       double* out, hipStream_t s)
 {
-  hipLaunchKernelGGL(k_dump_cols, dim3(nblk((long)d.n1 * d.n2 * d.n3)), dim3(kBlock), 0, s, d, P, p, ucum, gcent,
-                     out);
+  const long nz = (long)d.n1 * d.n2 * d.n3;
+  hipLaunchKernelGGL(k_dump_cols, dim3(nblk(nz)), dim3(kBlock), 0, s, d, P, p, ucum, gcent, out, nz, 1L, 0);
+}
+
+void launch_dump_rows(const hd::Dims& d, const Params& P, const double* p, const double* ucum, const double* gcent,
+                      double* rows, hipStream_t s)
+{
+  const long nz = (long)d.n1 * d.n2 * d.n3;
+  hipLaunchKernelGGL(k_dump_cols, dim3(nblk(nz)), dim3(kBlock), 0, s, d, P, p, ucum, gcent, rows + kDumpCoordCols,
+                     1L, (long)HARM_DUMP_NCOL, kDumpCurCols);
 }
 
 long ener_part_blocks(const hd::Dims& d)

@@ -244,6 +244,13 @@ enum { DC_P = 0, DC_PRESS = 8, DC_CS2 = 9, DC_SDEN = 10, DC_U = 11, DC_DIVB = 19
        DC_BCON = 28, DC_BCOV = 32, DC_VCHAR = 36, DC_GDET = 42, DC_FCON = 43, DC_FCOV = 49, DC_N = 55 };
 void launch_dump_cols(const hd::Dims& d, const Params& P, const double* p, const double* ucum, const double* gcent,
                       double* out, hipStream_t s);
+// The same columns straight into the dump's rows: rows[z * HARM_DUMP_NCOL +
+// c], columns kDumpCoordCols.. (the coordinate columns before them are left
+// as they are), with the kDumpCurCols current columns after gdet written as 0.
+enum { kDumpCoordCols = 9, kDumpCurCols = 8 };
+static_assert(kDumpCoordCols + DC_N + kDumpCurCols == HARM_DUMP_NCOL, "dump row layout");
+void launch_dump_rows(const hd::Dims& d, const Params& P, const double* p, const double* ucum, const double* gcent,
+                      double* rows, hipStream_t s);
 // source sample weight at the end of substep `to` of nto (advance.c:251-262)
 void launch_ener_dt4(double* en, const double* ds, int last, hipStream_t s);
 // sourceadd += dt4diag * dV * dUgeom(pb) over active zones when dt4diag > 0

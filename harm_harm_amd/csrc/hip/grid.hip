@@ -22,6 +22,7 @@
 #include <string.h>
 #include <vector>
 #include <algorithm>
+#include <mutex>
 #include "dgrid.h"
 #include "gkern.h"
 #include "dinvert.h"
@@ -994,6 +995,7 @@ struct harm_handle {
   Dims d;
   Params P;
   hipStream_t stream = nullptr;
+  int dev = 0;  // the device current at harm_create (the stream's)
   double* arr[HARM_NARR] = {nullptr};
   int32_t* pflag = nullptr;  // over the storage box; the one-layer ghost band holds bound_pflag's values
   double* pfix = nullptr;    // failure averaging's new primitives of the failed zones (SoA, own place)
@@ -1885,6 +1887,10 @@ extern "C" harm_status harm_create(const harm_config* cfg, harm_handle** out)
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return hh::set_error(HARM_ENOGPU, "harm_create: no GPU");
   harm_handle* h = new harm_handle();
   h->cfg = *cfg;
+  if (hipGetDevice(&h->dev) != hipSuccess) {
+    delete h;
+    return hh::set_error(HARM_ENOGPU, "harm_create: hipGetDevice failed");
+  }
   Dims& d = h->d;
   make_dims(cfg, h->d, h->P);
   harm_status st = HARM_OK;
@@ -2457,6 +2463,67 @@ extern "C" harm_status harm_dump_columns(harm_handle* h, int32_t which, double* 
     for (int q = 0; q <= DC_GDET; q++) r[c++] = col[(size_t)q * nz + t];
     for (int q = 0; q < 8; q++) r[c++] = 0.0;
     for (int q = DC_FCON; q < DC_N; q++) r[c++] = col[(size_t)q * nz + t];
+  }
+  return HARM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// harm_dump_columns with no allocation: the rows are built on the device in a
+// static buffer of kNoallocZones rows (one per device, 37.7 MB, held until the
+// process ends) and copied into the caller's rows.  Calls that overlap on one
+// device share the buffer and race.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr long kNoallocZones = 65536;
+static __device__ __align__(256) double scratch[kNoallocZones * HARM_DUMP_NCOL];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {nullptr};
+
+// the device address of `scratch` on device dev, looked up once per device
+// with that device current (each device holds its own copy of the array)
+harm_status scratch_on(int dev, double** out)
+{
+  if (dev < 0 || dev >= kMaxDevices) return hh::set_error(HARM_EINVAL, "noalloc: device %d out of range", dev);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    int cur = 0;
+    HH_HIP(hipGetDevice(&cur));
+    if (cur != dev) HH_HIP(hipSetDevice(dev));
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (cur != dev) HH_HIP(hipSetDevice(cur));
+    HH_TRY(hh::check_hip(e, "hipGetSymbolAddress(scratch)"));
+    g_scratch_ptr[dev] = (double*)ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return HARM_OK;
+}
+}  // namespace
+
+extern "C" harm_status harm_dump_columns_noalloc(harm_handle* h, int32_t which, double* rows)
+{
+  HH_TRY(check_handle(h));
+  if (!rows) return hh::set_error(HARM_EINVAL, "harm_dump_columns_noalloc: null argument");
+  if (which < 0 || which >= HARM_NARR)
+    return hh::set_error(HARM_EINVAL, "harm_dump_columns_noalloc: bad array %d", which);
+  const Dims& d = h->d;
+  const long nz = (long)d.n1 * d.n2 * d.n3;
+  if (nz > kNoallocZones)
+    return hh::set_error(HARM_EINVAL, "harm_dump_columns_noalloc: %ld zones, above the static buffer's %ld", nz,
+                         kNoallocZones);
+  double* dev = nullptr;
+  HH_TRY(scratch_on(h->dev, &dev));
+  // every column of every row the copy takes is written: the kernel fills
+  // the physics and current columns, the host the coordinates after the copy
+  launch_dump_rows(d, h->P, h->arr[which], h->arr[HARM_ARR_UCUM], h->geom[HARM_LOC_CENT], dev, h->stream);
+  HH_TRY(hh::check_launch("k_dump_cols"));
+  HH_HIP(hipMemcpyAsync(rows, dev, sizeof(double) * HARM_DUMP_NCOL * nz, hipMemcpyDeviceToHost, h->stream));
+  HH_HIP(hipStreamSynchronize(h->stream));
+  for (long t = 0; t < nz; t++) {
+    const int i = (int)(t % d.n1), j = (int)((t / d.n1) % d.n2), k = (int)(t / ((long)d.n1 * d.n2));
+    dump_coords(h, i, j, k, rows + t * HARM_DUMP_NCOL);
   }
   return HARM_OK;
 }

@@ -311,6 +311,13 @@ enum { HARM_DUMP_NCOL = 72, HARM_GDUMP_NCOL = 126 };
 /* Replaces dump_content() over the grid: rows[z * HARM_DUMP_NCOL + c] from
  * primitives `which` (HARM_ARR_*) on the host. */
 harm_status harm_dump_columns(harm_handle* h, int32_t which, double* rows);
+/* No counterpart in the reference: harm_dump_columns for callers that cannot
+ * allocate (code under graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same rows, built in a static device buffer of 65536
+ * zones (37.7 MB per device, held until the process ends) instead of
+ * allocated memory; grids above 65536 zones get HARM_EINVAL.  The buffer is
+ * shared by every call on a device: calls that overlap on one device race. */
+harm_status harm_dump_columns_noalloc(harm_handle* h, int32_t which, double* rows);
 /* Replaces gdump_content() over the grid: rows[z * HARM_GDUMP_NCOL + c]. */
 harm_status harm_gdump_columns(harm_handle* h, double* rows);
 /* Replace dump() and gdump() (text output, dump_header_general's header line,
