@@ -19,19 +19,25 @@
 //   * the optimal face weights and the blend of WENO5, WENO3 and SMONO values
 //     (choose_weno_order :2442, eno_cvt :884, eno_line_reconstruct :538).
 //
-// Device layout.  k_weno_dp forms dP for every direction at every zone that a
-// face reconstruction needs (compute_df_line_new interpline.c:3557, from the
-// zone's state and its two neighbours' along the direction).  k_weno_recon
-// then sweeps each storage line of one primitive in one thread: lanes of a
-// wave are consecutive i (directions 2 and 3: coalesced loads and stores), the
-// thread carries the line's last eight values and a four-zone pipeline of
-// weights in registers, and writes the left and right face values of zones
-// -1 .. N along the sweep; the face-flux kernels read them back.  The
-// arithmetic repeats the oracle's operation order (no contraction in this
-// translation unit, IEEE divisions), so the face values agree bit for bit.
+// Device layout.  k_weno_terms and k_weno_dp form dP for every direction at
+// every zone that a face reconstruction needs (compute_df_line_new
+// interpline.c:3557, from the zone's state and its two neighbours' along the
+// direction).  Along x1, where a line is contiguous in memory,
+// k_weno_recon_x1 gives each lane of a wave one zone of the line: the lane
+// forms its zone's weights, takes its neighbours' from the lanes two away for
+// the ratio test and the neighbours' ratios from the lanes one away, and
+// writes the zone's faces (coalesced loads and stores, no carried state).
+// Along x2 and x3, k_weno_recon sweeps segments of each storage line of one
+// primitive in one thread (lanes are consecutive i: coalesced), carrying the
+// last eight values and a four-zone pipeline of weights in registers.  The
+// face-flux kernels read the left and right face values of zones -1 .. N
+// back.  The arithmetic repeats the oracle's operation order (no contraction
+// in this translation unit, IEEE divisions), so the face values agree bit for
+// bit.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <float.h>
+#include <algorithm>
 #include "dgrid.h"
 #include "gkern.h"
 #include "hhost.h"
@@ -177,24 +183,39 @@ __device__ __forceinline__ void weights2(const double* u, double* L2, double* R2
   R2[1] = r1 / n;
 }
 
-// SMONO indicator (compute_mono_indicator_point_eno5)
+// SMONO indicator (compute_mono_indicator_point_eno5).  The indicator is
+// min over the three orders of |minmod(left, right)| against a round-off
+// cut, clamped to [0, 1].  It is evaluated lazily, with the same values:
+// minmod is 0 whenever its arguments are not strictly of one sign, and a
+// quotient by 12 keeps its dividend's sign (or is 0), so an order whose
+// numerators already disagree in sign gives 0 without its divisions; m = 0
+// gives the indicator 0 and m >= cut gives 1 (m / cut >= 1 exactly).
+__device__ __forceinline__ bool same_sign(double a, double b) { return (a > 0 && b > 0) || (a < 0 && b < 0); }
 __device__ __forceinline__ double smono(const double* y)
 {
-  double L1 = (((-3 * y[-2] + y[2]) + (-10 * y[-1] - 6 * y[1])) + 18 * y[0]) / 12.;
-  const double L2 = (((11 * y[-2] - y[2]) + (-20 * y[-1] + 4 * y[1])) + 6 * y[0]) / 12.;
   const double L3 = (((-3 * y[-2] - y[2]) + (10 * y[-1] + 6 * y[1])) - 12 * y[0]) / 2.;
-  double R1 = (((-y[-2] + 3 * y[2]) + (6 * y[-1] + 10 * y[1])) - 18 * y[0]) / 12.;
-  const double R2 = (((-y[-2] + 11 * y[2]) + (4 * y[-1] - 20 * y[1])) + 6 * y[0]) / 12.;
   const double R3 = (((y[-2] + 3 * y[2]) - (6 * y[-1] + 10 * y[1])) + 12 * y[0]) / 2.;
-  L1 = wminmod(L1, y[-1] - y[-2]);
-  R1 = wminmod(R1, y[2] - y[1]);
+  const double m3 = fabs(wminmod(L3, R3));
+  if (m3 == 0.0) return 0.0;
+  const double L2n = (((11 * y[-2] - y[2]) + (-20 * y[-1] + 4 * y[1])) + 6 * y[0]);
+  const double R2n = (((-y[-2] + 11 * y[2]) + (4 * y[-1] - 20 * y[1])) + 6 * y[0]);
+  if (!same_sign(L2n, R2n)) return 0.0;
+  const double L1n = (((-3 * y[-2] + y[2]) + (-10 * y[-1] - 6 * y[1])) + 18 * y[0]);
+  const double R1n = (((-y[-2] + 3 * y[2]) + (6 * y[-1] + 10 * y[1])) - 18 * y[0]);
+  const double dl = y[-1] - y[-2], dr = y[2] - y[1];
+  if (!same_sign(L1n, dl) || !same_sign(R1n, dr)) return 0.0;
+  const double L1 = wminmod(L1n / 12., dl);
+  const double R1 = wminmod(R1n / 12., dr);
+  const double m2 = fabs(wminmod(L2n / 12., R2n / 12.));
   double m = kBIG;
   m = wmin(m, fabs(wminmod(L1, R1)));
-  m = wmin(m, fabs(wminmod(L2, R2)));
-  m = wmin(m, fabs(wminmod(L3, R3)));
+  m = wmin(m, m2);
+  m = wmin(m, m3);
+  if (m == 0.0) return 0.0;
   const double norm = fabs(y[0]) + ((fabs(y[-1]) + fabs(y[1])) + (fabs(y[-2]) + fabs(y[2])));
   const double cut = norm * kSqrtWeps + kMinNum;
-  return wmax(0., wmin(1., (m - 0.0) / (cut - 0.0)));
+  if (m >= cut) return 1.0;
+  return (m - 0.0) / (cut - 0.0);
 }
 
 // five-point interpolant at the left (RIGHT = false) or right face
@@ -442,37 +463,34 @@ __global__ __launch_bounds__(kBlock) void k_weno_dp(Dims d, Box z, const double*
   }
 }
 
-// Segments of the storage lines along DIR, one primitive each, per thread.
-// Line index c = 0 .. n-1 is the sweep coordinate m = c - 4 (m = -4 .. N+3);
-// weights exist for c = 2 .. n-3, ratios for c = 2 .. n-3 (the left
+// Segments of the storage lines along DIR (2 or 3), one primitive each, per
+// thread.  Line index c = 0 .. n-1 is the sweep coordinate m = c - 4 (m = -4
+// .. N+3); weights exist for c = 2 .. n-3, ratios for c = 2 .. n-3 (the left
 // comparison from c = 4, the right one up to c = n-5), faces for c = 3 ..
 // n-4 (m = -1 .. N).  A thread owns the faces of zones f0 .. f1 and starts
 // five zones earlier so that the ratios of f0-1 and its neighbours exist.  At
 // step s the value y[s] arrives, zone s-2 gets its weights, zone s-4 its
-// ratio and zone s-5 its faces.
+// ratio and zone s-5 its faces.  Threads are the lines of the face box's
+// transverse plane with i fastest (no lanes padded per row); blockIdx.y is
+// (primitive, segment).
 template <int DIR>
-__global__ __launch_bounds__(kBlock) void k_weno_recon(Dims d, Box b, int seg, int nseg,
-                                                         const double* __restrict__ pb,
+__global__ __launch_bounds__(kBlock) void k_weno_recon(Dims d, Box b, int seg, const double* __restrict__ pb,
                                                          const double* __restrict__ dpd, double* __restrict__ L,
                                                          double* __restrict__ R)
 {
-  // lanes: first transverse index (j for DIR 1, i otherwise); y: (segment,
-  // second transverse index, variable)
-  const int n0 = DIR == 1 ? b.nj : b.ni;
+  const int n0 = b.ni;
   const int n1 = DIR == 3 ? b.nj : b.nk;
-  const int t0 = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int q = blockIdx.y * (kBlock / 64) + (threadIdx.x >> 6);
-  if (t0 >= n0 || q >= n1 * HARM_NPR * nseg) return;
-  const int v = q % HARM_NPR, r1 = q / HARM_NPR;
-  const int t1 = r1 % n1, sg = r1 / n1;
+  const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= (long)n0 * n1) return;
+  const int v = blockIdx.y % HARM_NPR, sg = blockIdx.y / HARM_NPR;
+  const int t0 = (int)(t % n0), t1 = (int)(t / n0);
   int i, j, k;
-  if (DIR == 1) { i = -4; j = b.js + t0; k = b.ks + t1; }
-  else if (DIR == 2) { i = b.is + t0; j = -4; k = b.ks + t1; }
+  if (DIR == 2) { i = b.is + t0; j = -4; k = b.ks + t1; }
   else { i = b.is + t0; j = b.js + t1; k = -4; }
-  const int N = DIR == 1 ? d.n1 : DIR == 2 ? d.n2 : d.n3;
+  const int N = DIR == 2 ? d.n2 : d.n3;
   const int n = N + 8;
   const int f0 = 3 + sg * seg, f1 = min(f0 + seg - 1, n - 4);
-  const long st = DIR == 1 ? 1 : DIR == 2 ? (long)d.ps1 : (long)d.ps1 * d.ns2;
+  const long st = DIR == 2 ? (long)d.ps1 : (long)d.ps1 * d.ns2;
   const long c0 = hd::cidx(d, i, j, k);
   const double* __restrict__ a = pb + (long)v * d.ncell + c0;
   double* __restrict__ Lo = L + (long)v * d.ncell + c0;
@@ -522,6 +540,68 @@ __global__ __launch_bounds__(kBlock) void k_weno_recon(Dims d, Box b, int seg, i
   }
 }
 
+// Direction 1 (lines contiguous in memory), lane-parallel: a group of GL
+// lanes holds GL consecutive zones c = (GL - 6) w + g (g = 0 .. GL-1) of one
+// storage line, so every load and store is contiguous.  Each lane forms its
+// zone's weights, then its ratio from the weights two lanes away, then the
+// faces from the ratios one lane away: lanes 3 .. GL-4 of a group own faces,
+// and nw = ceil((n - 6) / (GL - 6)) groups cover a line.  Groups are ordered
+// j fastest, so the two groups of a wave are neighbouring lines at the same
+// radii.  Same indexing and operations as the sweep above.  (Along x2 and x3
+// the same scheme, with strided loads, measured slower than the sweep.)
+constexpr int kX1Lanes = 32;
+__global__ __launch_bounds__(kBlock) void k_weno_recon_x1(Dims d, Box b, int nw, const double* __restrict__ pb,
+                                                            const double* __restrict__ dpd, double* __restrict__ L,
+                                                            double* __restrict__ R)
+{
+  constexpr int GL = kX1Lanes;
+  const int g = threadIdx.x & (GL - 1);
+  const long wave0 = (long)blockIdx.x * (kBlock / GL) + (threadIdx.x & ~63) / GL;
+  const long W = (long)blockIdx.x * (kBlock / GL) + threadIdx.x / GL;  // line segment
+  const long nall = (long)b.nj * b.nk * HARM_NPR * nw;
+  if (wave0 >= nall) return;  // the whole wave is past the end
+  const bool act = W < nall;
+  const long Wc = act ? W : nall - 1;
+  const int jj = (int)(Wc % b.nj);
+  long r = Wc / b.nj;
+  const int ws = (int)(r % nw);
+  r /= nw;
+  const int kk = (int)(r % b.nk), v = (int)(r / b.nk);
+  const int n = d.n1 + 8;
+  const int c = (GL - 6) * ws + g;
+  const long c0 = hd::cidx(d, -4, b.js + jj, b.ks + kk);
+  const double* __restrict__ a = pb + (long)v * d.ncell + c0;
+
+  double y[5];
+#pragma unroll
+  for (int m = 0; m < 5; m++) {
+    const int cm = c - 2 + m;
+    y[m] = (act && cm >= 0 && cm < n) ? a[cm] : 0.0;
+  }
+  ZoneW z = {};
+  if (act && c >= 2 && c <= n - 3) zone_weights(y + 2, z);
+  // ratio of zone c against zones c-2 and c+2
+  const double ul = __shfl_up(z.u[0], 2, GL), ur = __shfl_down(z.u[2], 2, GL);
+  double fd = 0.0;
+  if (act && c >= 2 && c <= n - 3 && g >= 2 && g <= GL - 3) {
+    const double wl = c >= 4 ? side_ratio(z.u[2], ul) : kBIG;
+    const double wr = c <= n - 5 ? side_ratio(z.u[0], ur) : kBIG;
+    fd = ratio_lof(wl, wr);
+  }
+  const double fdm = __shfl_up(fd, 1, GL), fdp = __shfl_down(fd, 1, GL);
+  if (act && c >= 3 && c <= n - 4 && g >= 3 && g <= GL - 4) {
+    const double dPi = dpd[c0 + c];
+    const double ss = wmin((fabs(dPi) - 0.5 * 0.5) / (0.5 * 0.5), 1.);
+    double lof = fd;
+    lof = wmax(lof, ss * fdp);
+    lof = wmax(lof, ss * fdm);
+    double lo, ro;
+    zone_faces(y + 2, z, lof, lo, ro);
+    L[(long)v * d.ncell + c0 + c] = lo;
+    R[(long)v * d.ncell + c0 + c] = ro;
+  }
+}
+
 }  // namespace
 
 // dP of every active direction over the zones the face reconstructions read;
@@ -539,24 +619,34 @@ void launch_weno_dp(const Dims& d, hipStream_t s, const double* pb, const double
 }
 
 // face values L, R (SoA like pb) of zones -1 .. N along dir over the
-// transverse range of the face box b; lines are cut into segments of about
-// kSeg zones so that every direction keeps several hundred thousand threads
+// transverse range of the face box b.
 void launch_weno_recon(int dir, const Dims& d, const Box& b, hipStream_t s, const double* pb, const double* dp,
                        double* L, double* R)
 {
-  constexpr int kSeg = 32;
-  const int N = dir == 1 ? d.n1 : dir == 2 ? d.n2 : d.n3;
-  const int nf = N + 2;  // zones -1 .. N
-  const int nseg = (nf + kSeg - 1) / kSeg;
-  const int seg = (nf + nseg - 1) / nseg;
-  const int n0 = dir == 1 ? b.nj : b.ni;
-  const int n1 = dir == 3 ? b.nj : b.nk;
-  const long ny = (long)n1 * HARM_NPR * nseg;
-  const dim3 grid((unsigned)((n0 + 63) / 64), (unsigned)((ny + kBlock / 64 - 1) / (kBlock / 64)));
   const double* dpd = dp + (long)(dir - 1) * d.ncell;
-  if (dir == 1) hipLaunchKernelGGL(k_weno_recon<1>, grid, dim3(kBlock), 0, s, d, b, seg, nseg, pb, dpd, L, R);
-  if (dir == 2) hipLaunchKernelGGL(k_weno_recon<2>, grid, dim3(kBlock), 0, s, d, b, seg, nseg, pb, dpd, L, R);
-  if (dir == 3) hipLaunchKernelGGL(k_weno_recon<3>, grid, dim3(kBlock), 0, s, d, b, seg, nseg, pb, dpd, L, R);
+  if (dir == 1) {
+    constexpr int GL = kX1Lanes;
+    const int n = d.n1 + 8;
+    const int nw = (n - 6 + GL - 7) / (GL - 6);
+    const long nall = (long)b.nj * b.nk * HARM_NPR * nw;
+    const long blocks = (nall * GL + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_weno_recon_x1, dim3((unsigned)blocks), dim3(kBlock), 0, s, d, b, nw, pb, dpd, L, R);
+    return;
+  }
+  // directions 2 and 3: lines cut into segments of at least kMinSeg faces,
+  // about kWaveTarget waves in all (a few per SIMD)
+  constexpr long kWaveTarget = 10240;
+  constexpr int kMinSeg = 16;
+  const int N = dir == 2 ? d.n2 : d.n3;
+  const int nf = N + 2;  // zones -1 .. N
+  const long nl = (long)b.ni * (dir == 3 ? b.nj : b.nk);
+  const long waves1 = (nl * HARM_NPR + 63) / 64;
+  int nseg = (int)std::max(1L, std::min((long)(nf / kMinSeg), (kWaveTarget + waves1 - 1) / waves1));
+  const int seg = (nf + nseg - 1) / nseg;
+  nseg = (nf + seg - 1) / seg;
+  const dim3 grid((unsigned)((nl + kBlock - 1) / kBlock), (unsigned)(HARM_NPR * nseg));
+  if (dir == 2) hipLaunchKernelGGL(k_weno_recon<2>, grid, dim3(kBlock), 0, s, d, b, seg, pb, dpd, L, R);
+  if (dir == 3) hipLaunchKernelGGL(k_weno_recon<3>, grid, dim3(kBlock), 0, s, d, b, seg, pb, dpd, L, R);
 }
 
 }  // namespace hk
