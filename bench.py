@@ -83,6 +83,14 @@ def algorithmic_bytes(g, kernel):
         return ghosts * 128
     if kernel == "fixup":
         return zones * 4  # the failure flags; failed zones are few
+    if kernel == "recon":
+        # WENO5BND per substep, four timed launches: the dP pass (read the
+        # primitives 64, write dP of three directions 24) and one line pass
+        # per direction (read the primitives 64 and dP 8, write the left and
+        # right face values 128); geometry per zone column.  Averaged per launch.
+        ndir = e0 + e1 + e2
+        per_substep = zones * (88 + ndir * 200) + n1 * n2 * geom
+        return per_substep / (1 + ndir)
     return None
 
 

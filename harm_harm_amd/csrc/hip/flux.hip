@@ -86,12 +86,15 @@ __device__ __forceinline__ void flux_body(Dims d, FaceTile T, const double* __re
                                                  int fluxtype, int do_ct, double* __restrict__ ds, const int blk,
                                                  const double* __restrict__ wl, const double* __restrict__ wr)
 {
-  __shared__ double s_l[HARM_NPR * kCellMax], s_r[HARM_NPR * kCellMax];
+  __shared__ double s_l[RECON == HARM_RECON_WENO5 ? 1 : HARM_NPR * kCellMax],
+      s_r[RECON == HARM_RECON_WENO5 ? 1 : HARM_NPR * kCellMax];
   const int tile = blk;
   const int st = (DIR == 1) ? 1 : (DIR == 2) ? d.ps1 : d.ps1 * d.ns2;
   const Box& b = T.b;
 
   // ---- phase 1: reconstruct the tile's cells ----
+  // (WENO5BND: the line reconstruction already wrote every cell's face
+  // values, and phase 2 reads them straight from wl, wr)
   // (32-bit index arithmetic: a face box stays far below 2^31 faces)
   int f0 = 0, g_lo = 0, ncell, bc = 0, bd = 0, pl = 0;
   const int rowc = T.nd + 1;  // dir 1: cells per row
@@ -119,7 +122,8 @@ __device__ __forceinline__ void flux_body(Dims d, FaceTile T, const double* __re
   // storage offset of every cell of the tile (-1: outside the face box),
   // decoded once per cell so the loop over (variable, cell) items below needs
   // no integer division
-  __shared__ int s_off[kCellMax];
+  __shared__ int s_off[RECON == HARM_RECON_WENO5 ? 1 : kCellMax];
+  if constexpr (RECON != HARM_RECON_WENO5) {
   for (int slot = threadIdx.x; slot < ncell; slot += kBlock) {
     int i, j, k, off = -1;
     if (DIR == 1) {
@@ -155,17 +159,11 @@ __device__ __forceinline__ void flux_body(Dims d, FaceTile T, const double* __re
     const int off = s_off[slot];
     if (off >= 0) {
       double l, r;
-      if (RECON == HARM_RECON_WENO5) {
-        // the face values of the line reconstruction (weno.hip)
-        l = wl[(long)v * d.ncell + off];
-        r = wr[(long)v * d.ncell + off];
-      } else {
-        const double* a = pb + (long)v * d.ncell + off;
-        double y[5];
+      const double* a = pb + (long)v * d.ncell + off;
+      double y[5];
 #pragma unroll
-        for (int m = 0; m < 5; m++) y[m] = a[(m - 2) * st];
-        hd::recon_cell<RECON>(y + 2, l, r);
-      }
+      for (int m = 0; m < 5; m++) y[m] = a[(m - 2) * st];
+      hd::recon_cell<RECON>(y + 2, l, r);
       s_l[v * kCellMax + slot] = l;
       s_r[v * kCellMax + slot] = r;
     }
@@ -177,6 +175,7 @@ __device__ __forceinline__ void flux_body(Dims d, FaceTile T, const double* __re
     }
   }
   __syncthreads();
+  }
 
   // ---- phase 2: one face per lane ----
   double dtij = kBIG;
@@ -209,16 +208,26 @@ __device__ __forceinline__ void flux_body(Dims d, FaceTile T, const double* __re
   }
   if (ok) {
     double p_l[HARM_NPR], p_r[HARM_NPR];
+    const long c = hd::cidx(d, i, j, k);
+    if constexpr (RECON == HARM_RECON_WENO5) {
+      // the right face value of the cell behind the face, the left one of the
+      // cell ahead (weno.hip)
 #pragma unroll
-    for (int v = 0; v < HARM_NPR; v++) {
-      p_l[v] = s_r[v * kCellMax + sl];
-      p_r[v] = s_l[v * kCellMax + sr];
+      for (int v = 0; v < HARM_NPR; v++) {
+        p_l[v] = wr[(long)v * d.ncell + c - st];
+        p_r[v] = wl[(long)v * d.ncell + c];
+      }
+    } else {
+#pragma unroll
+      for (int v = 0; v < HARM_NPR; v++) {
+        p_l[v] = s_r[v * kCellMax + sl];
+        p_r[v] = s_l[v * kCellMax + sr];
+      }
     }
     Geom G;
     hd::load_geom(gface, hd::pidx(d, i, j), d.nplane, G);
     double Fo[HARM_NPR];
     const double ct = face_riemann<DIR>(p_l, p_r, G, P, fluxtype, do_ct, Fo);
-    const long c = hd::cidx(d, i, j, k);
 #pragma unroll
     for (int pl = 0; pl < HARM_NPR; pl++) F[(long)pl * d.ncell + c] = Fo[pl];
     dtij = P.cour * P.dx[DIR] / ct;

@@ -1,8 +1,9 @@
 // gkern.h -- definitions shared by the grid step's translation units: the
 // device scalar slots, run parameters, face-tile plans, the XCD-aware block
 // order, block reductions and the per-grid descriptor of batched launches.
-// The face-flux kernels live in their own translation unit (flux.hip, built
-// with FMA contraction); everything else of the step is in grid.hip.
+// The face-flux kernels live in their own translation unit (flux.hip), the
+// WENO5BND line reconstruction in weno.hip; everything else of the step is in
+// grid.hip.
 #pragma once
 #include <hip/hip_runtime.h>
 #include "dgrid.h"

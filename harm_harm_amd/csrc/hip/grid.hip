@@ -5,11 +5,12 @@
 // geometry is stored once per (i,j) plane and reused across phi.  Each hot loop
 // of advance() is one kernel:
 //   k_flux<dir>   reconstruct + Riemann flux per face, fused   (flux.c:852;
-//                 in flux.hip, built with FMA contraction)
+//                 in flux.hip; WENO5BND face values from weno.hip)
 //   k_emf/k_ct_all Toth corner EMFs and face B fluxes          (fluxct.c:186)
 //   k_update      U(pi), flux divergence, connection source,
 //                 RK combine and the source dt limit           (advance.c:420)
-//   k_invert      Newton inversion + floors                    (advance.c:748)
+//   k_invert      Newton inversion (hot, check, cold retry)    (advance.c:748)
+//   k_floors      final substep: floors + Lorentz-factor limit (fixup.c:561)
 //   k_bound_*     radial extrapolation, polar axis, periodic   (bounds.tools.c)
 //   k_post_fixup  failure averaging                            (fixup.c:1061)
 // The timestep lives in device memory: reductions use atomicMin on the bit
@@ -283,17 +284,17 @@ __global__ __launch_bounds__(kBlock) void k_update(Dims d, Params P, const doubl
   update_body<DM, MODE>(d, P, pi, pb, F1, F2, F3, ui, uf, ucum, gcent, conn, cuf0, cuf1, cuf2, cun0, cun1, cun2, timeorder, ds, xcd_tile(blockIdx.x, gridDim.x));
 }
 
-// DIAG: also keeps the states the final substep's fixes started from
-// (invert_zone; fx = {p0, after-floor, flags}).
-template <bool DIAG = false>
+// The grid inversion is two passes.  The solve pass (every substep) is
+// invert_zone without the floors: the hot Newton solve, the inversion check
+// and the cold retry.  On the final substep the floors pass then applies
+// fixup1zone's floors and the Lorentz-factor limit to every zone
+// (floors_zone), reading the solve's primitives and flags back; kept apart,
+// the solve does not carry the floors' own inversion in its registers.  The
+// failure count of the final substep is taken after the floors.
 __device__ __forceinline__ void invert_body(Dims d, Params P, const double* __restrict__ U,
-                                                   const double* __restrict__ pg, double* __restrict__ pf,
-                                                   int32_t* __restrict__ pflag,
-                                                   const double* __restrict__ gcent, const double* __restrict__ floors,
-                                                   int finalstep, unsigned long long* __restrict__ di, const int blk,
-                                                   double* __restrict__ fx_p0 = nullptr,
-                                                   double* __restrict__ fx_pfl = nullptr,
-                                                   uint8_t* __restrict__ fx_flag = nullptr)
+                                            const double* __restrict__ pg, double* __restrict__ pf,
+                                            int32_t* __restrict__ pflag, const double* __restrict__ gcent,
+                                            int finalstep, unsigned long long* __restrict__ di, const int blk)
 {
   const int t = blk * kBlock + threadIdx.x;
   const int nact = d.n1 * d.n2 * d.n3;
@@ -307,25 +308,66 @@ __device__ __forceinline__ void invert_body(Dims d, Params P, const double* __re
     double Uz[HARM_NPR];
 #pragma unroll
     for (int pl = 0; pl < HARM_NPR; pl++) Uz[pl] = U[pl * nc + o];
-    nf = invert_zone<false, DIAG>(d, P, Uz, U, pg, pf, pflag, gcent, floors, finalstep, o, pp, ns, 0.0, nullptr, fx_p0,
-                                  fx_pfl, fx_flag) > 0;
+    const int fl = invert_zone<false, false, false>(d, P, Uz, U, pg, pf, pflag, gcent, nullptr, finalstep, o, pp, ns);
+    nf = !finalstep && fl > 0;
   }
   newton_counts_to(ns, nf, di);
 }
 
-// Four waves per SIMD (128 VGPRs, a few bytes of spill outside the Newton
-// loop): measured faster than three on the torus step.
-template <bool DIAG>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_invert(Dims d, Params P, const double* __restrict__ U,
-                                                   const double* __restrict__ pg, double* __restrict__ pf,
-                                                   int32_t* __restrict__ pflag,
-                                                   const double* __restrict__ gcent, const double* __restrict__ floors,
-                                                   int finalstep, unsigned long long* __restrict__ di,
-                                                   double* __restrict__ fx_p0, double* __restrict__ fx_pfl,
-                                                   uint8_t* __restrict__ fx_flag)
+// DIAG: also keeps the states the fixes started from (floors_zone; fx = {p0,
+// after-floor, flags}).
+template <bool DIAG = false>
+__device__ __forceinline__ void floors_body(Dims d, Params P, double* __restrict__ pf, int32_t* __restrict__ pflag,
+                                            const double* __restrict__ gcent, const double* __restrict__ floors,
+                                            unsigned long long* __restrict__ di, const int blk,
+                                            double* __restrict__ fx_p0 = nullptr,
+                                            double* __restrict__ fx_pfl = nullptr,
+                                            uint8_t* __restrict__ fx_flag = nullptr)
 {
-  invert_body<DIAG>(d, P, U, pg, pf, pflag, gcent, floors, finalstep, di, xcd_tile(blockIdx.x, gridDim.x), fx_p0,
-                    fx_pfl, fx_flag);
+  const int t = blk * kBlock + threadIdx.x;
+  const int nact = d.n1 * d.n2 * d.n3;
+  int ns = 0, nf = 0;
+  if (t < nact) {
+    int i, j, k;
+    zone_ijk(d, t, i, j, k);
+    const long o = hd::cidx(d, i, j, k);
+    const long pp = hd::pidx(d, i, j);
+    const long nc = d.ncell;
+    Geom G;
+    hd::load_geom(gcent, pp, d.nplane, G);
+    double pz[HARM_NPR];
+#pragma unroll
+    for (int pl = 0; pl < HARM_NPR; pl++) pz[pl] = pf[pl * nc + o];
+    const int fl0 = pflag[o];
+    bool changed;
+    const int fl = floors_zone<DIAG>(d, P, G, floors, pz, fl0, o, pp, ns, &changed, fx_p0, fx_pfl, fx_flag);
+    if (changed) {
+#pragma unroll
+      for (int pl = 0; pl < HARM_NPR; pl++) pf[pl * nc + o] = pz[pl];
+    }
+    if (fl != fl0) pflag[o] = fl;
+    nf = fl > 0;
+  }
+  newton_counts_to(ns, nf, di);
+}
+
+// Four waves per SIMD (128 VGPRs): measured faster than three on the torus
+// step.
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_invert(
+    Dims d, Params P, const double* __restrict__ U, const double* __restrict__ pg, double* __restrict__ pf,
+    int32_t* __restrict__ pflag, const double* __restrict__ gcent, int finalstep, unsigned long long* __restrict__ di)
+{
+  invert_body(d, P, U, pg, pf, pflag, gcent, finalstep, di, xcd_tile(blockIdx.x, gridDim.x));
+}
+
+template <bool DIAG>
+__global__ __launch_bounds__(kBlock) void k_floors(Dims d, Params P, double* __restrict__ pf,
+                                                   int32_t* __restrict__ pflag, const double* __restrict__ gcent,
+                                                   const double* __restrict__ floors,
+                                                   unsigned long long* __restrict__ di, double* __restrict__ fx_p0,
+                                                   double* __restrict__ fx_pfl, uint8_t* __restrict__ fx_flag)
+{
+  floors_body<DIAG>(d, P, pf, pflag, gcent, floors, di, xcd_tile(blockIdx.x, gridDim.x), fx_p0, fx_pfl, fx_flag);
 }
 
 // ---------------------------------------------------------------------------
@@ -1339,14 +1381,15 @@ harm_status enq_invert(harm_handle* h, int which_pb, int which_pf, int finalstep
     HH_TRY(launch_invert_grid_mlp(h->guess, d, h->P, U, h->arr[which_pb], h->arr[which_pf], h->pflag,
                                   h->geom[HARM_LOC_CENT], h->floors, finalstep, h->di, h->stream,
                                   diag ? h->fx_p0 : nullptr, h->fx_pfl, h->fx_flag));
-  } else if (diag) {
-    hipLaunchKernelGGL(k_invert<true>, dim3(nb), dim3(kBlock), 0, h->stream, d, h->P, U, h->arr[which_pb],
-                       h->arr[which_pf], h->pflag, h->geom[HARM_LOC_CENT], h->floors, finalstep, h->di, h->fx_p0,
-                       h->fx_pfl, h->fx_flag);
   } else {
-    hipLaunchKernelGGL(k_invert<false>, dim3(nb), dim3(kBlock), 0, h->stream, d, h->P, U, h->arr[which_pb],
-                       h->arr[which_pf], h->pflag, h->geom[HARM_LOC_CENT], h->floors, finalstep, h->di, nullptr,
-                       nullptr, nullptr);
+    hipLaunchKernelGGL(k_invert, dim3(nb), dim3(kBlock), 0, h->stream, d, h->P, U, h->arr[which_pb],
+                       h->arr[which_pf], h->pflag, h->geom[HARM_LOC_CENT], finalstep, h->di);
+    if (finalstep && diag)
+      hipLaunchKernelGGL(k_floors<true>, dim3(nb), dim3(kBlock), 0, h->stream, d, h->P, h->arr[which_pf], h->pflag,
+                         h->geom[HARM_LOC_CENT], h->floors, h->di, h->fx_p0, h->fx_pfl, h->fx_flag);
+    else if (finalstep)
+      hipLaunchKernelGGL(k_floors<false>, dim3(nb), dim3(kBlock), 0, h->stream, d, h->P, h->arr[which_pf], h->pflag,
+                         h->geom[HARM_LOC_CENT], h->floors, h->di, nullptr, nullptr, nullptr);
   }
   prof_end(h, HARM_K_INVERT);
   if (diag)
@@ -1571,7 +1614,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
 {
   BATCH_DECODE(true)
   invert_body(g.d, g.P, gmem(g.arr[finalstep ? HARM_ARR_UCUM : HARM_ARR_UF]), gmem(g.arr[which_pb]),
-              gmem(g.arr[which_pf]), gmem(g.pflag), gmem(g.geom[HARM_LOC_CENT]), gmem(g.floors), finalstep,
+              gmem(g.arr[which_pf]), gmem(g.pflag), gmem(g.geom[HARM_LOC_CENT]), finalstep, gmem(g.di), lb);
+}
+
+__global__ __launch_bounds__(kBlock) void kb_floors(const GridRef* __restrict__ G, const int* __restrict__ pre, int ng,
+                                                    int which_pf)
+{
+  BATCH_DECODE(true)
+  floors_body(g.d, g.P, gmem(g.arr[which_pf]), gmem(g.pflag), gmem(g.geom[HARM_LOC_CENT]), gmem(g.floors),
               gmem(g.di), lb);
 }
 
@@ -1811,6 +1861,7 @@ harm_status enq_batch_substep(harm_batch* b, int to, int nto, const double CUf[4
 #undef BUPD_MODES
 #undef BUPD
   hipLaunchKernelGGL(kb_invert, gz, blk, 0, s, G, P(BS_ZONE), ng, pbb[to], pff[to], finalstep);
+  if (finalstep) hipLaunchKernelGGL(kb_floors, gz, blk, 0, s, G, P(BS_ZONE), ng, pff[to]);
   hipLaunchKernelGGL(kb_scal, dim3(g1), dim3(64), 0, s, G, ng, 2);
   // bounds, failure averaging, bounds
   for (int pass = 0; pass < 2; pass++) {

@@ -511,8 +511,9 @@ harm_status pack(harm_mlp* m, hipStream_t s)
 // the zones of k_invert in the same order; per wave the features of 64 zones
 // go through the network on the matrix cores, each zone's Newton solve then
 // starts from the predicted W' (falling back to the previous primitives when
-// it fails), and the final substep's floors follow, as in k_invert (DIAG:
-// keeping the states the fixes started from, as k_invert<true>).
+// it fails), and the final substep's floors follow in the same kernel
+// (floors_zone, as the plain inversion's floors pass applies them; DIAG:
+// keeping the states the fixes started from).
 template <bool DIAG>
 __global__ __launch_bounds__(kBlock) void k_invert_grid_mlp(const uint16_t* __restrict__ wbf,
                                                             const float* __restrict__ w32,

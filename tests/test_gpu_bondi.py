@@ -3,7 +3,7 @@ oracle, and the flow staying steady.  The boundary pass starts from scrambled
 ghosts; whole steps compare primitives and dt; a mixed batch of tori and
 Bondi flows advances bit for bit as its grids do alone.  Tolerance: 1e-12
 relative per variable against its largest magnitude (the Bondi boundaries use
-device pow/exp/sqrt, the fluxes FMA contraction)."""
+device pow/exp/sqrt, which may round the last bit differently from glibc)."""
 import numpy as np
 import pytest
 

@@ -7,9 +7,9 @@ floor only for the signed components; the density and internal energy of
 primitives are held to 1e-12 of their own value in every zone).  The update,
 CT, inversion and boundary kernels keep the oracle's operation order without
 FMA contraction and reproduce it bit for bit (the radial extrapolation aside:
-device exp/log may round the last bit differently from glibc); the face-flux
-kernels are built with fused multiply-adds and fewer divisions, so fluxes
-differ in the last bits.  Inversion flags and Newton iteration counts are
+device exp/log may round the last bit differently from glibc); so do the
+WENO5BND line reconstruction and the face-flux kernels (no contraction, IEEE
+divisions).  Inversion flags and Newton iteration counts are
 compared exactly, and the flag arrays over the whole storage box (their
 bounded ghost layer included) after failure averaging and after whole steps.
 """
