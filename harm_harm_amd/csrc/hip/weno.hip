@@ -501,7 +501,7 @@ __global__ __launch_bounds__(kBlock) void k_weno_recon(Dims d, Box b, int seg, c
 #pragma unroll
   for (int m = 0; m < 8; m++) y[m] = 0.0;
   ZoneW zw[4] = {};  // zw[0] = zone s-2 ... zw[3] = zone s-5
-  ZoneW z6 = {};     // zone s-6
+  double z6u0 = 0.0;  // zone s-6's first weight (its ratio partner)
   double fd[3] = {0, 0, 0};  // ratio fractions of zones s-6, s-5, s-4
   const int s0 = max(0, f0 - 5);
 #pragma unroll 1
@@ -510,17 +510,17 @@ __global__ __launch_bounds__(kBlock) void k_weno_recon(Dims d, Box b, int seg, c
     for (int m = 0; m < 7; m++) y[m] = y[m + 1];
     y[7] = s < n ? a[(long)s * st] : 0.0;
     // weights of zone s-2
-    z6 = zw[3];
+    z6u0 = zw[3].u[0];
 #pragma unroll
     for (int m = 3; m > 0; m--) zw[m] = zw[m - 1];
     if (s >= 4 && s <= n - 1 && s - 2 >= f0 - 3) zone_weights(y + 5, zw[0]);
-    // ratio of zone r = s-4 (zw[2]) against zones r-2 (z6) and r+2 (zw[0])
+    // ratio of zone r = s-4 (zw[2]) against zones r-2 (z6u0) and r+2 (zw[0])
     const int r = s - 4;
     fd[0] = fd[1];
     fd[1] = fd[2];
     fd[2] = 0.0;
     if (r >= 2 && r >= f0 - 1) {
-      const double wl = r >= 4 ? side_ratio(zw[2].u[2], z6.u[0]) : kBIG;
+      const double wl = r >= 4 ? side_ratio(zw[2].u[2], z6u0) : kBIG;
       const double wr = r <= n - 5 ? side_ratio(zw[2].u[0], zw[0].u[2]) : kBIG;
       fd[2] = ratio_lof(wl, wr);
     }
@@ -540,15 +540,46 @@ __global__ __launch_bounds__(kBlock) void k_weno_recon(Dims d, Box b, int seg, c
   }
 }
 
-// Direction 1 (lines contiguous in memory), lane-parallel: a group of GL
-// lanes holds GL consecutive zones c = (GL - 6) w + g (g = 0 .. GL-1) of one
-// storage line, so every load and store is contiguous.  Each lane forms its
-// zone's weights, then its ratio from the weights two lanes away, then the
-// faces from the ratios one lane away: lanes 3 .. GL-4 of a group own faces,
-// and nw = ceil((n - 6) / (GL - 6)) groups cover a line.  Groups are ordered
-// j fastest, so the two groups of a wave are neighbouring lines at the same
-// radii.  Same indexing and operations as the sweep above.  (Along x2 and x3
-// the same scheme, with strided loads, measured slower than the sweep.)
+// One zone per lane of a GL-lane group (g = 0 .. GL-1 its lane, c its line
+// index, y[0..4] the values of zones c-2 .. c+2): the lane forms its zone's
+// weights, then its ratio from the weights two lanes away, then its
+// lower-order fraction from the ratios one lane away and its faces.  Lanes 3
+// .. GL-4 of a group own faces (returns true and sets lo, ro); dp() reads the
+// zone's dP and is called by those lanes only.  Every lane of the group must
+// call it (the ratios travel by shuffles).  Same operations as the sweep.
+template <int GL, class DP>
+__device__ __forceinline__ bool lane_faces(const double* y, int c, int n, int g, bool act, DP dp, double& lo,
+                                           double& ro)
+{
+  ZoneW z = {};
+  if (act && c >= 2 && c <= n - 3) zone_weights(y + 2, z);
+  // ratio of zone c against zones c-2 and c+2
+  const double ul = __shfl_up(z.u[0], 2, GL), ur = __shfl_down(z.u[2], 2, GL);
+  double fd = 0.0;
+  if (act && c >= 2 && c <= n - 3 && g >= 2 && g <= GL - 3) {
+    const double wl = c >= 4 ? side_ratio(z.u[2], ul) : kBIG;
+    const double wr = c <= n - 5 ? side_ratio(z.u[0], ur) : kBIG;
+    fd = ratio_lof(wl, wr);
+  }
+  const double fdm = __shfl_up(fd, 1, GL), fdp = __shfl_down(fd, 1, GL);
+  if (act && c >= 3 && c <= n - 4 && g >= 3 && g <= GL - 4) {
+    const double dPi = dp();
+    const double ss = wmin((fabs(dPi) - 0.5 * 0.5) / (0.5 * 0.5), 1.);
+    double lof = fd;
+    lof = wmax(lof, ss * fdp);
+    lof = wmax(lof, ss * fdm);
+    zone_faces(y + 2, z, lof, lo, ro);
+    return true;
+  }
+  return false;
+}
+
+// Direction 1 (lines contiguous in memory): a group of GL lanes holds GL
+// consecutive zones c = (GL - 6) w + g of one storage line, so every load and
+// store is contiguous; nw = ceil((n - 6) / (GL - 6)) groups cover a line.
+// Groups are ordered j fastest, so the two groups of a wave are neighbouring
+// lines at the same radii.  (Along x2 and x3 the same scheme, with strided
+// loads or staged through LDS, measured slower than the sweep.)
 constexpr int kX1Lanes = 32;
 __global__ __launch_bounds__(kBlock) void k_weno_recon_x1(Dims d, Box b, int nw, const double* __restrict__ pb,
                                                             const double* __restrict__ dpd, double* __restrict__ L,
@@ -578,25 +609,8 @@ __global__ __launch_bounds__(kBlock) void k_weno_recon_x1(Dims d, Box b, int nw,
     const int cm = c - 2 + m;
     y[m] = (act && cm >= 0 && cm < n) ? a[cm] : 0.0;
   }
-  ZoneW z = {};
-  if (act && c >= 2 && c <= n - 3) zone_weights(y + 2, z);
-  // ratio of zone c against zones c-2 and c+2
-  const double ul = __shfl_up(z.u[0], 2, GL), ur = __shfl_down(z.u[2], 2, GL);
-  double fd = 0.0;
-  if (act && c >= 2 && c <= n - 3 && g >= 2 && g <= GL - 3) {
-    const double wl = c >= 4 ? side_ratio(z.u[2], ul) : kBIG;
-    const double wr = c <= n - 5 ? side_ratio(z.u[0], ur) : kBIG;
-    fd = ratio_lof(wl, wr);
-  }
-  const double fdm = __shfl_up(fd, 1, GL), fdp = __shfl_down(fd, 1, GL);
-  if (act && c >= 3 && c <= n - 4 && g >= 3 && g <= GL - 4) {
-    const double dPi = dpd[c0 + c];
-    const double ss = wmin((fabs(dPi) - 0.5 * 0.5) / (0.5 * 0.5), 1.);
-    double lof = fd;
-    lof = wmax(lof, ss * fdp);
-    lof = wmax(lof, ss * fdm);
-    double lo, ro;
-    zone_faces(y + 2, z, lof, lo, ro);
+  double lo, ro;
+  if (lane_faces<GL>(y, c, n, g, act, [&] { return dpd[c0 + c]; }, lo, ro)) {
     L[(long)v * d.ncell + c0 + c] = lo;
     R[(long)v * d.ncell + c0 + c] = ro;
   }
