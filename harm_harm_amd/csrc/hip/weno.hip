@@ -504,11 +504,20 @@ __global__ __launch_bounds__(kBlock) void k_weno_recon(Dims d, Box b, int seg, c
   double z6u0 = 0.0;  // zone s-6's first weight (its ratio partner)
   double fd[3] = {0, 0, 0};  // ratio fractions of zones s-6, s-5, s-4
   const int s0 = max(0, f0 - 5);
+  // the loads run two steps (values) and one step (dP) ahead of their use,
+  // so that their latency overlaps a step's arithmetic
+  double ya = s0 < n ? a[(long)s0 * st] : 0.0;
+  double yb = s0 + 1 < n ? a[(long)(s0 + 1) * st] : 0.0;
+  double dpa = s0 - 5 >= f0 ? dpl[(long)(s0 - 5) * st] : 0.0;
 #pragma unroll 1
   for (int s = s0; s <= f1 + 5; s++) {
 #pragma unroll
     for (int m = 0; m < 7; m++) y[m] = y[m + 1];
-    y[7] = s < n ? a[(long)s * st] : 0.0;
+    y[7] = ya;
+    ya = yb;
+    yb = s + 2 < n ? a[(long)(s + 2) * st] : 0.0;
+    const double dPi = dpa;
+    dpa = (s + 1 - 5 >= f0 && s + 1 - 5 <= f1) ? dpl[(long)(s + 1 - 5) * st] : 0.0;
     // weights of zone s-2
     z6u0 = zw[3].u[0];
 #pragma unroll
@@ -527,7 +536,6 @@ __global__ __launch_bounds__(kBlock) void k_weno_recon(Dims d, Box b, int seg, c
     // faces of zone f = s-5 (zw[3], values y[0..4] centred on y[2])
     const int f = s - 5;
     if (f >= f0) {
-      const double dPi = dpl[(long)(f)*st];
       const double ss = wmin((fabs(dPi) - 0.5 * 0.5) / (0.5 * 0.5), 1.);
       double lof = fd[1];
       lof = wmax(lof, ss * fd[2]);
@@ -576,31 +584,33 @@ __device__ __forceinline__ bool lane_faces(const double* y, int c, int n, int g,
 
 // Direction 1 (lines contiguous in memory): a group of GL lanes holds GL
 // consecutive zones c = (GL - 6) w + g of one storage line, so every load and
-// store is contiguous; nw = ceil((n - 6) / (GL - 6)) groups cover a line.
-// Groups are ordered j fastest, so the two groups of a wave are neighbouring
-// lines at the same radii.  (Along x2 and x3 the same scheme, with strided
-// loads or staged through LDS, measured slower than the sweep.)
+// store is contiguous; nw = ceil((n - 6) / (GL - 6)) groups cover a line.  A
+// wave holds the same segment of two neighbouring lines j (same radii); the
+// wave's segment, line pair, k and primitive are decoded once per wave in
+// scalar registers.  (Along x2 and x3 the same scheme, with strided loads or
+// staged through LDS, measured slower than the sweep.)
 constexpr int kX1Lanes = 32;
 __global__ __launch_bounds__(kBlock) void k_weno_recon_x1(Dims d, Box b, int nw, const double* __restrict__ pb,
                                                             const double* __restrict__ dpd, double* __restrict__ L,
                                                             double* __restrict__ R)
 {
   constexpr int GL = kX1Lanes;
-  const int g = threadIdx.x & (GL - 1);
-  const long wave0 = (long)blockIdx.x * (kBlock / GL) + (threadIdx.x & ~63) / GL;
-  const long W = (long)blockIdx.x * (kBlock / GL) + threadIdx.x / GL;  // line segment
-  const long nall = (long)b.nj * b.nk * HARM_NPR * nw;
-  if (wave0 >= nall) return;  // the whole wave is past the end
-  const bool act = W < nall;
-  const long Wc = act ? W : nall - 1;
-  const int jj = (int)(Wc % b.nj);
-  long r = Wc / b.nj;
-  const int ws = (int)(r % nw);
+  static_assert(64 / GL == 2, "two line groups per wave");
+  const int njp = (b.nj + 1) / 2;
+  const int wv = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)));
+  if (wv >= njp * nw * b.nk * HARM_NPR) return;  // the whole wave is past the end
+  int r = wv;
+  const int jp = r % njp;
+  r /= njp;
+  const int ws = r % nw;
   r /= nw;
-  const int kk = (int)(r % b.nk), v = (int)(r / b.nk);
+  const int kk = r % b.nk, v = r / b.nk;
+  const int g = threadIdx.x & (GL - 1);
+  const int jj = 2 * jp + ((threadIdx.x & 63) >= GL);
+  const bool act = jj < b.nj;
   const int n = d.n1 + 8;
   const int c = (GL - 6) * ws + g;
-  const long c0 = hd::cidx(d, -4, b.js + jj, b.ks + kk);
+  const long c0 = hd::cidx(d, -4, b.js + (act ? jj : b.nj - 1), b.ks + kk);
   const double* __restrict__ a = pb + (long)v * d.ncell + c0;
 
   double y[5];
@@ -642,8 +652,8 @@ void launch_weno_recon(int dir, const Dims& d, const Box& b, hipStream_t s, cons
     constexpr int GL = kX1Lanes;
     const int n = d.n1 + 8;
     const int nw = (n - 6 + GL - 7) / (GL - 6);
-    const long nall = (long)b.nj * b.nk * HARM_NPR * nw;
-    const long blocks = (nall * GL + kBlock - 1) / kBlock;
+    const long nwave = (long)((b.nj + 1) / 2) * nw * b.nk * HARM_NPR;
+    const long blocks = (nwave + kBlock / 64 - 1) / (kBlock / 64);
     hipLaunchKernelGGL(k_weno_recon_x1, dim3((unsigned)blocks), dim3(kBlock), 0, s, d, b, nw, pb, dpd, L, R);
     return;
   }
