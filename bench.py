@@ -10,9 +10,11 @@ resident in HBM before the timed region; the timed region is K steps between
 two device synchronisations.
 
 Prints one JSON line: the throughput, a roofline record for the dominant kernel
-(algorithmic HBM bytes per launch over its event-measured mean duration) and a
-CPU baseline (the C restatement, OpenMP, timed on this host's cores on a
-bounded sample of the same workload).
+(algorithmic HBM bytes per launch over its event-measured mean duration, with
+the measured traffic and FP64 rate from the committed counter summary), the
+whole step against HBM, the chosen form's learned inversion guess timed on the
+same state (`chosen_form`), and a CPU baseline (the C restatement, OpenMP,
+timed on this host's cores on a bounded sample of the same workload).
 """
 import argparse
 import json
@@ -26,7 +28,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 (vector and matrix)
 GRID = (256, 128, 64)
-COUNTERS = os.path.join(ROOT, "profiles", "counters_step_torus256x128x64_weno5_hll_rk3.md")
+COUNTERS = os.path.join(ROOT, "profiles", "counters_step_torus256x128x64_weno5bnd_hll_rk3.md")
 
 
 def measured_counters(path=COUNTERS):
@@ -64,8 +66,11 @@ def algorithmic_bytes(g, kernel):
         else:
             ni, nj, nk = n1 + 2 * e0, n2 + 2 * e1, n3 + 1
         faces = ni * nj * nk
-        # read 8 primitives per cell of the swept block, write 8 fluxes per face
-        return faces * (64 + 64) + ni * nj * geom
+        # PARA: read 8 primitives per cell of the swept block, write 8 fluxes
+        # per face; WENO5BND: read the 2 x 8 face values the line pass wrote
+        # instead of the primitives
+        import harm_harm_amd as H
+        return faces * ((128 if g.cfg.recon == H.RECON["weno5"] else 64) + 64) + ni * nj * geom
     zones = n1 * n2 * n3
     if kernel == "update":
         # inside a step every substep moves 448 B per zone: pb 64 + 3 face
@@ -495,6 +500,9 @@ def main():
     ap.add_argument("--mlp-plain-steps", type=int, default=200, help="large --mlp: plain steps before training")
     ap.add_argument("--mlp-samples", type=int, default=1 << 20, help="large --mlp: zones sampled for training")
     ap.add_argument("--mlp-train-steps", type=int, default=3000, help="large --mlp: Adam steps")
+    ap.add_argument("--no-chosen", action="store_true", help="step: skip the learned-guess (chosen form) timing")
+    ap.add_argument("--mlp-samples-step", type=int, default=1 << 18, help="step: zones sampled for training")
+    ap.add_argument("--mlp-train-steps-step", type=int, default=2000, help="step: Adam steps")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="before the run, one device-to-device copy of this many MiB (the reference dispatch "
                          "for scaling HBM byte counters)")
@@ -589,11 +597,21 @@ def main():
         fp64 = {"bound": "fp64", "kernel": dom, "achieved": tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": tf / FP64_PEAK_TFLOPS, "flops_per_launch": md["flops"],
                 "source": "rocprofv3 --pmc SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64, " + os.path.relpath(COUNTERS, ROOT)}
-    # whole step against HBM: algorithmic bytes of every kernel group per step
+    # whole step against HBM.  Algorithmic bytes per zone-update of the plain
+    # step, independent of how it is cut into kernels: per substep, recon +
+    # flux per direction read p 64 and write F 64 (384), CT 144, update 512,
+    # inversion 196 = 1236 B, three substeps.  Beside it the compulsory
+    # traffic of this build's kernel groups (the WENO5BND face values and dP
+    # make the split reconstruction move more than the fused unit would).
+    alg_zu = 3 * (3 * 128 + 144 + 512 + 196)
+    zps = zones * args.steps / el
     step_bytes = sum((algorithmic_bytes(g, k) or 0) * prof[k][1] / prof_steps for k in prof if prof[k][1])
-    step_roof = {"bytes_per_step_alg": step_bytes, "achieved": step_bytes / (el / args.steps) / 1e9,
-                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                 "frac": step_bytes / (el / args.steps) / 1e9 / HBM_PEAK_GBS}
+    step_roof = {"bytes_per_zone_update_alg": alg_zu, "achieved": zps * alg_zu / 1e9, "peak": HBM_PEAK_GBS,
+                 "unit": "GB/s", "frac": zps * alg_zu / 1e9 / HBM_PEAK_GBS,
+                 "bytes_per_step_kernel_groups": step_bytes,
+                 "kernel_groups_GBs": step_bytes / (el / args.steps) / 1e9}
+
+    chosen = None if args.no_chosen else chosen_form(g, cfg, args, el)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -627,9 +645,68 @@ def main():
         "step_roofline_hbm": step_roof,
         "kernels": kernels,
         "ener_tracking": ener,
+        "chosen_form": chosen,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+
+
+def chosen_form(g, cfg, args, el_plain):
+    """The step with the learned inversion guess (the chosen form's MLP
+    half; the fused persistent step kernel is not built): the network is
+    trained on the GPU from zones of the current state, then the same number
+    of steps is timed from that state with every inversion seeded by it, and
+    one step of each form from the same state is compared."""
+    import numpy as np
+    import harm_harm_amd as H
+    from harm_harm_amd.mlp import MLP
+    t0 = time.time()
+    n = args.mlp_samples_step
+    dU, dgeo, dp = g.sample_zones(n, seed=11)
+    X, T, ok = MLP.pairs(dU, dgeo, dp, n, gammafail=cfg.gammafail)
+    order = np.nonzero(ok.numpy())[0].astype(np.int32)
+    np.random.default_rng(0).shuffle(order)
+    m = MLP(seed=1)
+    m.fit_norm(X, T, ok, n)
+    losses = m.train(X, T, n, H.DeviceArray.from_numpy(order), args.mlp_train_steps_step, lr=3e-3, lr_end=3e-5,
+                     chunk=max(1, args.mlp_train_steps_step // 4))
+    t_train = time.time() - t0
+    p0, (t_s, dt_s), n0 = g.get("p"), g.time(), g.nstep()
+    # one step of each form from the same state
+    g.step(1)
+    b = g.active(g.get("p"))
+    (ns_plain1, nf_plain1), _ = g.newton_stats()
+    g.set("p", p0)
+    g.set_time(t_s, dt_s, n0)
+    g.set_inversion_guess(m)
+    g.step(1)
+    a = g.active(g.get("p"))
+    (ns_seed1, nf_seed1), _ = g.newton_stats()
+    rel = max(float(np.abs(a[..., v] - b[..., v]).max() / max(np.abs(b[..., v]).max(), 1e-300)) for v in range(8))
+    # timed steps with the guess (graph captured by the step above)
+    g.set("p", p0)
+    g.set_time(t_s, dt_s, n0)
+    g.sync()
+    t1 = time.perf_counter()
+    g.step(args.steps)
+    g.sync()
+    el = time.perf_counter() - t1
+    (_, _), (ns, nf) = g.newton_stats()
+    _, fb = g.guess_stats()
+    g.set_inversion_guess(None)
+    zones = cfg.n1 * cfg.n2 * cfg.n3
+    return {"form": "plain multi-kernel step with the MLP inversion guess (8-64-64-2 BF16 on MFMA); "
+                    "the fused persistent step kernel is not built",
+            "value": zones * args.steps / el, "unit": "zone-updates/s", "ms_per_step": el / args.steps * 1e3,
+            "speedup_over_plain": el_plain / el,
+            "newton_iterations_per_zone_substep": ns / (zones * 3.0 * args.steps),
+            "newton_iterations_per_zone_substep_plain_one_step": ns_plain1 / (zones * 3.0),
+            "newton_iterations_per_zone_substep_seeded_one_step": ns_seed1 / (zones * 3.0),
+            "inversion_failures": nf, "fallbacks_to_plain_guess": fb,
+            "one_step_max_rel_difference_vs_plain": rel,
+            "one_step_failures_plain_vs_seeded": [nf_plain1, nf_seed1],
+            "training": {"sampled_zones": n, "adam_steps": args.mlp_train_steps_step, "final_loss": losses[-1],
+                         "seconds": t_train}}
 
 
 if __name__ == "__main__":
