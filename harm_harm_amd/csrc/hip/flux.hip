@@ -262,8 +262,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) 
   }
   const GridRef& g = G[gi];
   flux_body<DIR, RECON>(g.d, g.ft[DIR], gmem(g.arr[which]), gmem(g.arr[HARM_ARR_F1 + DIR - 1]), gmem(g.geom[DIR]), g.P, g.fluxtype,
-                        do_ct, gmem(g.ds), lb, RECON == HARM_RECON_WENO5 ? gmem(g.wl) : nullptr,
-                        RECON == HARM_RECON_WENO5 ? gmem(g.wr) : nullptr);
+                        do_ct, gmem(g.ds), lb,
+                        RECON == HARM_RECON_WENO5 ? gmem(g.wl) + (long)(DIR - 1) * HARM_NPR * g.d.ncell : nullptr,
+                        RECON == HARM_RECON_WENO5 ? gmem(g.wr) + (long)(DIR - 1) * HARM_NPR * g.d.ncell : nullptr);
 }
 
 }  // namespace

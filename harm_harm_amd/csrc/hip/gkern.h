@@ -180,7 +180,7 @@ struct GridRef {
   double* ds;
   unsigned long long* di;
   double* facc;
-  const double* wl;  // WENO5BND face values of the current direction (or null)
+  const double* wl;  // WENO5BND face values, one block of HARM_NPR x ncell per direction (or null)
   const double* wr;
   FaceTile ft[4];
 };

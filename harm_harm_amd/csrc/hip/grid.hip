@@ -1047,8 +1047,10 @@ struct harm_handle {
   double* floors = nullptr;
   double* rcent = nullptr;
   double* emf = nullptr;
-  double* wl = nullptr;    // WENO5BND: face values of the current direction's lines (SoA like p)
+  double* wl = nullptr;    // WENO5BND: face values of the lines, one SoA block (like p) per direction
   double* wr = nullptr;
+  hipStream_t side[2] = {nullptr, nullptr};  // WENO5BND: directions 2 and 3 of fluxcalc, forked from stream
+  hipEvent_t evfork = nullptr, evjoin[2] = {nullptr, nullptr};
   double* wdp = nullptr;   // WENO5BND: dP measure per direction (3 x ncell)
   double* wtm = nullptr;   // WENO5BND: per-zone terms of dP (10 x ncell)
   double* facc = nullptr;  // boundary-flux partial sums
@@ -1289,6 +1291,10 @@ FaceTile plan_face_tile(const Dims& d, int dir, int td_force = 0)
   return T;
 }
 
+// WENO5BND face values of direction dir (its block of wl, wr)
+double* wl_dir(harm_handle* h, int dir) { return h->wl + (size_t)(dir - 1) * HARM_NPR * h->d.ncell; }
+double* wr_dir(harm_handle* h, int dir) { return h->wr + (size_t)(dir - 1) * HARM_NPR * h->d.ncell; }
+
 harm_status enq_fluxcalc(harm_handle* h, int which_p, int do_ct)
 {
   const Dims& d = h->d;
@@ -1302,22 +1308,43 @@ harm_status enq_fluxcalc(harm_handle* h, int which_p, int do_ct)
     launch_weno_dp(d, s, pb, h->geom[HARM_LOC_CENT], h->P, h->wtm, h->wdp);
     prof_end(h, HARM_K_RECON);
   }
-  // WENO5BND: each direction's lines are reconstructed into wl, wr just before its face kernel reads them
+  // WENO5BND: each direction's lines are reconstructed into its own block of
+  // wl, wr just before its face kernel reads them.  Outside the timed
+  // profiling run the three directions go out on three streams (forked after
+  // the dP pass, joined before the CT), so the line passes along x2 and x3,
+  // which wait on long division chains, share the GPU with the others; in a
+  // captured step they are parallel branches of the graph.
+  const bool fork = weno && !h->prof && h->side[0];
+  if (fork) HH_HIP(hipEventRecord(h->evfork, s));
+  auto dir_stream = [&](int dir) -> hipStream_t {
+    if (!fork || dir == 1) return s;
+    hipStream_t q = h->side[dir - 2];
+    (void)hipStreamWaitEvent(q, h->evfork, 0);
+    return q;
+  };
 #define LAUNCH_FLUX(DIR, F, G)                                                                              \
   do {                                                                                                      \
+    hipStream_t sd = dir_stream(DIR);                                                                       \
+    double* wl = weno ? wl_dir(h, DIR) : nullptr;                                                           \
+    double* wr = weno ? wr_dir(h, DIR) : nullptr;                                                           \
     if (weno) {                                                                                             \
       prof_begin(h);                                                                                        \
-      launch_weno_recon(DIR, d, h->ft[DIR].b, s, pb, h->wdp, h->wl, h->wr);                                 \
+      launch_weno_recon(DIR, d, h->ft[DIR].b, sd, pb, h->wdp, wl, wr);                                      \
       prof_end(h, HARM_K_RECON);                                                                            \
     }                                                                                                       \
     prof_begin(h);                                                                                          \
-    launch_flux(DIR, recon, h->ft[DIR], s, d, pb, F, G, h->P, ft, do_ct, h->ds, h->wl, h->wr);              \
+    launch_flux(DIR, recon, h->ft[DIR], sd, d, pb, F, G, h->P, ft, do_ct, h->ds, wl, wr);                   \
     prof_end(h, HARM_K_FLUX1 + DIR - 1);                                                                    \
+    if (fork && DIR > 1) HH_HIP(hipEventRecord(h->evjoin[DIR - 2], sd));                                    \
   } while (0)
   if (d.n1 > 1) LAUNCH_FLUX(1, h->arr[HARM_ARR_F1], h->geom[HARM_LOC_FACE1]);
   if (d.n2 > 1) LAUNCH_FLUX(2, h->arr[HARM_ARR_F2], h->geom[HARM_LOC_FACE2]);
   if (d.n3 > 1) LAUNCH_FLUX(3, h->arr[HARM_ARR_F3], h->geom[HARM_LOC_FACE3]);
 #undef LAUNCH_FLUX
+  if (fork) {
+    if (d.n2 > 1) HH_HIP(hipStreamWaitEvent(s, h->evjoin[0], 0));
+    if (d.n3 > 1) HH_HIP(hipStreamWaitEvent(s, h->evjoin[1], 0));
+  }
   HH_TRY(hh::check_launch("k_flux"));
   if (do_ct) {
     prof_begin(h);
@@ -1824,7 +1851,7 @@ harm_status enq_batch_substep(harm_batch* b, int to, int nto, const double CUf[4
     if (weno)                                                                                               \
       for (harm_handle* h : b->grids)                                                                       \
         if (h->ft[DIR].nblk > 0 && (DIR == 1 ? h->d.n1 : DIR == 2 ? h->d.n2 : h->d.n3) > 1)                 \
-          launch_weno_recon(DIR, h->d, h->ft[DIR].b, s, h->arr[pbb[to]], h->wdp, h->wl, h->wr);             \
+          launch_weno_recon(DIR, h->d, h->ft[DIR].b, s, h->arr[pbb[to]], h->wdp, wl_dir(h, DIR), wr_dir(h, DIR)); \
     launch_flux_batch(DIR, b->recon, (unsigned)b->ntot[BS_FLUX1 + DIR - 1], s, G, P(BS_FLUX1 + DIR - 1), ng, \
                       pbb[to], 1);                                                                          \
   }
@@ -1963,12 +1990,17 @@ extern "C" harm_status harm_create(const harm_config* cfg, harm_handle** out)
   h->emf_nk = d.n3 + (d.n3 > 1);
   TRYC(alloc((void**)&h->emf, sizeof(double) * 3 * (size_t)h->emf_ni * h->emf_nj * h->emf_nk));
   if (cfg->recon == HARM_RECON_WENO5) {
-    TRYC(alloc((void**)&h->wl, sizeof(double) * HARM_NPR * d.ncell));
-    TRYC(alloc((void**)&h->wr, sizeof(double) * HARM_NPR * d.ncell));
+    TRYC(alloc((void**)&h->wl, sizeof(double) * 3 * HARM_NPR * d.ncell));
+    TRYC(alloc((void**)&h->wr, sizeof(double) * 3 * HARM_NPR * d.ncell));
+    for (int q = 0; q < 2; q++) {
+      TRYC(hh::check_hip(hipStreamCreateWithFlags(&h->side[q], hipStreamNonBlocking), "hipStreamCreate"));
+      TRYC(hh::check_hip(hipEventCreateWithFlags(&h->evjoin[q], hipEventDisableTiming), "hipEventCreate"));
+    }
+    TRYC(hh::check_hip(hipEventCreateWithFlags(&h->evfork, hipEventDisableTiming), "hipEventCreate"));
     TRYC(alloc((void**)&h->wdp, sizeof(double) * 3 * d.ncell));
     TRYC(alloc((void**)&h->wtm, sizeof(double) * 10 * d.ncell));
-    TRYC(hh::check_hip(hipMemset(h->wl, 0, sizeof(double) * HARM_NPR * d.ncell), "hipMemset"));
-    TRYC(hh::check_hip(hipMemset(h->wr, 0, sizeof(double) * HARM_NPR * d.ncell), "hipMemset"));
+    TRYC(hh::check_hip(hipMemset(h->wl, 0, sizeof(double) * 3 * HARM_NPR * d.ncell), "hipMemset"));
+    TRYC(hh::check_hip(hipMemset(h->wr, 0, sizeof(double) * 3 * HARM_NPR * d.ncell), "hipMemset"));
     TRYC(hh::check_hip(hipMemset(h->wdp, 0, sizeof(double) * 3 * d.ncell), "hipMemset"));
   }
   TRYC(alloc((void**)&h->ds, sizeof(double) * DS_N));
@@ -2006,6 +2038,11 @@ extern "C" harm_status harm_destroy(harm_handle* h)
   void* bufs[] = {h->pflag, h->pfix, h->gcorn3, h->conn, h->floors, h->rcent, h->emf, h->wl, h->wr, h->wdp, h->wtm, h->facc,
                   h->en, h->enpart, h->entot, h->fx_p0, h->fx_pfl, h->fx_flag, h->ds, h->di};
   for (void* b : bufs) (void)hipFree(b);
+  for (int q = 0; q < 2; q++) {
+    if (h->side[q]) (void)hipStreamDestroy(h->side[q]);
+    if (h->evjoin[q]) (void)hipEventDestroy(h->evjoin[q]);
+  }
+  if (h->evfork) (void)hipEventDestroy(h->evfork);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return HARM_OK;
@@ -2912,9 +2949,12 @@ extern "C" harm_status harm_tune(harm_handle* h, int32_t reps, int32_t td_out[4]
       const double* G = h->geom[dir];
       if (recon == HARM_RECON_WENO5) {  // the face kernel reads this direction's line reconstruction
         launch_weno_dp(d, s, pb, h->geom[HARM_LOC_CENT], h->P, h->wtm, h->wdp);
-        launch_weno_recon(dir, d, T.b, s, pb, h->wdp, h->wl, h->wr);
+        launch_weno_recon(dir, d, T.b, s, pb, h->wdp, wl_dir(h, dir), wr_dir(h, dir));
       }
-      auto launch = [&]() { launch_flux(dir, recon, T, s, d, pb, F, G, h->P, ft, 1, h->ds, h->wl, h->wr); };
+      auto launch = [&]() {
+        launch_flux(dir, recon, T, s, d, pb, F, G, h->P, ft, 1, h->ds, h->wl ? wl_dir(h, dir) : nullptr,
+                    h->wr ? wr_dir(h, dir) : nullptr);
+      };
       launch();  // warm
       (void)hipEventRecord(e0, s);
       for (int r = 0; r < reps; r++) launch();

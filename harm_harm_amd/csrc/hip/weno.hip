@@ -659,7 +659,7 @@ void launch_weno_recon(int dir, const Dims& d, const Box& b, hipStream_t s, cons
   }
   // directions 2 and 3: lines cut into segments of at least kMinSeg faces,
   // about kWaveTarget waves in all (a few per SIMD)
-  constexpr long kWaveTarget = 10240;
+  constexpr long kWaveTarget = 16384;  // measured: 4 segments along x3, 8 along x2 at 256x128x64
   constexpr int kMinSeg = 16;
   const int N = dir == 2 ? d.n2 : d.n3;
   const int nf = N + 2;  // zones -1 .. N
