@@ -120,6 +120,7 @@ _OPTIONAL_PROTOS = [
     ("harm_reset_ener", _I, [_P]),
     ("harm_get_ener", _I, [_P, _I, _P]),
     ("harm_dump_columns", _I, [_P, _I, _P]),
+    ("harm_currents", _I, [_P, _I]),
     ("harm_gdump_columns", _I, [_P, _P]),
     ("harm_write_dump", _I, [_P, _I, ctypes.c_char_p]),
     ("harm_write_gdump", _I, [_P, ctypes.c_char_p]),

@@ -13,6 +13,7 @@
 //   k_ener_reduce    fixed-order sums of per-block partials, one block per
 //                    channel
 //   k_dump_cols      the physics columns of HARM's main dump (dump.c:439)
+//   k_current_*      the dump's four-current (phys.c:552-1000)
 //
 //   k_ener_fix       the final substep's floor and gamma-limit changes from
 //                    the states the inversion kept (diag_fixup, fixup.c:220)
@@ -326,15 +327,14 @@ __global__ void k_ener_divb_final(const double* __restrict__ part, long n, doubl
 
 // The physics columns of one zone of HARM's main dump (dump_content,
 // dump.c:439-600, CALCFARADAYANDCURRENTS): column c (DC_* order) of zone t =
-// i + n1 (j + n2 k), the dump's row order, at out[c' * cs + t * rs], where
-// c' = c below DC_FCON and c + jgap from there on; with jgap > 0 the jgap
-// columns between gdet and F (the four-current, not kept) are written as 0.
-// U is the conserved state the last inversion consumed (udump =
-// unewglobal, initbase.c:885).
+// i + n1 (j + n2 k), the dump's row order, at out[c * cs + t * rs].  U is the
+// conserved state the last inversion consumed (udump = unewglobal,
+// initbase.c:885); J^mu from jcon (0 without it) and J_mu = lower_vec.
 __global__ __launch_bounds__(kBlock) void k_dump_cols(Dims d, Params P, const double* __restrict__ p,
                                                       const double* __restrict__ ucum,
-                                                      const double* __restrict__ gcent, double* __restrict__ out,
-                                                      long cs, long rs, int jgap)
+                                                      const double* __restrict__ gcent,
+                                                      const double* __restrict__ jcon, double* __restrict__ out,
+                                                      long cs, long rs)
 {
   const long nz = (long)d.n1 * d.n2 * d.n3;
   const long t = (long)blockIdx.x * kBlock + threadIdx.x;
@@ -346,8 +346,7 @@ __global__ __launch_bounds__(kBlock) void k_dump_cols(Dims d, Params P, const do
   double pz[HARM_NPR];
 #pragma unroll
   for (int pl = 0; pl < HARM_NPR; pl++) pz[pl] = p[pl * nc + o];
-  auto put = [&](int c, double v) { out[(long)(c < DC_FCON ? c : c + jgap) * cs + t * rs] = v; };
-  for (int c = 0; c < jgap; c++) out[(long)(DC_FCON + c) * cs + t * rs] = 0.0;
+  auto put = [&](int c, double v) { out[(long)c * cs + t * rs] = v; };
 #pragma unroll
   for (int pl = 0; pl < HARM_NPR; pl++) put(DC_P + pl, pz[pl]);
   // ideal-gas EOS columns (idealgaseos.c:19, 48, 71)
@@ -387,6 +386,19 @@ __global__ __launch_bounds__(kBlock) void k_dump_cols(Dims d, Params P, const do
     put(DC_VCHAR + 2 * (dir - 1) + 1, vmax);
   }
   put(DC_GDET, G.gdet);
+  {
+    double jc[4] = {0.0, 0.0, 0.0, 0.0}, jl[4];
+    if (jcon) {
+#pragma unroll
+      for (int m = 0; m < 4; m++) jc[m] = jcon[(long)m * nc + o];
+    }
+    hd::lower(jc, G, jl);
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+      put(DC_JCON + m, jc[m]);
+      put(DC_JCOV + m, jl[m]);
+    }
+  }
   // Faraday F^{mu nu} at the centre (current_precalc CURTYPEFARADAY,
   // phys.c:701-708) and F_{mu nu} (lowerf, phys.tools.c:1449)
   const double ig = -1.0 / G.gdet;
@@ -418,18 +430,126 @@ __global__ __launch_bounds__(kBlock) void k_dump_cols(Dims d, Params P, const do
 }  // namespace
 
 void launch_dump_cols(const hd::Dims& d, const Params& P, const double* p, const double* ucum, const double* gcent,
-                      double* out, hipStream_t s)
+                      const double* jcon, double* out, hipStream_t s)
 {
   const long nz = (long)d.n1 * d.n2 * d.n3;
-  hipLaunchKernelGGL(k_dump_cols, dim3(nblk(nz)), dim3(kBlock), 0, s, d, P, p, ucum, gcent, out, nz, 1L, 0);
+  hipLaunchKernelGGL(k_dump_cols, dim3(nblk(nz)), dim3(kBlock), 0, s, d, P, p, ucum, gcent, jcon, out, nz, 1L);
 }
 
 void launch_dump_rows(const hd::Dims& d, const Params& P, const double* p, const double* ucum, const double* gcent,
-                      double* rows, hipStream_t s)
+                      const double* jcon, double* rows, hipStream_t s)
 {
   const long nz = (long)d.n1 * d.n2 * d.n3;
-  hipLaunchKernelGGL(k_dump_cols, dim3(nblk(nz)), dim3(kBlock), 0, s, d, P, p, ucum, gcent, rows + kDumpCoordCols,
-                     1L, (long)HARM_DUMP_NCOL, kDumpCurCols);
+  hipLaunchKernelGGL(k_dump_cols, dim3(nblk(nz)), dim3(kBlock), 0, s, d, P, p, ucum, gcent, jcon,
+                     rows + kDumpCoordCols, 1L, (long)HARM_DUMP_NCOL);
+}
+
+namespace {
+// current_precalc (phys.c:622-686) for CURTYPET, X, Y, Z at the centre of
+// every storage zone: the two previous F^{it} samples move to slots 4 and 5,
+// slot 0 takes F^{it} now and slots 1-3 the components the x1, x2, x3
+// derivatives of current_calc_1 read (slot 3's first sample is the
+// reference's F^{tr}, phys.c:679).
+__global__ __launch_bounds__(kBlock) void k_current_precalc(Dims d, Params P, const double* __restrict__ p,
+                                                            const double* __restrict__ gcent,
+                                                            double* __restrict__ cf)
+{
+  const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+  const long nz = (long)d.ns1 * d.ns2 * d.ns3;
+  if (t >= nz) return;
+  const int i = (int)(t % d.ns1) - d.ng1, j = (int)((t / d.ns1) % d.ns2) - d.ng2,
+            k = (int)(t / ((long)d.ns1 * d.ns2)) - d.ng3;
+  const long o = hd::cidx(d, i, j, k), nc = d.ncell;
+  Geom G;
+  hd::load_geom(gcent, hd::pidx(d, i, j), d.nplane, G);
+  double pz[HARM_NPR];
+#pragma unroll
+  for (int pl = 0; pl < HARM_NPR; pl++) pz[pl] = p[pl * nc + o];
+  State q;
+  hd::get_state(pz, G, P.gam, q);
+  double bcon[4], b[4];
+  bcon[0] = pz[HARM_B1] * q.ucov[1] + pz[HARM_B2] * q.ucov[2] + pz[HARM_B3] * q.ucov[3];
+#pragma unroll
+  for (int m = 1; m < 4; m++) bcon[m] = (pz[HARM_B1 - 1 + m] + bcon[0] * q.ucon[m]) / q.ucon[0];
+  hd::lower(bcon, G, b);
+  const double* u = q.ucov;
+  const double gd = G.gdet;
+  auto F = [&](int slot, int c) -> double& { return cf[(long)(slot * 3 + c) * nc + o]; };
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    F(4, c) = F(5, c);
+    F(5, c) = F(0, c);
+  }
+  F(0, 0) = -1.0 / gd * (-b[3] * u[2] + b[2] * u[3]);
+  F(0, 1) = -1.0 / gd * (b[3] * u[1] - b[1] * u[3]);
+  F(0, 2) = -1.0 / gd * (-b[2] * u[1] + b[1] * u[2]);
+  F(1, 0) = -1.0 / gd * (b[3] * u[2] - b[2] * u[3]);
+  F(1, 1) = -1.0 / gd * (-b[3] * u[0] + b[0] * u[3]);
+  F(1, 2) = -1.0 / gd * (b[2] * u[0] - b[0] * u[2]);
+  F(2, 0) = -1.0 / gd * (-b[3] * u[1] + b[1] * u[3]);
+  F(2, 1) = -1.0 / gd * (b[3] * u[0] - b[0] * u[3]);
+  F(2, 2) = -1.0 / gd * (-b[1] * u[0] + b[0] * u[1]);
+  F(3, 0) = 1.0 / gd * (-b[3] * u[2] + b[2] * u[3]);
+  F(3, 1) = 1.0 / gd * (b[2] * u[0] - b[0] * u[2]);
+  F(3, 2) = 1.0 / gd * (-b[1] * u[0] + b[0] * u[1]);
+}
+
+// current_calc_1 (phys.c:846-1000) at the active zones: centred differences
+// of gdet F over two zones (none along a dimension of one zone), and from
+// the third call on the time derivative at the newest of three F^{it}
+// samples dtl and dtr apart.
+__global__ __launch_bounds__(kBlock) void k_current_calc(Dims d, Params P, const double* __restrict__ cf,
+                                                         const double* __restrict__ gcent, int timeterms, double dtl,
+                                                         double dtr, double* __restrict__ jcon)
+{
+  const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= (long)d.n1 * d.n2 * d.n3) return;
+  const int i = (int)(t % d.n1), j = (int)((t / d.n1) % d.n2), k = (int)(t / ((long)d.n1 * d.n2));
+  const int s1 = d.n1 > 1, s2 = d.n2 > 1, s3 = d.n3 > 1;
+  const long nc = d.ncell, gd = (long)HARM_G_GDET * d.nplane;
+  const long o = hd::cidx(d, i, j, k);
+  const long orp = hd::cidx(d, i + s1, j, k), orm = hd::cidx(d, i - s1, j, k);
+  const long ohp = hd::cidx(d, i, j + s2, k), ohm = hd::cidx(d, i, j - s2, k);
+  const long opp = hd::cidx(d, i, j, k + s3), opm = hd::cidx(d, i, j, k - s3);
+  const double gt = gcent[gd + hd::pidx(d, i, j)];
+  const double grp = gcent[gd + hd::pidx(d, i + s1, j)], grm = gcent[gd + hd::pidx(d, i - s1, j)];
+  const double ghp = gcent[gd + hd::pidx(d, i, j + s2)], ghm = gcent[gd + hd::pidx(d, i, j - s2)];
+  auto F = [&](int slot, int c, long at) { return cf[(long)(slot * 3 + c) * nc + at]; };
+  const double idx1 = 1.0 / (2.0 * P.dx[1]), idx2 = 1.0 / (2.0 * P.dx[2]), idx3 = 1.0 / (2.0 * P.dx[3]);
+  double tt[4] = {0.0, 0.0, 0.0, 0.0};
+  if (timeterms) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const double fl = F(4, c, o), f0 = F(5, c, o), fr = F(0, c, o);
+      const double AA = (dtr * (fl - f0) + dtl * (fr - f0)) / (dtl * dtr * (dtl + dtr));
+      const double BB = (-dtr * dtr * (fl - f0) + dtl * dtl * (fr - f0)) / (dtl * dtr * (dtl + dtr));
+      tt[1 + c] = 2.0 * AA * dtr + BB;
+    }
+  }
+  jcon[0 * nc + o] = +1. / gt * (grp * F(1, 0, orp) - grm * F(1, 0, orm)) * idx1
+                     + 1. / gt * (ghp * F(2, 0, ohp) - ghm * F(2, 0, ohm)) * idx2
+                     + 1. / gt * (gt * F(3, 0, opp) - gt * F(3, 0, opm)) * idx3;
+  jcon[1 * nc + o] = +tt[1] + 1. / gt * (ghp * F(2, 1, ohp) - ghm * F(2, 1, ohm)) * idx2
+                     + 1. / gt * (gt * F(3, 1, opp) - gt * F(3, 1, opm)) * idx3;
+  jcon[2 * nc + o] = +tt[2] + 1. / gt * (grp * F(1, 1, orp) - grm * F(1, 1, orm)) * idx1
+                     + 1. / gt * (gt * F(3, 2, opp) - gt * F(3, 2, opm)) * idx3;
+  jcon[3 * nc + o] = +tt[3] + 1. / gt * (grp * F(1, 2, orp) - grm * F(1, 2, orm)) * idx1
+                     + 1. / gt * (ghp * F(2, 2, ohp) - ghm * F(2, 2, ohm)) * idx2;
+}
+}  // namespace
+
+void launch_current_precalc(const hd::Dims& d, const Params& P, const double* p, const double* gcent, double* cf,
+                            hipStream_t s)
+{
+  hipLaunchKernelGGL(k_current_precalc, dim3(nblk((long)d.ns1 * d.ns2 * d.ns3)), dim3(kBlock), 0, s, d, P, p, gcent,
+                     cf);
+}
+
+void launch_current_calc(const hd::Dims& d, const Params& P, const double* cf, const double* gcent, int timeterms,
+                         double dtl, double dtr, double* jcon, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_current_calc, dim3(nblk((long)d.n1 * d.n2 * d.n3)), dim3(kBlock), 0, s, d, P, cf, gcent,
+                     timeterms, dtl, dtr, jcon);
 }
 
 long ener_part_blocks(const hd::Dims& d)

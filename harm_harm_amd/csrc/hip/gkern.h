@@ -17,7 +17,7 @@ constexpr double kBIG = 1.e+100;
 
 // device scalar slots
 enum { DS_DT = 0, DS_T = 1, DS_NDT1 = 2, DS_NDT2 = 3, DS_NDT3 = 4, DS_ACCDT = 5, DS_LASTNDT = 6, DS_NSTEP = 7,
-       DS_FLUXACC = 8, DS_N = 16 };
+       DS_FLUXACC = 8, DS_DTSTEP = 12, DS_N = 16 };  // DS_DTSTEP: the dt of the last whole step
 // Newton counters: the current step's, and their sums over the steps since
 // the time was last set
 // and the zones whose learned-guess solve fell back to the plain guess
@@ -242,16 +242,25 @@ void launch_ener_fix(const hd::Dims& d, const Params& P, const double* pf, const
 // u_mu, b^mu, b_mu, (vmin, vmax) per direction, gdet, F^{mu nu}, F_{mu nu}.
 // The coordinates and the current columns are the host's.
 enum { DC_P = 0, DC_PRESS = 8, DC_CS2 = 9, DC_SDEN = 10, DC_U = 11, DC_DIVB = 19, DC_UCON = 20, DC_UCOV = 24,
-       DC_BCON = 28, DC_BCOV = 32, DC_VCHAR = 36, DC_GDET = 42, DC_FCON = 43, DC_FCOV = 49, DC_N = 55 };
+       DC_BCON = 28, DC_BCOV = 32, DC_VCHAR = 36, DC_GDET = 42, DC_JCON = 43, DC_JCOV = 47, DC_FCON = 51,
+       DC_FCOV = 57, DC_N = 63 };
 void launch_dump_cols(const hd::Dims& d, const Params& P, const double* p, const double* ucum, const double* gcent,
-                      double* out, hipStream_t s);
+                      const double* jcon, double* out, hipStream_t s);
 // The same columns straight into the dump's rows: rows[z * HARM_DUMP_NCOL +
 // c], columns kDumpCoordCols.. (the coordinate columns before them are left
-// as they are), with the kDumpCurCols current columns after gdet written as 0.
-enum { kDumpCoordCols = 9, kDumpCurCols = 8 };
-static_assert(kDumpCoordCols + DC_N + kDumpCurCols == HARM_DUMP_NCOL, "dump row layout");
+// as they are).  jcon: J^mu (4 x ncell, harm_currents), or null for zeros.
+enum { kDumpCoordCols = 9 };
+static_assert(kDumpCoordCols + DC_N == HARM_DUMP_NCOL, "dump row layout");
 void launch_dump_rows(const hd::Dims& d, const Params& P, const double* p, const double* ucum, const double* gcent,
-                      double* rows, hipStream_t s);
+                      const double* jcon, double* rows, hipStream_t s);
+// The main dump's four-current (CURRENTCALC1): the Faraday samples of every
+// storage zone (current_precalc phys.c:622; cf: 18 x ncell, slot * 3 + c) and
+// J^mu of the active zones from them (current_calc_1 phys.c:846; jcon: 4 x
+// ncell), the time term from the third call on.
+void launch_current_precalc(const hd::Dims& d, const Params& P, const double* p, const double* gcent, double* cf,
+                            hipStream_t s);
+void launch_current_calc(const hd::Dims& d, const Params& P, const double* cf, const double* gcent, int timeterms,
+                         double dtl, double dtr, double* jcon, hipStream_t s);
 // source sample weight at the end of substep `to` of nto (advance.c:251-262)
 void launch_ener_dt4(double* en, const double* ds, int last, hipStream_t s);
 // sourceadd += dt4diag * dV * dUgeom(pb) over active zones when dt4diag > 0

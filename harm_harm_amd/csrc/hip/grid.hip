@@ -970,6 +970,7 @@ __global__ void k_scal_end_step(double* ds, unsigned long long* di, double safe)
   double fullndt = ds[DS_LASTNDT];
   const double dt = ds[DS_DT];
   if (fullndt > safe * dt) fullndt = safe * dt;
+  ds[DS_DTSTEP] = dt;
   ds[DS_T] = ds[DS_T] + dt;
   ds[DS_DT] = fullndt;
   ds[DS_NSTEP] = ds[DS_NSTEP] + 1.0;
@@ -1051,6 +1052,10 @@ struct harm_handle {
   double* wr = nullptr;
   hipStream_t side[2] = {nullptr, nullptr};  // WENO5BND: directions 2 and 3 of fluxcalc, forked from stream
   hipEvent_t evfork = nullptr, evjoin[2] = {nullptr, nullptr};
+  double* cf = nullptr;    // the dump's four-current: Faraday samples (18 x ncell) and J^mu (4 x ncell),
+  double* jcon = nullptr;  // allocated by the first harm_currents
+  long cur_calls = 0;      // current_calc_1's call count and the dt of its last call
+  double cur_lastdt = 0.0;
   double* wdp = nullptr;   // WENO5BND: dP measure per direction (3 x ncell)
   double* wtm = nullptr;   // WENO5BND: per-zone terms of dP (10 x ncell)
   double* facc = nullptr;  // boundary-flux partial sums
@@ -1732,6 +1737,7 @@ __global__ void kb_scal(const GridRef* __restrict__ G, int ng, int what)
     double fullndt = ds[DS_LASTNDT];
     const double dt = ds[DS_DT];
     if (fullndt > safe * dt) fullndt = safe * dt;
+    ds[DS_DTSTEP] = dt;
     ds[DS_T] = ds[DS_T] + dt;
     ds[DS_DT] = fullndt;
     ds[DS_NSTEP] = ds[DS_NSTEP] + 1.0;
@@ -2036,6 +2042,7 @@ extern "C" harm_status harm_destroy(harm_handle* h)
   }
   for (hipEvent_t e : h->evpool) (void)hipEventDestroy(e);
   void* bufs[] = {h->pflag, h->pfix, h->gcorn3, h->conn, h->floors, h->rcent, h->emf, h->wl, h->wr, h->wdp, h->wtm, h->facc,
+                  h->cf, h->jcon,
                   h->en, h->enpart, h->entot, h->fx_p0, h->fx_pfl, h->fx_flag, h->ds, h->di};
   for (void* b : bufs) (void)hipFree(b);
   for (int q = 0; q < 2; q++) {
@@ -2533,7 +2540,7 @@ extern "C" harm_status harm_dump_columns(harm_handle* h, int32_t which, double* 
   const long nz = (long)d.n1 * d.n2 * d.n3;
   double* dcol = nullptr;
   HH_TRY(alloc((void**)&dcol, sizeof(double) * DC_N * nz));
-  launch_dump_cols(d, h->P, h->arr[which], h->arr[HARM_ARR_UCUM], h->geom[HARM_LOC_CENT], dcol, h->stream);
+  launch_dump_cols(d, h->P, h->arr[which], h->arr[HARM_ARR_UCUM], h->geom[HARM_LOC_CENT], h->jcon, dcol, h->stream);
   std::vector<double> col((size_t)DC_N * nz);
   harm_status st = hh::check_launch("k_dump_cols");
   if (st == HARM_OK) st = hh::check_hip(hipStreamSynchronize(h->stream), "harm_dump_columns");
@@ -2541,16 +2548,12 @@ extern "C" harm_status harm_dump_columns(harm_handle* h, int32_t which, double* 
     st = hh::check_hip(hipMemcpy(col.data(), dcol, sizeof(double) * col.size(), hipMemcpyDeviceToHost), "copy");
   (void)hipFree(dcol);
   if (st != HARM_OK) return st;
-  // dump_content's column order: coordinates, then the device columns with
-  // the four-current j^mu, j_mu (8 columns, 0 here) between gdet and F
+  // dump_content's column order: coordinates, then the device columns
   for (long t = 0; t < nz; t++) {
     const int i = (int)(t % d.n1), j = (int)((t / d.n1) % d.n2), k = (int)(t / ((long)d.n1 * d.n2));
     double* r = rows + t * HARM_DUMP_NCOL;
     dump_coords(h, i, j, k, r);
-    int c = 9;
-    for (int q = 0; q <= DC_GDET; q++) r[c++] = col[(size_t)q * nz + t];
-    for (int q = 0; q < 8; q++) r[c++] = 0.0;
-    for (int q = DC_FCON; q < DC_N; q++) r[c++] = col[(size_t)q * nz + t];
+    for (int q = 0; q < DC_N; q++) r[kDumpCoordCols + q] = col[(size_t)q * nz + t];
   }
   return HARM_OK;
 }
@@ -2604,8 +2607,9 @@ extern "C" harm_status harm_dump_columns_noalloc(harm_handle* h, int32_t which, 
   double* dev = nullptr;
   HH_TRY(scratch_on(h->dev, &dev));
   // every column of every row the copy takes is written: the kernel fills
-  // the physics and current columns, the host the coordinates after the copy
-  launch_dump_rows(d, h->P, h->arr[which], h->arr[HARM_ARR_UCUM], h->geom[HARM_LOC_CENT], dev, h->stream);
+  // the physics columns (the four-current included), the host the
+  // coordinates after the copy
+  launch_dump_rows(d, h->P, h->arr[which], h->arr[HARM_ARR_UCUM], h->geom[HARM_LOC_CENT], h->jcon, dev, h->stream);
   HH_TRY(hh::check_launch("k_dump_cols"));
   HH_HIP(hipMemcpyAsync(rows, dev, sizeof(double) * HARM_DUMP_NCOL * nz, hipMemcpyDeviceToHost, h->stream));
   HH_HIP(hipStreamSynchronize(h->stream));
@@ -2613,6 +2617,36 @@ extern "C" harm_status harm_dump_columns_noalloc(harm_handle* h, int32_t which, 
     const int i = (int)(t % d.n1), j = (int)((t / d.n1) % d.n2), k = (int)(t / ((long)d.n1 * d.n2));
     dump_coords(h, i, j, k, rows + t * HARM_DUMP_NCOL);
   }
+  return HARM_OK;
+}
+
+// The four-current the main dump carries (CALCFARADAYANDCURRENTS 1,
+// CURRENTCALC1): at the start of a run (compute_currents_t0,
+// initbase.c:1414, CURRENTST0) and after each step that ends at a main dump
+// (post_stepch's block, step_ch.c:156-170): the Faraday samples of every
+// zone, then J^mu from centred differences and, from the third call on, the
+// time derivative over the last three samples (dtl: the step dt at the
+// previous call, dtr: now).
+extern "C" harm_status harm_currents(harm_handle* h, int32_t at_start)
+{
+  HH_TRY(check_handle(h));
+  const Dims& d = h->d;
+  if (!h->cf) {
+    HH_TRY(alloc((void**)&h->cf, sizeof(double) * 18 * d.ncell));
+    HH_TRY(alloc((void**)&h->jcon, sizeof(double) * 4 * d.ncell));
+    HH_HIP(hipMemsetAsync(h->cf, 0, sizeof(double) * 18 * d.ncell, h->stream));
+    HH_HIP(hipMemsetAsync(h->jcon, 0, sizeof(double) * 4 * d.ncell, h->stream));
+  }
+  double v[DS_N];
+  HH_HIP(hipStreamSynchronize(h->stream));
+  HH_HIP(hipMemcpy(v, h->ds, sizeof(v), hipMemcpyDeviceToHost));
+  const double dt = at_start ? v[DS_DT] : v[DS_DTSTEP];
+  launch_current_precalc(d, h->P, h->arr[HARM_ARR_P], h->geom[HARM_LOC_CENT], h->cf, h->stream);
+  launch_current_calc(d, h->P, h->cf, h->geom[HARM_LOC_CENT], h->cur_calls >= 2, h->cur_lastdt, dt, h->jcon,
+                      h->stream);
+  HH_TRY(hh::check_launch("k_current"));
+  h->cur_calls++;
+  h->cur_lastdt = dt;
   return HARM_OK;
 }
 

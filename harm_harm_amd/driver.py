@@ -142,6 +142,10 @@ def run(grid, tf, DTdump=None, DTener=None, outdir=".", gdump=True, max_steps=No
     def diag(call):
         t, _ = grid.time()
         if dump_c.due(t, call):
+            if call == "step":
+                # post_stepch computes the dump's four-current after a step
+                # that ends at a main dump (step_ch.c:156-170)
+                grid.currents()
             grid.write_dump(os.path.join(outdir, "dumps", "dump%04d" % dump_c.count), which)
             if DTr is not None:
                 grid.write_restart(os.path.join(outdir, "dumps", "rdump--%04d" % dump_c.count), restart_binary, tf)
@@ -155,6 +159,7 @@ def run(grid, tf, DTdump=None, DTener=None, outdir=".", gdump=True, max_steps=No
                 f.write(ener_line(grid.ener(which)))
             ener_c.done(t)
 
+    grid.currents(at_start=True)  # post_init's compute_currents_t0 (initbase.c:1373)
     diag("init")
     steps, onemorestep = 0, False
     while True:

@@ -218,6 +218,11 @@ class Grid:
     # ---- HARM-format dumps (dump.c) ----
     DUMP_NCOL, GDUMP_NCOL = 72, 126
 
+    def currents(self, at_start=False):
+        """The four-current the main dump carries (harm_currents): at the start
+        of a run, or after a step that ends at a main dump."""
+        check(lib().harm_currents(self.h, int(bool(at_start))), "harm_currents")
+
     def dump_columns(self, which="p"):
         """the main dump's rows (dump_content): (n1*n2*n3, 72), i fastest"""
         c = self.cfg

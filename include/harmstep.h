@@ -302,12 +302,20 @@ harm_status harm_set_ener_tracking(harm_handle* h, int32_t on);
  * then j, then k: ti tj tk, X1..X3 (MKS), r th ph, the 8 primitives, p,
  * cs2, Sden, U[8] (the conserved state of the last inversion), |div B|,
  * u^mu, u_mu, b^mu, b_mu, v1m v1p v2m v2p v3m v3p, gdet, j^mu, j_mu,
- * F^{mu nu} (01,02,03,12,13,23), F_{mu nu}.  The current columns j are
- * written as 0: the time history they need is not kept.
+ * F^{mu nu} (01,02,03,12,13,23), F_{mu nu}.  The current columns j hold
+ * the four-current of the last harm_currents call (0 before the first).
  * Grid dump: HARM_GDUMP_NCOL columns: ti tj tk, X, V, Gamma^a_{bc} (a
  * slowest), g^{ab}, g_{ab}, gdet, conn2 (0 with gdet in the equations),
  * dV^a/dX^b. */
 enum { HARM_DUMP_NCOL = 72, HARM_GDUMP_NCOL = 126 };
+/* Replaces current_doprecalc()/current_calc() (phys.c:552-1000,
+ * CALCFARADAYANDCURRENTS with CURRENTCALC1): with at_start != 0 as
+ * compute_currents_t0 does at the start of a run (initbase.c:1414), else as
+ * post_stepch does after a step that ends at a main dump (step_ch.c:156-170):
+ * the Faraday samples of every zone from HARM_ARR_P, then J^mu (the time
+ * derivative over the last three samples from the third call on).  Allocates
+ * its arrays on the first call. */
+harm_status harm_currents(harm_handle* h, int32_t at_start);
 /* Replaces dump_content() over the grid: rows[z * HARM_DUMP_NCOL + c] from
  * primitives `which` (HARM_ARR_*) on the host. */
 harm_status harm_dump_columns(harm_handle* h, int32_t which, double* rows);

@@ -1337,6 +1337,107 @@ void o_dump_row(const o_grid* g, const double* p, const double* ucum, int i, int
   }
 }
 
+/* The four-current of the main dump (CALCFARADAYANDCURRENTS 1, WHICHCURRENTCALC
+ * CURRENTCALC1, definit.h:595-597).  cf holds 6 slots of 3 Faraday samples
+ * per storage zone, cf[zone * 18 + slot * 3 + c] (zone = i + ns1 (j + ns2 k)
+ * over the storage box).
+ *
+ * o_current_precalc: current_doprecalc (phys.c:552) for CURTYPET, X, Y, Z at
+ * zone centres over the whole storage box with current_precalc's formulas
+ * (phys.c:622-686): slot 4 <- slot 5, slot 5 <- slot 0 (the two previous
+ * samples of F^{it}), then slot 0 = (F^{rt}, F^{ht}, F^{pt}) now, slot 1 =
+ * (F^{tr}, F^{hr}, F^{pr}), slot 2 = (F^{th}, F^{rh}, F^{ph}), slot 3 =
+ * (F^{rt}, F^{rp}, F^{hp}) as the x1, x2, x3 derivatives take them. */
+void o_current_precalc(const o_grid* g, const double* p, double* cf)
+{
+  dims_t d = D(g);
+  for (int k = -d.ng3; k < d.n3 + d.ng3; k++)
+    for (int j = -d.ng2; j < d.n2 + d.ng2; j++)
+      for (int i = -d.ng1; i < d.n1 + d.ng1; i++) {
+        const double* pr = p + ix(&d, i, j, k);
+        double* f = cf + ix(&d, i, j, k) / NPR * 18;
+        o_geom G;
+        geom_at(g, &d, 0, i, j, &G);
+        o_state q;
+        o_get_state(pr, &G, g->gam, &q);
+        double bcon[4], bcov[4];
+        bcon[0] = pr[O_B1] * q.ucov[1] + pr[O_B2] * q.ucov[2] + pr[O_B3] * q.ucov[3];
+        for (int m = 1; m < 4; m++) bcon[m] = (pr[O_B1 - 1 + m] + bcon[0] * q.ucon[m]) / q.ucon[0];
+        for (int a = 0; a < 4; a++) {
+          bcov[a] = 0.0;
+          for (int b = 0; b < 4; b++) bcov[a] += G.gcov[a][b] * bcon[b];
+        }
+        const double* u = q.ucov;
+        const double* b = bcov;
+        const double gd = G.gdet;
+        for (int c = 0; c < 3; c++) {
+          f[12 + c] = f[15 + c];
+          f[15 + c] = f[c];
+        }
+        f[0] = -1.0 / gd * (-b[3] * u[2] + b[2] * u[3]);
+        f[1] = -1.0 / gd * (b[3] * u[1] - b[1] * u[3]);
+        f[2] = -1.0 / gd * (-b[2] * u[1] + b[1] * u[2]);
+        f[3] = -1.0 / gd * (b[3] * u[2] - b[2] * u[3]);
+        f[4] = -1.0 / gd * (-b[3] * u[0] + b[0] * u[3]);
+        f[5] = -1.0 / gd * (b[2] * u[0] - b[0] * u[2]);
+        f[6] = -1.0 / gd * (-b[3] * u[1] + b[1] * u[3]);
+        f[7] = -1.0 / gd * (b[3] * u[0] - b[0] * u[3]);
+        f[8] = -1.0 / gd * (-b[1] * u[0] + b[0] * u[1]);
+        f[9] = 1.0 / gd * (-b[3] * u[2] + b[2] * u[3]);
+        f[10] = 1.0 / gd * (b[2] * u[0] - b[0] * u[2]);
+        f[11] = 1.0 / gd * (-b[1] * u[0] + b[0] * u[1]);
+      }
+}
+
+/* current_calc_1 (phys.c:846-1000) at the active zones: J^mu from centred
+ * differences of gdet F over two zones (x3 differences vanish when n3 = 1:
+ * kp1mac(k) = k) and, from the third call on (timeterms != 0), the time
+ * derivative of F^{it} at the newest of three samples dtl and dtr apart
+ * (the quadratic through them).  jcon[(i + n1 (j + n2 k)) * 4 + mu]. */
+void o_current_calc(const o_grid* g, const double* cf, int timeterms, double dtl, double dtr, double* jcon)
+{
+  dims_t d = D(g);
+  const double idx1 = 1.0 / (2.0 * g->dx[1]), idx2 = 1.0 / (2.0 * g->dx[2]), idx3 = 1.0 / (2.0 * g->dx[3]);
+  const int s1 = d.n1 > 1, s2 = d.n2 > 1, s3 = d.n3 > 1;
+  for (int k = 0; k < d.n3; k++)
+    for (int j = 0; j < d.n2; j++)
+      for (int i = 0; i < d.n1; i++) {
+        o_geom Gt, Grp, Grm, Ghp, Ghm;
+        geom_at(g, &d, 0, i, j, &Gt);
+        geom_at(g, &d, 0, i + s1, j, &Grp);
+        geom_at(g, &d, 0, i - s1, j, &Grm);
+        geom_at(g, &d, 0, i, j + s2, &Ghp);
+        geom_at(g, &d, 0, i, j - s2, &Ghm);
+        const double gt = Gt.gdet;
+        const double* f = cf + ix(&d, i, j, k) / NPR * 18;
+        const double* frp = cf + ix(&d, i + s1, j, k) / NPR * 18;
+        const double* frm = cf + ix(&d, i - s1, j, k) / NPR * 18;
+        const double* fhp = cf + ix(&d, i, j + s2, k) / NPR * 18;
+        const double* fhm = cf + ix(&d, i, j - s2, k) / NPR * 18;
+        const double* fpp = cf + ix(&d, i, j, k + s3) / NPR * 18;
+        const double* fpm = cf + ix(&d, i, j, k - s3) / NPR * 18;
+        double tt[4] = {0.0, 0.0, 0.0, 0.0};
+        if (timeterms) {
+          for (int c = 0; c < 3; c++) {
+            const double fl = f[12 + c], f0 = f[15 + c], fr = f[c];
+            const double AA = (dtr * (fl - f0) + dtl * (fr - f0)) / (dtl * dtr * (dtl + dtr));
+            const double BB = (-dtr * dtr * (fl - f0) + dtl * dtl * (fr - f0)) / (dtl * dtr * (dtl + dtr));
+            tt[1 + c] = 2.0 * AA * dtr + BB;
+          }
+        }
+        double* J = jcon + (((long)k * d.n2 + j) * d.n1 + i) * 4;
+        J[0] = +1. / gt * (Grp.gdet * frp[3] - Grm.gdet * frm[3]) * idx1
+               + 1. / gt * (Ghp.gdet * fhp[6] - Ghm.gdet * fhm[6]) * idx2
+               + 1. / gt * (gt * fpp[9] - gt * fpm[9]) * idx3;
+        J[1] = +tt[1] + 1. / gt * (Ghp.gdet * fhp[7] - Ghm.gdet * fhm[7]) * idx2
+               + 1. / gt * (gt * fpp[10] - gt * fpm[10]) * idx3;
+        J[2] = +tt[2] + 1. / gt * (Grp.gdet * frp[4] - Grm.gdet * frm[4]) * idx1
+               + 1. / gt * (gt * fpp[11] - gt * fpm[11]) * idx3;
+        J[3] = +tt[3] + 1. / gt * (Grp.gdet * frp[5] - Grm.gdet * frm[5]) * idx1
+               + 1. / gt * (Ghp.gdet * fhp[8] - Ghm.gdet * fhm[8]) * idx2;
+      }
+}
+
 void o_step(const o_grid* g, double* p, double* pk0, double* pk1, double* ui, double* uf, double* ucum, double* F1,
             double* F2, double* F3, int32_t* pflag, double* dt, double* fluxacc)
 {

@@ -75,6 +75,11 @@ void o_step(const o_grid* g, double* p, double* pk0, double* pk1, double* ui, do
 /* totals over active zones (constotal dump_ener.c:882, divbmaxavg :680):
  * out[0..7] = sum U(p) dV, out[8] = max |div B|, out[9] = sum |div B| */
 void o_ener_totals(const o_grid* g, const double* p, double* out);
+/* The main dump's four-current (current_precalc phys.c:622, current_calc_1
+ * :846; CURRENTCALC1): cf = 18 doubles per storage zone (6 slots x 3 Faraday
+ * samples); jcon = 4 per active zone. */
+void o_current_precalc(const o_grid* g, const double* p, double* cf);
+void o_current_calc(const o_grid* g, const double* cf, int timeterms, double dtl, double dtr, double* jcon);
 /* dump_content's physics columns for active zone (i,j,k): 55 values, the
  * dump's columns 10-52 and 61-72 (dump.c:439-600) */
 void o_dump_row(const o_grid* g, const double* p, const double* ucum, int i, int j, int k, double* out);

@@ -10,7 +10,8 @@
  *
  * Pinning (tests/test_oracle_phys.py, tests/test_oracle_grid.py,
  * tests/test_oracle_weno.py, tests/test_oracle_fixup.py,
- * tests/test_connection_closed_form.py): per-zone state, fluxes, wave
+ * tests/test_oracle_current.py, tests/test_connection_closed_form.py):
+ * per-zone state, fluxes, wave
  * speeds and the inversion against textbook GRMHD and round trips; the
  * WENO5BND line reconstruction against polynomial exactness, measured
  * fifth-order convergence, the SMONO derivative stencils against numpy's
@@ -21,7 +22,9 @@
  * closed-form MKS Kerr-Schild connection; the radial outflow extrapolation
  * against power laws (exact on the log-radius grid), constant gdet B^1 and
  * the minmod continuation of B_3; failure averaging against brute-force
- * means of good neighbours; the cold inversion against cold states.
+ * means of good neighbours; the cold inversion against cold states; the
+ * dump's four-current against the Levi-Civita form of F and exact
+ * differences of linear and quadratic samples.
  * Parity unpinned: the reference's uninitialised reads in the WENO dP line
  * (o_weno.c states the choice made) cannot be pinned against anything.
  */

@@ -214,6 +214,10 @@ def _grid_protos(L):
     L.o_ener_totals.restype = None
     L.o_dump_row.argtypes = [P, _P, _P, _I, _I, _I, _P]
     L.o_dump_row.restype = None
+    L.o_current_precalc.argtypes = [P, _P, _P]
+    L.o_current_precalc.restype = None
+    L.o_current_calc.argtypes = [P, _P, _I, _D, _D, _P]
+    L.o_current_calc.restype = None
     L.o_rk_tables.argtypes = [_I, _P, _P]
     L.o_rk_tables.restype = _I
     for n in ("o_bound", "o_fluxcalc", "o_update", "o_invert", "o_post_fixup", "o_step"):
@@ -359,6 +363,21 @@ class OGrid:
         return {"U": tot[:8].copy(), "pdot": en[EN_PDOT:EN_PDOT + 48].reshape(6, 8).copy(),
                 "pcum": en[EN_PCUM:EN_PCUM + 48].reshape(6, 8).copy(), "fladd": terms.sum(0), "fladdterms": terms,
                 "sourceadd": en[EN_SRCADD:EN_SRCADD + 8].copy(), "divbmax": tot[8], "divbavg": tot[9] / nz}
+
+    def current_slots(self):
+        """zeroed Faraday samples of the current calculation: (ns3, ns2, ns1, 6, 3)"""
+        return np.zeros((self.ns3, self.ns2, self.ns1, 6, 3))
+
+    def current_precalc(self, p, cf):
+        lib().o_current_precalc(ctypes.byref(self.g), p.ctypes.data, cf.ctypes.data)
+
+    def current_calc(self, cf, timeterms, dtl, dtr):
+        """J^mu of the active zones, (n3, n2, n1, 4)"""
+        c = self.cfg
+        j = np.zeros((c.n3, c.n2, c.n1, 4))
+        lib().o_current_calc(ctypes.byref(self.g), cf.ctypes.data, int(timeterms), float(dtl), float(dtr),
+                             j.ctypes.data)
+        return j
 
     def dump_rows(self, p, ucum):
         """dump_content's 55 physics columns (o_dump_row) of every active zone,

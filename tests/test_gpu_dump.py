@@ -5,7 +5,11 @@ harm_gdump_columns, harm_write_gdump) through the C ABI.
   test_oracle_dump.py) after three steps, per column to 1e-12 of the
   column's scale (the inversion, state and EOS columns are bit-exact; the
   fast speeds and |div B| of the device's own arithmetic), the coordinates
-  against harm_get_coords, the current columns 0;
+  against harm_get_coords, the current columns 0 before any harm_currents;
+* the four-current columns after harm_currents at the start and after two
+  steps (the third call takes the time derivative) against the oracle's
+  o_current_precalc / o_current_calc on the same states, and J_mu the
+  lowered J^mu;
 * the written file parses back to the reference's header (30 fields, with
   the column count last) and one row of 72 values per zone equal to the
   columns at the printed precision;
@@ -62,6 +66,40 @@ def test_dump_columns_match_oracle(case):
         Xc, Vc = (ctypes.c_double * 4)(), (ctypes.c_double * 4)()
         H.check(H.lib().harm_get_coords(g.h, i, j, k, Xc, Vc), "coords")
         assert list(rows[t, 3:6]) == list(Xc[1:4]) and list(rows[t, 6:9]) == list(Vc[1:4])
+
+
+@pytest.mark.parametrize("case", [((24, 16, 8), "weno5", "hll", 3), ((32, 24, 1), "para", "laxf", 2)])
+def test_current_columns_match_oracle(case):
+    cfg, og, g = _state(*case)
+    cf = og.current_slots()
+    calls, lastdt = 0, 0.0
+    for n in range(3):
+        if n == 0:
+            g.currents(at_start=True)
+            dt = g.time()[1]
+        else:
+            dt = g.time()[1]  # the dt the next step takes
+            g.step(1)
+            g.currents()
+        og.current_precalc(g.get("p"), cf)
+        J = og.current_calc(cf, calls >= 2, lastdt, dt)
+        calls, lastdt = calls + 1, dt
+        rows = g.dump_columns()
+        jc = rows[:, 52:56]
+        want = J.reshape(-1, 4)
+        for m in range(4):
+            s_ = np.abs(want[:, m]).max()
+            err = np.abs(jc[:, m] - want[:, m]).max() / (s_ if s_ > 0 else 1.0)
+            print(case, "call", n + 1, "J component", m, "scale", s_, "relative difference", err)
+            assert err <= 1e-12, (n, m, err)
+        # J_mu = g_{mu nu} J^nu at the zone centre
+        gc = og.geom[0][og.ng2:og.ng2 + cfg.n2, og.ng1:og.ng1 + cfg.n1, :10]
+        sym = [[0, 1, 2, 3], [1, 4, 5, 6], [2, 5, 7, 8], [3, 6, 8, 9]]
+        G = np.array([[gc[..., sym[a][b]] for b in range(4)] for a in range(4)])  # (4, 4, n2, n1)
+        Jz = J  # (n3, n2, n1, 4)
+        low = np.einsum("abji,kjib->kjia", G, Jz).reshape(-1, 4)
+        assert np.allclose(rows[:, 56:60], low, rtol=1e-12, atol=1e-12 * max(np.abs(low).max(), 1e-300))
+    assert np.abs(rows[:, 53:56]).max() > 0.0  # the time term and the spatial terms are there
 
 
 def test_write_dump_format(tmp_path):
