@@ -87,6 +87,10 @@ __global__ __launch_bounds__(kBlock) void k_emf(Dims d, int ni, int nj, int nk, 
 // The three directions' B fluxes from the corner EMFs (fluxct.c:1106-1161)
 // in one pass: thread (i, j, k) of the union of the three face boxes writes
 // face (i, j, k) of every direction whose box holds it.
+// The face fluxes' own field component (F1[B1], F2[B2], F3[B3]) is left as the
+// face kernel wrote it: with B^dir continuous across the face (BDIRCONT) its
+// induction flux B^dir v^dir - B^dir v^dir, its jump and so its Riemann flux
+// are exactly +0, the value flux_ct assigns (fluxct.c:1106-1161).
 __device__ __forceinline__ void ct_all_body(Dims d, int ni, int nj, int nk, const double* __restrict__ emf,
                                             double* __restrict__ F1, double* __restrict__ F2,
                                             double* __restrict__ F3, const int blk)
@@ -102,19 +106,16 @@ __device__ __forceinline__ void ct_all_body(Dims d, int ni, int nj, int nk, cons
   const long nc = d.ncell;
   const bool in_i = i < d.n1, in_j = j < d.n2, in_k = k < d.n3;
   if (e0 && in_j && in_k) {
-    F1[HARM_B1 * nc + o] = 0.;
     F1[HARM_B2 * nc + o] = 0.5 * (E(2, i, j, k) + E(2, i, j + e1, k));
     F1[HARM_B3 * nc + o] = -0.5 * (E(1, i, j, k) + E(1, i, j, k + e2));
   }
   if (e1 && in_i && in_k) {
     F2[HARM_B1 * nc + o] = -0.5 * (E(2, i, j, k) + E(2, i + e0, j, k));
-    F2[HARM_B2 * nc + o] = 0.;
     F2[HARM_B3 * nc + o] = 0.5 * (E(0, i, j, k) + E(0, i, j, k + e2));
   }
   if (e2 && in_i && in_j) {
     F3[HARM_B1 * nc + o] = 0.5 * (E(1, i, j, k) + E(1, i + e0, j, k));
     F3[HARM_B2 * nc + o] = -0.5 * (E(0, i, j, k) + E(0, i, j + e1, k));
-    F3[HARM_B3 * nc + o] = 0.;
   }
 }
 
