@@ -74,3 +74,52 @@ def test_config2_two_rk3_steps_match_oracle(setup):
     fg = np.array(g.boundary_fluxes())
     assert np.allclose(fg, fa, rtol=1e-12, atol=1e-300)
     g.close()
+
+
+def test_config2_1000_steps_checked_per_step(setup):
+    """The configuration's 1000 steps: at steps 1, 250, 500 and 1000 the
+    device's step is compared per zone with the oracle's step from the same
+    state (primitives and the next dt), and after the run Toth's corner
+    divergence of gdet B is at round-off and no inversion has failed.  (The
+    per-step comparison isolates the step: over 1000 steps two trajectories
+    part at the rate the flow amplifies the last-bit differences of the
+    radial boundary's exp and log, device against glibc.)"""
+    cfg, og, p = setup
+    g = Grid(cfg)
+    g.set("p", p)
+    dt0 = 0.5 * min(og.fluxcalc(p, og.zeros(), og.zeros(), og.zeros()))
+    g.set_time(0.0, dt0)
+    done = 0
+    for at in (0, 249, 499, 999):
+        if at > done:
+            g.step(at - done)
+            done = at
+        g.sync()
+        p0, (t0, dt) = g.get("p"), g.time()
+        g.step(1)
+        done += 1
+        st = og.new_state(p0)
+        dto = og.step(st, dt)
+        assert_zones_close(act(og, g.get("p")), act(og, st["p"]), f"step {done} primitives", positive=PRIM_POSITIVE)
+        assert abs(g.time()[1] - dto) <= 1e-12 * dto, (done, g.time()[1], dto)
+    assert done == 1000
+    (_, _), (_, nfail) = g.newton_stats()
+    pn = g.get("p")
+    gc = og.geom[0]
+    gd = gc[..., 21][None, :, :]
+    B = [pn[..., 5 + m] * gd for m in range(3)]
+    n1, n2, n3 = cfg.n1, cfg.n2, cfg.n3
+    sl = lambda a, di, dj, dk: a[og.ng3 + dk:og.ng3 + dk + n3 - 1, og.ng2 + dj:og.ng2 + dj + n2 - 1,
+                                 og.ng1 + di:og.ng1 + di + n1 - 1]
+    div = 0.0
+    for c, s in ((0, og.dx[1]), (1, og.dx[2]), (2, og.dx[3])):
+        accm = 0.0
+        for di in (0, 1):
+            for dj in (0, 1):
+                for dk in (0, 1):
+                    accm = accm + ([di, dj, dk][c] * 2 - 1) * sl(B[c], di, dj, dk)
+        div = div + accm / (4 * s)
+    scale = max(float(np.abs(sl(b, 0, 0, 0)).max()) for b in B) / min(og.dx[1], og.dx[2], og.dx[3])
+    print("after 1000 steps: t", g.time()[0], "failures", nfail, "corner div B / scale", np.abs(div).max() / scale)
+    assert np.abs(div).max() / scale < 1e-12
+    g.close()
