@@ -501,6 +501,8 @@ def main():
     ap.add_argument("--mlp-samples", type=int, default=1 << 20, help="large --mlp: zones sampled for training")
     ap.add_argument("--mlp-train-steps", type=int, default=3000, help="large --mlp: Adam steps")
     ap.add_argument("--no-chosen", action="store_true", help="step: skip the learned-guess (chosen form) timing")
+    ap.add_argument("--host-steps", type=int, default=3,
+                    help="step: steps timed with the primitives set from and read back to host memory around each")
     ap.add_argument("--mlp-samples-step", type=int, default=1 << 18, help="step: zones sampled for training")
     ap.add_argument("--mlp-train-steps-step", type=int, default=2000, help="step: Adam steps")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -577,6 +579,24 @@ def main():
             "U_rho": e1["U"][0], "fladd_rho": e1["fladd"][0], "divbmax": e1["divbmax"], "divbavg": e1["divbavg"]}
 
     zones = n1 * n2 * n3
+    # the same step for a caller whose state lives in host memory: the
+    # primitives go through harm_set_array before and harm_get_array after
+    # every step (HARM's own interleaved layout <-> the device's planes on the
+    # host, pageable copies over PCIe).  Reported beside the resident rate,
+    # never as the value.
+    host = None
+    if args.host_steps > 0:
+        ph = g.get("p")
+        t0 = time.perf_counter()
+        for _ in range(args.host_steps):
+            g.set("p", ph)
+            g.step(1)
+            ph = g.get("p")
+        el_h = time.perf_counter() - t0
+        host = {"steps": args.host_steps, "ms_per_step": el_h / args.host_steps * 1e3,
+                "zone_updates_per_s": zones * args.host_steps / el_h,
+                "bytes_each_way_per_step": int(ph.nbytes),
+                "note": "primitives (storage zones, 8 doubles each) set before and read after every step"}
     value = zones * args.steps / el
     # dominant kernel group by total event time
     dom = max((k for k in prof if prof[k][1] > 0), key=lambda k: prof[k][0])
@@ -645,6 +665,7 @@ def main():
         "step_roofline_hbm": step_roof,
         "kernels": kernels,
         "ener_tracking": ener,
+        "host_buffers": host,
         "chosen_form": chosen,
         "cpu_baseline": cpu,
     }

@@ -1055,6 +1055,7 @@ struct harm_handle {
   hipEvent_t evfork = nullptr, evjoin[2] = {nullptr, nullptr};
   double* cf = nullptr;    // the dump's four-current: Faraday samples (18 x ncell) and J^mu (4 x ncell),
   double* jcon = nullptr;  // allocated by the first harm_currents
+  double* xfer = nullptr;  // HARM's interleaved layout of one array (NPR x storage cells), staged for set/get
   long cur_calls = 0;      // current_calc_1's call count and the dt of its last call
   double cur_lastdt = 0.0;
   double* wdp = nullptr;   // WENO5BND: dP measure per direction (3 x ncell)
@@ -2043,7 +2044,7 @@ extern "C" harm_status harm_destroy(harm_handle* h)
   }
   for (hipEvent_t e : h->evpool) (void)hipEventDestroy(e);
   void* bufs[] = {h->pflag, h->pfix, h->gcorn3, h->conn, h->floors, h->rcent, h->emf, h->wl, h->wr, h->wdp, h->wtm, h->facc,
-                  h->cf, h->jcon,
+                  h->cf, h->jcon, h->xfer,
                   h->en, h->enpart, h->entot, h->fx_p0, h->fx_pfl, h->fx_flag, h->ds, h->di};
   for (void* b : bufs) (void)hipFree(b);
   for (int q = 0; q < 2; q++) {
@@ -2118,19 +2119,50 @@ static inline size_t dev_cell(const Dims& d, long c)
   return (size_t)row * d.ps1 + (size_t)(c - row * d.ns1);
 }
 
+namespace {
+
+// HARM's interleaved prim/cons layout (a[c][pl], c = (kk*ns2 + jj)*ns1 + ii
+// over the storage box) <-> the device planes (a[pl][row*ps1 + ii]).  One
+// thread per device cell: the planes are read or written coalesced, the
+// interleaved side as 64 contiguous bytes per thread.  Row padding
+// (ii >= ns1) is written as zero on the way in.
+template <bool TO_PLANES>
+__global__ __launch_bounds__(kBlock) void k_xfer(Dims d, const double* __restrict__ src, double* __restrict__ dst)
+{
+  const long o = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (o >= d.ncell) return;
+  const long row = o / d.ps1, ii = o - row * d.ps1;
+  const bool in = row < (long)d.ns2 * d.ns3 && ii < d.ns1;
+  const long c = row * d.ns1 + ii;
+  if constexpr (TO_PLANES) {
+    for (int pl = 0; pl < HARM_NPR; pl++) dst[(long)pl * d.ncell + o] = in ? src[c * HARM_NPR + pl] : 0.0;
+  } else {
+    if (!in) return;
+    for (int pl = 0; pl < HARM_NPR; pl++) dst[c * HARM_NPR + pl] = src[(long)pl * d.ncell + o];
+  }
+}
+
+harm_status xfer_buffer(harm_handle* h)
+{
+  if (!h->xfer) HH_TRY(alloc((void**)&h->xfer, sizeof(double) * HARM_NPR * (size_t)h->d.ns1 * h->d.ns2 * h->d.ns3));
+  return HARM_OK;
+}
+
+}  // namespace
+
 extern "C" harm_status harm_set_array(harm_handle* h, int32_t which, const double* host_aos)
 {
   HH_TRY(check_handle(h));
   HH_TRY(check_which(which));
   const Dims& d = h->d;
-  const long nl = (long)d.ns1 * d.ns2 * d.ns3, nc = d.ncell;
-  std::vector<double> soa((size_t)HARM_NPR * nc, 0.0);
-  for (long c = 0; c < nl; c++) {
-    const size_t o = dev_cell(d, c);
-    for (int pl = 0; pl < HARM_NPR; pl++) soa[(size_t)pl * nc + o] = host_aos[c * HARM_NPR + pl];
-  }
+  const size_t n = (size_t)HARM_NPR * d.ns1 * d.ns2 * d.ns3;
+  HH_TRY(xfer_buffer(h));
+  HH_HIP(hipMemcpyAsync(h->xfer, host_aos, sizeof(double) * n, hipMemcpyHostToDevice, h->stream));
+  hipLaunchKernelGGL(k_xfer<true>, dim3((unsigned)((d.ncell + kBlock - 1) / kBlock)), dim3(kBlock), 0, h->stream, d,
+                     (const double*)h->xfer, h->arr[which]);
+  HH_HIP(hipGetLastError());
   HH_HIP(hipStreamSynchronize(h->stream));
-  return upload(h, h->arr[which], soa.data(), soa.size());
+  return HARM_OK;
 }
 
 extern "C" harm_status harm_get_array(harm_handle* h, int32_t which, double* host_aos)
@@ -2138,14 +2170,13 @@ extern "C" harm_status harm_get_array(harm_handle* h, int32_t which, double* hos
   HH_TRY(check_handle(h));
   HH_TRY(check_which(which));
   const Dims& d = h->d;
-  const long nl = (long)d.ns1 * d.ns2 * d.ns3, nc = d.ncell;
-  std::vector<double> soa((size_t)HARM_NPR * nc);
+  const size_t n = (size_t)HARM_NPR * d.ns1 * d.ns2 * d.ns3;
+  HH_TRY(xfer_buffer(h));
+  hipLaunchKernelGGL(k_xfer<false>, dim3((unsigned)((d.ncell + kBlock - 1) / kBlock)), dim3(kBlock), 0, h->stream, d,
+                     (const double*)h->arr[which], h->xfer);
+  HH_HIP(hipGetLastError());
+  HH_HIP(hipMemcpyAsync(host_aos, h->xfer, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
   HH_HIP(hipStreamSynchronize(h->stream));
-  HH_HIP(hipMemcpy(soa.data(), h->arr[which], sizeof(double) * soa.size(), hipMemcpyDeviceToHost));
-  for (long c = 0; c < nl; c++) {
-    const size_t o = dev_cell(d, c);
-    for (int pl = 0; pl < HARM_NPR; pl++) host_aos[c * HARM_NPR + pl] = soa[(size_t)pl * nc + o];
-  }
   return HARM_OK;
 }
 

@@ -172,6 +172,9 @@ enum { HARM_ARR_P = 0,     /* pglobal (primitives)                           */
        HARM_ARR_UCUM = 5,  /* unewglobal                                     */
        HARM_ARR_F1 = 6, HARM_ARR_F2 = 7, HARM_ARR_F3 = 8, /* face fluxes     */
        HARM_NARR = 9 };
+/* Both copy the whole storage box over PCIe and convert between this layout
+ * and the device planes with a kernel (a staging buffer of the box's size is
+ * allocated by the first call); both return when the copy is complete. */
 harm_status harm_set_array(harm_handle* h, int32_t which, const double* host_aos);
 harm_status harm_get_array(harm_handle* h, int32_t which, double* host_aos);
 /* pflag[k][j][i] (FLAGUTOPRIMFAIL) over the storage box */
